@@ -1,0 +1,158 @@
+"""Benchmark of the collision-and-contact step on one MI355X.
+
+Workload (default): narrowphase stress — 262,144 convex hulls of 8-32 points on
+ellipsoids with semi-axes U[0.05,0.3] m, centres from Gaussian clusters of 512 bodies
+in a 200 m cube (~4 candidate pairs per body); one step = LBVH rebuild + pair query
++ GJK/EPA + manifold update, every leaf moved, no solver or integration. FP64.
+
+Prints one JSON line. value = body-steps/s (bodies x timed steps / wall time of the
+timed steps, inputs resident in HBM); contact-pairs/s rides along in the config.
+The roofline is taken for the dominant kernel from HIP events recorded on the
+library's own stream around that kernel during the timed steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "body-steps/s + contact-pairs/s, batched FP32 vs plain FP64, 1 MI355X"
+
+WORKLOADS = {
+    "hulls": (1, "narrowphase stress: 262,144 convex hulls (8-32 points on ellipsoids, "
+                 "semi-axes U[0.05,0.3] m) in 512-body Gaussian clusters in a 200 m cube; "
+                 "LBVH rebuild + pair query + GJK/EPA + manifolds per step, FP64"),
+    "boxes": (0, "1,024 random boxes over a static ground box, dt=1/60, 10 PGS iterations, FP64"),
+    "pyramids": (2, "64 pyramids of 15-layer unit-box stacks (7,680 boxes), dt=1/120, "
+                    "20 PGS iterations, FP64"),
+}
+
+
+def algorithmic_bytes(stage, s, pairs, hits_frac):
+    """Compulsory HBM bytes of one launch of the narrowphase kernels (FP64).
+    Per pair, GJK reads the key (8 B), each body's pose (pos 24 + quat 32), shape id and
+    vertex offsets (12) and every vertex of both shapes once (24 B each), and writes a
+    status byte; a hit also writes its hit-list entry (4) and simplex (37 x 8).
+    EPA re-reads, per hit, the hit-list entry, key, poses, offsets, vertices and
+    simplex, and writes 10 doubles of contact data and a status byte."""
+    off = s["shape_offsets"].astype(np.int64)
+    nv = (off[1:] - off[:-1])[s["shape"]]
+    a = (pairs >> np.uint64(32)).astype(np.int64)
+    b = (pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    verts = 24.0 * (nv[a] + nv[b]).astype(np.float64)
+    per_pair_fixed = 8 + 2 * (24 + 32 + 12)
+    if stage == "gjk":
+        return float(np.sum(per_pair_fixed + verts + 1) + hits_frac * len(pairs) * (4 + 37 * 8))
+    if stage == "epa":
+        return float(hits_frac * np.sum(4 + per_pair_fixed + verts + 37 * 8 + 80 + 1))
+    raise ValueError(stage)
+
+
+def cpu_baseline(s, budget_s=12.0):
+    """The oracle (FP64 CPU restatement, one thread) on the same workload, stepped until
+    about budget_s seconds have elapsed (at least one step)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+
+    o = oracle_ffi.OracleWorld(iterations=s["iterations"], solver_mode=1, flags=s["flags"])
+    o.load_scene(s)
+    steps, t = 0, 0.0
+    while t < budget_s and steps < 200:
+        t += o.step(s["dt"], 1)
+        steps += 1
+    n = len(s["shape"]) - int(s["n_static"])
+    return {"value": n * steps / t, "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the full workload ({n} dynamic bodies), "
+                      f"{t:.1f} s on one host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(each GPU would run its own worlds; see DESIGN.md on sharding)", file=sys.stderr)
+        sys.exit(2)
+
+    import physkit_amd as pk
+
+    kind, desc = WORKLOADS[args.workload]
+    s = pk.scene(kind, None, args.n)
+    g = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
+    g.load_scene(s)
+    dt = s["dt"]
+    n_dyn = len(s["shape"]) - int(s["n_static"])
+    print(f"[bench] workload={args.workload} bodies={len(s['shape'])} steps={args.steps} "
+          f"warmup={args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        g.step(dt)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    pairs_total = 0
+    hits_total = 0
+    stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms = g.step_timed(dt, 1)
+        for k, v in ms.items():
+            stage[k] += v
+        c = g.counters()
+        pairs_total += c["pairs"]
+        hits_total += c["gjk_hits"]
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    c = g.counters()
+    pairs = g.pairs()
+    hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
+
+    # dominant kernel of the narrowphase and its roofline
+    dom = max(("gjk", "epa"), key=lambda k: stage[k])
+    t_launch = stage[dom] / 1e3 / args.steps
+    alg = algorithmic_bytes(dom, s, pairs, hits_frac) if len(pairs) else 0.0
+    achieved = alg / t_launch / 1e9 if t_launch > 0 else 0.0
+    roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None, "algorithmic_bytes_per_launch": alg,
+            "avg_launch_ms": round(t_launch * 1e3, 4)}
+    out = {
+        "metric": METRIC,
+        "value": round(n_dyn * args.steps / wall, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc, "bodies": len(s["shape"]), "dynamic_bodies": n_dyn,
+                   "pairs_per_step": c["pairs"], "gjk_hits_per_step": c["gjk_hits"],
+                   "contact_pairs_per_s": round(pairs_total / wall, 1),
+                   "colliding_pairs_per_s": round(hits_total / wall, 1),
+                   "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stage.items()},
+                   "parallelism": "1 GPU, one world"},
+        "roofline": roof,
+    }
+    g.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(s)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/* physkit_amd: MI355X (gfx950) implementation of PhysKit's collision-and-contact world
+ * step behind a C ABI. Plain pointers and sizes only; no exceptions cross this
+ * boundary. Every entry point returns a pkg_status; pkg_last_error() gives the text.
+ *
+ * The plug point this library replaces is PhysKit's
+ *   virtual void world_base::step_impl(quantity<si::second> dt)
+ * (include/physkit/core/world.h:340, implemented by world::step_impl in
+ * src/world.cpp:20-56). A world hands its bodies to a pkg_ctx once, then calls
+ * pkg_step() where it would call step_impl(), and reads poses, pairs, manifolds
+ * and begin/end events back.
+ *
+ * Body ids are the reference's arena slot indices (include/physkit/detail/arena.h:
+ * 64-80, used as broadphase object ids at world.h:205-207), so pair keys are the
+ * reference's keys: (min_id << 32) | max_id (collision_phases.h:63-69).
+ *
+ * Threading: one pkg_ctx per host thread; calls on one ctx are not re-entrant. All
+ * device work of a ctx is ordered on one HIP stream owned by the ctx.
+ */
+#ifndef PHYSKIT_AMD_H
+#define PHYSKIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pkg_status
+{
+    PKG_OK = 0,
+    PKG_INVALID_ARG = 1,
+    PKG_OOM = 2,
+    PKG_OVERFLOW_PAIRS = 3,
+    PKG_OVERFLOW_EPA = 4,
+    PKG_DEVICE_ERROR = 5,
+    PKG_NO_DEVICE = 6
+} pkg_status;
+
+typedef struct pkg_ctx pkg_ctx;
+
+/* pkg_config.flags */
+#define PKG_FLAG_ALL_MOVED 1u    /* every dynamic leaf counts as moved each step       */
+#define PKG_FLAG_NO_SOLVER 2u    /* skip contact setup and solve                        */
+#define PKG_FLAG_NO_INTEGRATE 4u /* skip velocity/position integration and leaf update */
+
+/* Mirrors world_desc (include/physkit/core/world.h:23-49): gravity and solver
+ * iterations; defaults there are (0,-9.81,0) and 10. */
+typedef struct pkg_config
+{
+    double gravity[3];
+    uint32_t solver_iterations;
+    uint32_t flags;
+} pkg_config;
+
+/* Bodies, structure of arrays, n entries each; quaternions are (w, x, y, z);
+ * inertia is the local 3x3 tensor row-major (object_desc, include/physkit/core/
+ * object.h:15-109). A static body (is_static=1) has infinite mass, as
+ * object_desc::mass() returns for body_type::stat (object.h:81-85); its mass entry
+ * is ignored. */
+typedef struct pkg_body_desc
+{
+    const uint8_t *is_static;
+    const uint32_t *shape;
+    const double *pos;         /* 3n */
+    const double *vel;         /* 3n */
+    const double *angvel;      /* 3n */
+    const double *quat;        /* 4n */
+    const double *mass;        /* n  */
+    const double *inertia;     /* 9n */
+    const double *restitution; /* n  */
+    const double *friction;    /* n  */
+} pkg_body_desc;
+
+/* Per-body state written by pkg_download_bodies (any pointer may be NULL). */
+typedef struct pkg_body_state
+{
+    double *pos;    /* 3n */
+    double *vel;    /* 3n */
+    double *angvel; /* 3n */
+    double *quat;   /* 4n */
+    double *fat;    /* 6n: broadphase leaf bounds min xyz, max xyz */
+    uint8_t *moved; /* n: leaf reinserted in the last step */
+} pkg_body_state;
+
+/* Counters of the last step. */
+typedef struct pkg_counters
+{
+    uint64_t moved;    /* leaves reinserted (dynamic_bvh::update_leaf returned true) */
+    uint64_t pairs;    /* active broadphase pairs = pairs through GJK               */
+    uint64_t gjk_hits; /* pairs whose GJK found an intersection                    */
+    uint64_t contacts; /* manifold contact points after the narrowphase            */
+    uint64_t rows;     /* contact solver points (penetration > 0)                   */
+    uint64_t colours;  /* solver colours                                            */
+    uint64_t epa_overflow; /* pairs whose EPA needed the large-polytope path         */
+    uint64_t candidates;   /* candidate pairs emitted by the tree query              */
+} pkg_counters;
+
+/* Replaces constructing a physkit::world (world.h:382-385). */
+pkg_status pkg_create(pkg_ctx **out, const pkg_config *cfg);
+pkg_status pkg_destroy(pkg_ctx *ctx);
+const char *pkg_last_error(const pkg_ctx *ctx);
+
+/* Convex meshes (mesh::make / box / sphere / pyramid, src/mesh.cpp:16-137):
+ * n_shapes shapes, shape s owning vertices [offsets[s], offsets[s+1]) of verts (xyz).
+ * The local AABB is aabb::from_points over the vertices (mesh.h:191). */
+pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *offsets,
+                             const double *verts);
+
+/* Replaces world_base::create_rigid for n bodies with ids 0..n-1 (world.h:202-208);
+ * resets pairs and manifolds. Inverse mass/inertia and the world inertia follow
+ * particle::particle (include/physkit/core/particle.h:16-28, 140-146); each leaf
+ * starts with its exact world bounds, as dynamic_bvh::add stores them (bvh.h:294-302). */
+pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *bodies);
+
+/* Overwrites kinematic state of existing bodies (a caller moving objects between
+ * steps). ids: count entries; arrays as in pkg_body_desc (pos/vel/angvel/quat). */
+pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids,
+                              const double *pos, const double *vel, const double *angvel,
+                              const double *quat);
+
+/* world::step_impl(dt) (src/world.cpp:20-56), nsteps times. Synchronous: returns
+ * when the device results of the last step are final. */
+pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps);
+
+pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out);
+
+/* Sorted active pair keys (pair_manager::active_pairs, collision_phases.h:54). */
+pkg_status pkg_get_pairs(pkg_ctx *ctx, uint64_t *keys, size_t cap, size_t *n);
+
+/* Manifolds in pair-key order (narrow_phase::manifolds, collision_phases.h:322):
+ * counts[p] contacts for pair p and data[(p*4+c)*13 + f] with f = normal xyz,
+ * local_a xyz, local_b xyz, depth, normal impulse, tangent impulses (2). */
+pkg_status pkg_get_manifolds(pkg_ctx *ctx, int32_t *counts, double *data, size_t cap,
+                             size_t *n);
+
+/* Begin (1) / end (2) events of the last step in pair-key order: what
+ * narrow_phase::calculate reports through on_coll_beg / on_coll_end
+ * (collision_phases.h:314-318, world.h:327-338). */
+pkg_status pkg_get_events(pkg_ctx *ctx, uint64_t *keys, int32_t *kinds, size_t cap,
+                          size_t *n);
+
+pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out);
+
+/* Colours of the solver manifolds of the last step (manifolds with at least one
+ * contact row, in pair-key order). */
+pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, size_t cap, size_t *n);
+
+/* pkg_step with per-stage device time, measured with HIP events on the ctx stream.
+ * Stages: 0 integrate_vel + leaf update, 1 LBVH build, 2 pair-set update, 3 GJK
+ * kernel, 4 EPA kernel(s), 5 manifold update kernel, 6 contact solver,
+ * 7 integrate_pos. stage_ms[i] is the sum over the nsteps steps (up to 8 entries). */
+pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms, int n_stages);
+
+/* Standalone narrowphase: gjk_epa(a, b) (src/collision.cpp:512-518) over n shape
+ * pairs. Shape i of side a: kind_a[i] (0 aabb, 1 obb, 2 mesh instance), params
+ * 10 doubles at prm_a[10*i] (aabb: min3 max3; obb: center3 quat4 half3; mesh:
+ * pos3 quat4) and, for meshes, the vertex range [vbeg_a[i], vend_a[i]) of verts.
+ * out[10*i]: normal3, world_a3, world_b3, depth; hit[i]: 0 separated, 1 contact,
+ * 2 GJK hit with a degenerate EPA simplex (no contact). All pointers are host
+ * pointers. */
+pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a,
+                             const double *prm_a, const uint32_t *vbeg_a, const uint32_t *vend_a,
+                             const int32_t *kind_b, const double *prm_b, const uint32_t *vbeg_b,
+                             const uint32_t *vend_b, const double *verts, uint32_t n_verts,
+                             double *out, int32_t *hit);
+
+/* ---- synthetic scenes (input generation for tests and benchmarks) ---- */
+typedef struct pkg_scene pkg_scene;
+typedef struct pkg_scene_info
+{
+    uint32_t n_shapes;
+    uint32_t n_verts;
+    uint32_t n_bodies;
+    uint32_t n_static;
+    double dt;
+    uint32_t iterations;
+    uint32_t steps;
+    uint32_t flags; /* pkg_config.flags the scene is meant to run with */
+} pkg_scene_info;
+
+/* kind 0: 1,024 random boxes over a ground box; kind 1: convex-hull clusters
+ * (narrowphase stress); kind 2: box pyramids (solver stress). n_override != 0
+ * changes the body count (kind 0, 1) or pyramid count (kind 2). */
+pkg_status pkg_scene_create(pkg_scene **out, int kind, uint64_t seed, uint32_t n_override);
+pkg_status pkg_scene_info_get(const pkg_scene *s, pkg_scene_info *info);
+/* Copies the scene into caller buffers sized from pkg_scene_info. */
+pkg_status pkg_scene_copy(const pkg_scene *s, uint32_t *shape_offsets, double *verts,
+                          uint8_t *is_static, uint32_t *shape, double *pos, double *vel,
+                          double *angvel, double *quat, double *mass, double *inertia,
+                          double *restitution, double *friction);
+pkg_status pkg_scene_destroy(pkg_scene *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSKIT_AMD_H */

@@ -1,0 +1,311 @@
+// Oracle (test infrastructure only): C ABI over the FP64 restatement so pytest can
+// drive it through ctypes. Never linked into the product.
+#include "pko.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+using namespace pko;
+
+namespace
+{
+v3 ld3(const double *p) { return {p[0], p[1], p[2]}; }
+void st3(double *p, v3 v)
+{
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+shape_ref make_shape(int kind, const double *prm, const double *verts, int nv, mesh &storage)
+{
+    shape_ref s{};
+    s.kind = kind;
+    if (kind == 0)
+        s.box = {ld3(prm), ld3(prm + 3)};
+    else if (kind == 1)
+        s.obb = {ld3(prm), qt{prm[3], prm[4], prm[5], prm[6]}, ld3(prm + 7)};
+    else
+    {
+        std::vector<v3> vv(static_cast<std::size_t>(nv));
+        for (int i = 0; i < nv; ++i) vv[static_cast<std::size_t>(i)] = ld3(verts + 3 * i);
+        storage = make_mesh(vv.data(), vv.size());
+        s.inst = {&storage, ld3(prm), qt{prm[3], prm[4], prm[5], prm[6]}};
+    }
+    return s;
+}
+} // namespace
+
+extern "C"
+{
+
+// flags: 1 = every dynamic leaf counts as moved, 2 = skip the contact solver,
+//        4 = skip velocity/position integration
+void *pko_world_new(const double *gravity, int iterations, int solver_mode, int flags)
+{
+    auto *w = new world();
+    w->gravity = ld3(gravity);
+    w->iterations = iterations;
+    w->solver_mode = solver_mode;
+    w->all_moved = (flags & 1) != 0;
+    w->run_solver = (flags & 2) == 0;
+    w->run_integrate = (flags & 4) == 0;
+    w->finalize_world();
+    return w;
+}
+
+void pko_world_free(void *h) { delete static_cast<world *>(h); }
+
+int pko_add_shape(void *h, const double *verts, int n)
+{
+    auto *w = static_cast<world *>(h);
+    std::vector<v3> vv(static_cast<std::size_t>(n));
+    for (int i = 0; i < n; ++i) vv[static_cast<std::size_t>(i)] = ld3(verts + 3 * i);
+    w->shapes.push_back(make_mesh(vv.data(), vv.size()));
+    return static_cast<int>(w->shapes.size() - 1);
+}
+
+int pko_add_bodies(void *h, int n, const uint8_t *is_static, const uint32_t *shape,
+                   const double *pos, const double *vel, const double *angvel, const double *quat,
+                   const double *mass, const double *inertia, const double *restitution,
+                   const double *friction)
+{
+    auto *w = static_cast<world *>(h);
+    for (int i = 0; i < n; ++i)
+    {
+        body b;
+        b.is_static = is_static[i] != 0;
+        b.shape = shape[i];
+        if (b.shape >= w->shapes.size()) return -1;
+        b.pos = ld3(pos + 3 * i);
+        b.vel = ld3(vel + 3 * i);
+        b.angvel = ld3(angvel + 3 * i);
+        b.q = qt{quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+        b.mass = b.is_static ? __builtin_inf() : mass[i]; // object.h:81-85
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) b.I_local.m[r][c] = inertia[9 * i + 3 * r + c];
+        b.restitution = restitution[i];
+        b.friction = friction[i];
+        w->add_body(b);
+    }
+    return static_cast<int>(w->bodies.size());
+}
+
+double pko_step(void *h, double dt, int nsteps)
+{
+    auto *w = static_cast<world *>(h);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nsteps; ++i) w->step(dt);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+int pko_num_bodies(void *h) { return static_cast<int>(static_cast<world *>(h)->bodies.size()); }
+
+void pko_get_bodies(void *h, double *pos, double *vel, double *angvel, double *quat, double *fat,
+                    uint8_t *moved)
+{
+    auto *w = static_cast<world *>(h);
+    for (std::size_t i = 0; i < w->bodies.size(); ++i)
+    {
+        const body &b = w->bodies[i];
+        st3(pos + 3 * i, b.pos);
+        st3(vel + 3 * i, b.vel);
+        st3(angvel + 3 * i, b.angvel);
+        quat[4 * i] = b.q.w;
+        quat[4 * i + 1] = b.q.x;
+        quat[4 * i + 2] = b.q.y;
+        quat[4 * i + 3] = b.q.z;
+        st3(fat + 6 * i, b.fat.min);
+        st3(fat + 6 * i + 3, b.fat.max);
+        moved[i] = b.moved ? 1 : 0;
+    }
+}
+
+// world inertia and its inverse, 9 row-major doubles per body
+void pko_get_derived(void *h, double *iw, double *iw_inv, double *inv_mass)
+{
+    auto *w = static_cast<world *>(h);
+    for (std::size_t i = 0; i < w->bodies.size(); ++i)
+    {
+        const body &b = w->bodies[i];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+            {
+                iw[9 * i + 3 * r + c] = b.Iw.m[r][c];
+                iw_inv[9 * i + 3 * r + c] = b.Iw_inv.m[r][c];
+            }
+        inv_mass[i] = b.inv_mass;
+    }
+}
+
+size_t pko_num_pairs(void *h) { return static_cast<world *>(h)->active.size(); }
+
+void pko_get_pairs(void *h, uint64_t *keys)
+{
+    auto *w = static_cast<world *>(h);
+    std::copy(w->active.begin(), w->active.end(), keys);
+}
+
+// per pair: count, then 4 contacts x 13 doubles
+// (normal3, local_a3, local_b3, depth, normal impulse, tangent impulses 2)
+void pko_get_manifolds(void *h, int32_t *counts, double *data)
+{
+    auto *w = static_cast<world *>(h);
+    for (std::size_t i = 0; i < w->manifolds.size(); ++i)
+    {
+        const manifold &m = w->manifolds[i];
+        counts[i] = m.n;
+        for (int c = 0; c < 4; ++c)
+        {
+            double *d = data + (i * 4 + static_cast<std::size_t>(c)) * 13;
+            const contact &k = m.c[static_cast<std::size_t>(c)];
+            if (c >= m.n)
+            {
+                std::memset(d, 0, 13 * sizeof(double));
+                continue;
+            }
+            st3(d, k.normal);
+            st3(d + 3, k.local_a);
+            st3(d + 6, k.local_b);
+            d[9] = k.depth;
+            d[10] = k.ln;
+            d[11] = k.lt1;
+            d[12] = k.lt2;
+        }
+    }
+}
+
+size_t pko_get_events(void *h, uint64_t *keys, int32_t *kinds, size_t cap)
+{
+    auto *w = static_cast<world *>(h);
+    size_t n = std::min(cap, w->events.size());
+    for (size_t i = 0; i < n; ++i)
+    {
+        keys[i] = w->events[i].key;
+        kinds[i] = w->events[i].kind;
+    }
+    return w->events.size();
+}
+
+// moved, pairs, gjk_hits, contacts, rows, colours
+void pko_get_counters(void *h, uint64_t *out)
+{
+    auto *w = static_cast<world *>(h);
+    out[0] = w->counters.moved;
+    out[1] = w->counters.pairs;
+    out[2] = w->counters.gjk_hits;
+    out[3] = w->counters.contacts;
+    out[4] = w->counters.rows;
+    out[5] = w->counters.colours;
+}
+
+// colours of the manifolds that produced solver rows in the last step
+size_t pko_get_colours(void *h, uint32_t *out, size_t cap)
+{
+    auto *w = static_cast<world *>(h);
+    size_t n = std::min(cap, w->colours.size());
+    std::copy(w->colours.begin(), w->colours.begin() + static_cast<std::ptrdiff_t>(n), out);
+    return w->colours.size();
+}
+
+// Standalone narrowphase (collision.cpp:512-518). kind: 0 aabb (min3,max3),
+// 1 obb (center3, quat wxyz, half3), 2 mesh instance (pos3, quat wxyz) over verts.
+// out: normal3, world_a3, world_b3, depth. stats: gjk iters, epa iters, polytope, faces.
+int pko_gjk_epa(int kind_a, const double *pa, const double *verts_a, int nva, int kind_b,
+                const double *pb, const double *verts_b, int nvb, double *out, int *stats)
+{
+    mesh ma, mb;
+    shape_ref a = make_shape(kind_a, pa, verts_a, nva, ma);
+    shape_ref b = make_shape(kind_b, pb, verts_b, nvb, mb);
+    gjk_stats st;
+    auto r = gjk_epa(a, b, &st);
+    if (stats)
+    {
+        stats[0] = st.gjk_iters;
+        stats[1] = st.epa_iters;
+        stats[2] = st.polytope;
+        stats[3] = st.faces;
+    }
+    if (!r) return st.hit ? 2 : 0; // 2: GJK hit but EPA found a degenerate simplex
+    st3(out, r->normal);
+    st3(out + 3, r->world_a);
+    st3(out + 6, r->world_b);
+    out[9] = r->depth;
+    return 1;
+}
+
+// mesh generators (mesh.cpp:25-137): writes vertices, returns the vertex count
+int pko_mesh_box(const double *half, double *out)
+{
+    mesh m = mesh_box(ld3(half));
+    for (std::size_t i = 0; i < m.verts.size(); ++i) st3(out + 3 * i, m.verts[i]);
+    return static_cast<int>(m.verts.size());
+}
+int pko_mesh_sphere(double r, int stacks, int sectors, double *out)
+{
+    mesh m = mesh_sphere(r, static_cast<unsigned>(stacks), static_cast<unsigned>(sectors));
+    for (std::size_t i = 0; i < m.verts.size(); ++i) st3(out + 3 * i, m.verts[i]);
+    return static_cast<int>(m.verts.size());
+}
+int pko_mesh_pyramid(double bh, double h, double *out)
+{
+    mesh m = mesh_pyramid(bh, h);
+    for (std::size_t i = 0; i < m.verts.size(); ++i) st3(out + 3 * i, m.verts[i]);
+    return static_cast<int>(m.verts.size());
+}
+
+// world AABB of a mesh instance (mesh.cpp:398)
+void pko_instance_bounds(const double *verts, int nv, const double *pos, const double *quat,
+                         double *out)
+{
+    std::vector<v3> vv(static_cast<std::size_t>(nv));
+    for (int i = 0; i < nv; ++i) vv[static_cast<std::size_t>(i)] = ld3(verts + 3 * i);
+    mesh m = make_mesh(vv.data(), vv.size());
+    mesh_instance inst{&m, ld3(pos), qt{quat[0], quat[1], quat[2], quat[3]}};
+    aabb b = inst.bounds();
+    st3(out, b.min);
+    st3(out + 3, b.max);
+}
+
+void pko_instance_support(const double *verts, int nv, const double *pos, const double *quat,
+                          const double *dir, double *out)
+{
+    std::vector<v3> vv(static_cast<std::size_t>(nv));
+    for (int i = 0; i < nv; ++i) vv[static_cast<std::size_t>(i)] = ld3(verts + 3 * i);
+    mesh m = make_mesh(vv.data(), vv.size());
+    mesh_instance inst{&m, ld3(pos), qt{quat[0], quat[1], quat[2], quat[3]}};
+    st3(out, inst.support(ld3(dir)));
+}
+
+void pko_obb_support(const double *prm, const double *dir, double *out)
+{
+    obb_shape o{ld3(prm), qt{prm[3], prm[4], prm[5], prm[6]}, ld3(prm + 7)};
+    st3(out, o.support(ld3(dir)));
+}
+
+void pko_sincos(const double *x, double *s, double *c, int n)
+{
+    for (int i = 0; i < n; ++i) sincos_pk(x[i], s + i, c + i);
+}
+
+void pko_exp_map(const double *w, double dt, double *q)
+{
+    qt r = exp_map(ld3(w), dt);
+    q[0] = r.w;
+    q[1] = r.x;
+    q[2] = r.y;
+    q[3] = r.z;
+}
+
+void pko_inverse3(const double *m, double *out)
+{
+    m3 a;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a.m[r][c] = m[3 * r + c];
+    m3 b = inverse(a);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[3 * r + c] = b.m[r][c];
+}
+
+} // extern "C"

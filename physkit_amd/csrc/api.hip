@@ -1,0 +1,855 @@
+// C ABI and host orchestration of one world step on one HIP stream.
+// Stage order is world::step_impl (src/world.cpp:20-56):
+//   integrate_vel + leaf update -> pairs -> narrowphase -> contact setup/solve ->
+//   integrate_pos.
+// Host synchronisation points (counts that size the next launch): candidate count,
+// pair count, GJK hit count, EPA overflow count, solver rows, colour rounds.
+#include "physkit_amd.h"
+#include "pk_access.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace
+{
+using namespace pk;
+
+struct pk_error : std::runtime_error
+{
+    pkg_status st;
+    pk_error(pkg_status s, const std::string &m) : std::runtime_error(m), st(s) {}
+};
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e == hipErrorOutOfMemory) throw pk_error(PKG_OOM, std::string(what) + ": out of device memory");
+    if (e != hipSuccess) throw pk_error(PKG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T> struct dbuf
+{
+    T *p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) // contents are not kept
+    {
+        if (n <= cap) return;
+        release();
+        size_t want = std::max<size_t>(n, 1);
+        hip_check(hipMalloc(reinterpret_cast<void **>(&p), want * sizeof(T)), "hipMalloc");
+        cap = want;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+__global__ void k_set_state(bodies_dev b, uint32_t count, const uint32_t *ids, const double *pos,
+                            const double *vel, const double *ang, const double *quat)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint32_t i = ids[t];
+    if (pos) st3(b.pos, b.n, i, mk(pos[3 * t], pos[3 * t + 1], pos[3 * t + 2]));
+    if (vel) st3(b.vel, b.n, i, mk(vel[3 * t], vel[3 * t + 1], vel[3 * t + 2]));
+    if (ang) st3(b.ang, b.n, i, mk(ang[3 * t], ang[3 * t + 1], ang[3 * t + 2]));
+    if (quat) // particle::orientation(q) refreshes the derived state, particle.h:40-44
+    {
+        qt q = mkq(quat[4 * t], quat[4 * t + 1], quat[4 * t + 2], quat[4 * t + 3]);
+        stq(b.quat, b.n, i, q);
+        m3 R = rotation_matrix(q);
+        m3 Rt = transpose(R);
+        st9(b.Iw, b.n, i, mul(mul(R, ld9(b.I_local, b.n, i)), Rt));
+        st9(b.Iw_inv, b.n, i, mul(mul(R, ld9(b.Iinv_local, b.n, i)), Rt));
+    }
+}
+
+int id_bits(uint32_t n)
+{
+    int b = 0;
+    while (b < 32 && (static_cast<uint64_t>(1) << b) < n) ++b;
+    return b < 1 ? 1 : b;
+}
+} // namespace
+
+struct pkg_ctx
+{
+    hipStream_t st = nullptr;
+    pkg_config cfg{};
+    double gnorm = 9.81;
+    std::string err;
+
+    uint32_t n_shapes = 0, n_verts = 0;
+    dbuf<uint32_t> shape_off;
+    dbuf<double> verts, shape_box;
+
+    uint32_t n = 0;
+    dbuf<double> pos, vel, ang, acc, torque, quat, mass, inv_mass, I_local, Iinv_local, Iw, Iw_inv, rest,
+        fric, fat;
+    dbuf<uint32_t> shape;
+    dbuf<uint8_t> is_static, moved;
+
+    dbuf<unsigned long long> counters;
+    unsigned long long *h_counters = nullptr;
+
+    // broadphase
+    dbuf<uint64_t> morton, morton_alt;
+    dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, visit;
+    dbuf<int32_t> left, right;
+    dbuf<double> node_box;
+    dbuf<unsigned long long> scene;
+    sort_scratch sc;
+    dbuf<uint64_t> comb_keys, comb_keys_alt;
+    dbuf<uint32_t> comb_vals, comb_vals_alt, flag, fpos, bsums, total;
+
+    // pair list and manifold cache, double-buffered (cur = last step's result)
+    dbuf<uint64_t> keys[2];
+    dbuf<uint32_t> src;
+    dbuf<int32_t> mcount[2];
+    dbuf<double> mdata[2];
+    uint32_t mcap[2] = {0, 0};
+    uint32_t P[2] = {0, 0};
+    int cur = 0;
+    dbuf<uint8_t> events;
+
+    // narrowphase
+    dbuf<uint32_t> hit_list, ovf_list;
+    dbuf<double> simplex, info;
+    dbuf<uint8_t> status;
+    dbuf<char> big_pool;
+
+    // solver
+    dbuf<double> rows;
+    dbuf<uint8_t> row_valid;
+    dbuf<uint32_t> colour, colour_count, colour_start, cursor, order;
+    dbuf<unsigned long long> claim;
+    uint32_t n_colours = 0;
+
+    pkg_counters last{};
+    bool have_bodies = false;
+
+    bodies_dev B()
+    {
+        bodies_dev b;
+        b.n = n;
+        b.pos = pos.p;
+        b.vel = vel.p;
+        b.ang = ang.p;
+        b.acc = acc.p;
+        b.torque = torque.p;
+        b.quat = quat.p;
+        b.mass = mass.p;
+        b.inv_mass = inv_mass.p;
+        b.I_local = I_local.p;
+        b.Iinv_local = Iinv_local.p;
+        b.Iw = Iw.p;
+        b.Iw_inv = Iw_inv.p;
+        b.restitution = rest.p;
+        b.friction = fric.p;
+        b.fat = fat.p;
+        b.shape = shape.p;
+        b.is_static = is_static.p;
+        b.moved = moved.p;
+        return b;
+    }
+    shapes_dev S()
+    {
+        shapes_dev s;
+        s.n = n_shapes;
+        s.off = shape_off.p;
+        s.verts = verts.p;
+        s.local_box = shape_box.p;
+        return s;
+    }
+    manifolds_dev MF(int k)
+    {
+        manifolds_dev m;
+        m.cap = mcap[k];
+        m.count = mcount[k].p;
+        m.data = mdata[k].p;
+        return m;
+    }
+
+    unsigned long long read_counter(int slot)
+    {
+        hip_check(hipMemcpyAsync(h_counters + slot, counters.p + slot, sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, st),
+                  "counter copy");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        return h_counters[slot];
+    }
+    void reset_counter(int slot) { launch_fill_u64(counters.p + slot, 0ull, 1, st); }
+
+    void step_once(double dt, hipEvent_t *ev);
+};
+
+namespace
+{
+void record(hipEvent_t *ev, int k, hipStream_t st)
+{
+    if (ev) hip_check(hipEventRecord(ev[k], st), "event record");
+}
+} // namespace
+
+void pkg_ctx::step_once(double dt, hipEvent_t *ev)
+{
+    const bool integrate = (cfg.flags & PKG_FLAG_NO_INTEGRATE) == 0;
+    const bool solve = (cfg.flags & PKG_FLAG_NO_SOLVER) == 0;
+    const bool all_moved = (cfg.flags & PKG_FLAG_ALL_MOVED) != 0;
+    bodies_dev b = B();
+    shapes_dev s = S();
+    launch_fill_u64(counters.p, 0ull, C_NCOUNTERS, st);
+    record(ev, 0, st);
+
+    // ---- 1. gravity, integrate_vel, leaf update (world.cpp:22-34)
+    if (integrate)
+        launch_integrate_vel(b, s, cfg.gravity[0], cfg.gravity[1], cfg.gravity[2], dt, counters.p, st);
+    else
+        launch_fill_u32(reinterpret_cast<uint32_t *>(moved.p), 0u, (n + 3) / 4, st);
+    record(ev, 1, st);
+
+    // ---- 2. broadphase (collision_phases.h:377-436)
+    lbvh_dev t;
+    t.n = n;
+    t.morton = morton.p;
+    t.morton_alt = morton_alt.p;
+    t.ids = ids.p;
+    t.ids_alt = ids_alt.p;
+    t.left = left.p;
+    t.right = right.p;
+    t.parent = parent.p;
+    t.escape = escape.p;
+    t.node_box = node_box.p;
+    t.leaf_body = leaf_body.p;
+    t.visit = visit.p;
+    t.scene = scene.p;
+    launch_lbvh_build(t, b, sc, st);
+    record(ev, 2, st);
+    if (t.morton != morton.p)
+    {
+        std::swap(morton.p, morton_alt.p);
+        std::swap(ids.p, ids_alt.p);
+    }
+    const uint32_t Pprev = P[cur];
+    const int nxt = 1 - cur;
+    uint32_t M = 0;
+    for (int attempt = 0;; ++attempt)
+    {
+        size_t want = std::max<size_t>(comb_keys.cap, static_cast<size_t>(Pprev) + 4 * static_cast<size_t>(n) + 1024);
+        comb_keys.reserve(want);
+        comb_vals.reserve(want);
+        uint32_t cap = static_cast<uint32_t>(std::min<size_t>(comb_keys.cap, 0xFFFFFFF0u));
+        reset_counter(C_COMBINED);
+        launch_prev_filter(keys[cur].p, Pprev, b, comb_keys.p, comb_vals.p, cap, counters.p + C_COMBINED, st);
+        launch_pair_query(t, b, all_moved, comb_keys.p, comb_vals.p, cap, counters.p, st);
+        unsigned long long cnt = read_counter(C_COMBINED);
+        if (read_counter(C_TREE_FAIL)) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
+        if (cnt <= cap)
+        {
+            M = static_cast<uint32_t>(cnt);
+            break;
+        }
+        if (cnt > 0xFFFFFFF0ull || attempt > 2)
+            throw pk_error(PKG_OVERFLOW_PAIRS, "candidate pair count exceeds 32-bit indexing");
+        size_t grow = static_cast<size_t>(cnt + cnt / 4 + 1024);
+        comb_keys.release();
+        comb_vals.release();
+        comb_keys.reserve(grow);
+        comb_vals.reserve(grow);
+    }
+    comb_keys_alt.reserve(M);
+    comb_vals_alt.reserve(M);
+    uint64_t *ck = comb_keys.p;
+    uint32_t *cv = comb_vals.p;
+    if (M > 1)
+    {
+        int bits = id_bits(n);
+        int shifts[16];
+        int ns = 0;
+        for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
+        for (int sft = 32; sft < 32 + bits; sft += 8) shifts[ns++] = sft;
+        if (radix_sort_u64(comb_keys.p, comb_vals.p, comb_keys_alt.p, comb_vals_alt.p, M, shifts, ns, sc, st))
+        {
+            ck = comb_keys_alt.p;
+            cv = comb_vals_alt.p;
+        }
+    }
+    flag.reserve(M);
+    fpos.reserve(M);
+    bsums.reserve(scan_block_sums(M));
+    total.reserve(1);
+    src.reserve(std::max<uint32_t>(M, 1));
+    keys[nxt].reserve(std::max<uint32_t>(M, 1));
+    uint32_t Pn = 0;
+    if (M > 0)
+    {
+        // per-key source index staged in the value buffer the sort left free
+        uint32_t *staged = (cv == comb_vals_alt.p) ? comb_vals.p : comb_vals_alt.p;
+        launch_dedupe_flags(ck, cv, M, flag.p, staged, st);
+        exclusive_scan_u32(flag.p, fpos.p, M, bsums.p, total.p, st);
+        launch_compact_pairs(ck, staged, flag.p, fpos.p, M, keys[nxt].p, src.p, st);
+        uint32_t h_total = 0;
+        hip_check(hipMemcpyAsync(&h_total, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total copy");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        Pn = h_total;
+    }
+    record(ev, 3, st);
+
+    // ---- 3. narrowphase (collision_phases.h:244-320)
+    if (mcap[nxt] < Pn)
+    {
+        uint32_t c = std::max<uint32_t>(Pn + Pn / 4, 1024);
+        mcount[nxt].release();
+        mdata[nxt].release();
+        mcount[nxt].reserve(c);
+        mdata[nxt].reserve(static_cast<size_t>(c) * 4 * kMF);
+        mcap[nxt] = c;
+    }
+    uint32_t ncap = std::max<uint32_t>(mcap[nxt], 1);
+    hit_list.reserve(ncap);
+    ovf_list.reserve(ncap);
+    simplex.reserve(static_cast<size_t>(ncap) * 37);
+    info.reserve(static_cast<size_t>(ncap) * 10);
+    status.reserve(ncap);
+    events.reserve(ncap);
+    gjk_out_dev g;
+    g.hit_list = hit_list.p;
+    g.simplex = simplex.p;
+    g.cap = static_cast<uint32_t>(std::min(hit_list.cap, simplex.cap / 37));
+    g.info = info.p;
+    g.status = status.p;
+    g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
+    if (Pn)
+    {
+        launch_gjk(keys[nxt].p, Pn, b, s, g, counters.p, st);
+        record(ev, 4, st);
+        uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
+        record(ev, 9, st);
+        launch_epa(keys[nxt].p, hits, b, s, g, ovf_list.p, counters.p, st);
+        uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
+        if (novf)
+        {
+            big_pool.reserve(static_cast<size_t>(novf) * epa_big_bytes());
+            launch_epa_big(keys[nxt].p, ovf_list.p, novf, b, s, g, big_pool.p, counters.p, st);
+            if (read_counter(C_EPA_BIG_FAIL))
+                throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
+        }
+        record(ev, 5, st);
+        launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p, counters.p, st);
+    }
+    else
+    {
+        record(ev, 4, st);
+        record(ev, 9, st);
+        record(ev, 5, st);
+    }
+    record(ev, 6, st);
+
+    // ---- 4. contacts (constraint.h:1052-1201)
+    n_colours = 0;
+    if (solve && Pn)
+    {
+        rows.reserve(static_cast<size_t>(4) * ncap * kRowF);
+        row_valid.reserve(static_cast<size_t>(4) * ncap);
+        colour.reserve(ncap);
+        claim.reserve(std::max<uint32_t>(n, 1));
+        rows_dev r;
+        r.cap4 = static_cast<uint32_t>(std::min(rows.cap / kRowF, row_valid.cap));
+        r.d = rows.p;
+        r.valid = row_valid.p;
+        double thr = 2.0 * gnorm * dt;
+        launch_contact_setup(keys[nxt].p, Pn, b, MF(nxt), r, dt, thr, colour.p, counters.p, st);
+        if (read_counter(C_ROWS) > 0)
+        {
+            uint32_t c = 0;
+            while (true)
+            {
+                launch_fill_u64(claim.p, ~0ull, n, st);
+                launch_colour_claim(keys[nxt].p, Pn, b, colour.p, claim.p, st);
+                reset_counter(C_REMAIN);
+                launch_colour_assign(keys[nxt].p, Pn, b, colour.p, claim.p, c, counters.p, st);
+                ++c;
+                if (read_counter(C_REMAIN) == 0) break;
+                if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
+            }
+            n_colours = c;
+            colour_count.reserve(c);
+            colour_start.reserve(c + 1);
+            cursor.reserve(c);
+            order.reserve(Pn);
+            launch_colour_bucket(colour.p, Pn, c, colour_count.p, colour_start.p, cursor.p, order.p, st);
+            std::vector<uint32_t> cs(c + 1);
+            hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                      "colour copy");
+            hip_check(hipStreamSynchronize(st), "stream sync");
+            for (uint32_t k = 0; k < c; ++k) launch_solve_colour(keys[nxt].p, order.p, cs[k], cs[k + 1], b, r, true, st);
+            for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                for (uint32_t k = 0; k < c; ++k)
+                    launch_solve_colour(keys[nxt].p, order.p, cs[k], cs[k + 1], b, r, false, st);
+            launch_impulse_writeback(Pn, MF(nxt), r, st);
+        }
+    }
+    record(ev, 7, st);
+
+    // ---- 5. integrate_pos (world.cpp:50-55)
+    if (integrate) launch_integrate_pos(b, dt, st);
+    record(ev, 8, st);
+
+    P[nxt] = Pn;
+    cur = nxt;
+    hip_check(hipMemcpyAsync(h_counters, counters.p, C_NCOUNTERS * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+              "counters copy");
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    last.moved = h_counters[C_MOVED];
+    last.pairs = Pn;
+    last.gjk_hits = h_counters[C_HITS];
+    last.contacts = h_counters[C_CONTACTS];
+    last.rows = h_counters[C_ROWS];
+    last.colours = n_colours;
+    last.epa_overflow = h_counters[C_EPA_OVF];
+    last.candidates = h_counters[C_COMBINED];
+}
+
+// ------------------------------------------------------------------ C ABI
+#define PKG_GUARD(ctx, ...)                                                                        \
+    try                                                                                            \
+    {                                                                                              \
+        __VA_ARGS__;                                                                               \
+        return PKG_OK;                                                                             \
+    }                                                                                              \
+    catch (const pk_error &e)                                                                      \
+    {                                                                                              \
+        if (ctx) (ctx)->err = e.what();                                                            \
+        return e.st;                                                                               \
+    }                                                                                              \
+    catch (const std::exception &e)                                                                \
+    {                                                                                              \
+        if (ctx) (ctx)->err = e.what();                                                            \
+        return PKG_DEVICE_ERROR;                                                                   \
+    }
+
+namespace
+{
+template <class T> void upload(dbuf<T> &d, const T *h, size_t count, hipStream_t st)
+{
+    d.reserve(count);
+    if (count) hip_check(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+}
+// AoS (k components per element) -> planar
+std::vector<double> planar(const double *a, uint32_t n, int k)
+{
+    std::vector<double> o(static_cast<size_t>(n) * k);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < k; ++c) o[static_cast<size_t>(c) * n + i] = a[static_cast<size_t>(i) * k + c];
+    return o;
+}
+void aos(const std::vector<double> &p, uint32_t n, int k, double *out)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < k; ++c) out[static_cast<size_t>(i) * k + c] = p[static_cast<size_t>(c) * n + i];
+}
+template <class T> std::vector<T> download(const dbuf<T> &d, size_t count, hipStream_t st)
+{
+    std::vector<T> h(count);
+    if (count) hip_check(hipMemcpyAsync(h.data(), d.p, count * sizeof(T), hipMemcpyDeviceToHost, st), "download");
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    return h;
+}
+} // namespace
+
+extern "C"
+{
+
+pkg_status pkg_create(pkg_ctx **out, const pkg_config *cfg)
+{
+    if (!out || !cfg) return PKG_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PKG_NO_DEVICE;
+    auto *c = new pkg_ctx();
+    try
+    {
+        c->cfg = *cfg;
+        const double *g = cfg->gravity;
+        c->gnorm = std::sqrt((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]); // world.h:383
+        hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "stream create");
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), C_NCOUNTERS * sizeof(unsigned long long)),
+                  "pinned alloc");
+        c->counters.reserve(C_NCOUNTERS);
+        c->scene.reserve(6);
+    }
+    catch (const pk_error &e)
+    {
+        delete c;
+        return e.st;
+    }
+    *out = c;
+    return PKG_OK;
+}
+
+pkg_status pkg_destroy(pkg_ctx *ctx)
+{
+    if (!ctx) return PKG_INVALID_ARG;
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    // every dbuf member frees in release(); the ctx owns nothing else on the device
+    ctx->shape_off.release(); ctx->verts.release(); ctx->shape_box.release();
+    for (auto *d : {&ctx->pos, &ctx->vel, &ctx->ang, &ctx->acc, &ctx->torque, &ctx->quat, &ctx->mass,
+                    &ctx->inv_mass, &ctx->I_local, &ctx->Iinv_local, &ctx->Iw, &ctx->Iw_inv, &ctx->rest,
+                    &ctx->fric, &ctx->fat, &ctx->node_box, &ctx->simplex, &ctx->info, &ctx->rows,
+                    &ctx->mdata[0], &ctx->mdata[1]})
+        d->release();
+    for (auto *d : {&ctx->shape, &ctx->ids, &ctx->ids_alt, &ctx->parent, &ctx->escape, &ctx->leaf_body,
+                    &ctx->visit, &ctx->comb_vals, &ctx->comb_vals_alt, &ctx->flag, &ctx->fpos, &ctx->bsums,
+                    &ctx->total, &ctx->src, &ctx->hit_list, &ctx->ovf_list, &ctx->colour, &ctx->colour_count,
+                    &ctx->colour_start, &ctx->cursor, &ctx->order})
+        d->release();
+    for (auto *d : {&ctx->is_static, &ctx->moved, &ctx->events, &ctx->status, &ctx->row_valid}) d->release();
+    for (auto *d : {&ctx->morton, &ctx->morton_alt, &ctx->comb_keys, &ctx->comb_keys_alt, &ctx->keys[0], &ctx->keys[1]})
+        d->release();
+    ctx->left.release(); ctx->right.release(); ctx->mcount[0].release(); ctx->mcount[1].release();
+    ctx->counters.release(); ctx->scene.release(); ctx->claim.release(); ctx->big_pool.release();
+    if (ctx->sc.hist) (void)hipFree(ctx->sc.hist);
+    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+    return PKG_OK;
+}
+
+const char *pkg_last_error(const pkg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *offsets, const double *verts)
+{
+    if (!ctx || !offsets || (n_shapes && !verts)) return PKG_INVALID_ARG;
+    for (uint32_t i = 0; i < n_shapes; ++i)
+        if (offsets[i + 1] <= offsets[i])
+        {
+            ctx->err = "every shape needs at least one vertex (mesh::make asserts non-empty, mesh.cpp:19)";
+            return PKG_INVALID_ARG;
+        }
+    PKG_GUARD(ctx, {
+        uint32_t nv = offsets[n_shapes];
+        upload(ctx->shape_off, offsets, n_shapes + 1, ctx->st);
+        upload(ctx->verts, verts, static_cast<size_t>(nv) * 3, ctx->st);
+        ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
+        ctx->n_shapes = n_shapes;
+        ctx->n_verts = nv;
+        launch_shape_boxes(ctx->S(), ctx->st);
+        hip_check(hipStreamSynchronize(ctx->st), "shape upload");
+    })
+}
+
+pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
+{
+    if (!ctx || !d) return PKG_INVALID_ARG;
+    if (n && (!d->is_static || !d->shape || !d->pos || !d->vel || !d->angvel || !d->quat || !d->mass ||
+              !d->inertia || !d->restitution || !d->friction))
+        return PKG_INVALID_ARG;
+    for (uint32_t i = 0; i < n; ++i)
+        if (d->shape[i] >= ctx->n_shapes)
+        {
+            ctx->err = "body references a shape that was not uploaded (object must have a mesh, object.h:121)";
+            return PKG_INVALID_ARG;
+        }
+    PKG_GUARD(ctx, {
+        hipStream_t st = ctx->st;
+        ctx->n = n;
+        std::vector<double> mass(n);
+        for (uint32_t i = 0; i < n; ++i)
+            mass[i] = d->is_static[i] ? std::numeric_limits<double>::infinity() : d->mass[i]; // object.h:81-85
+        auto up = [&](dbuf<double> &b, const double *a, int k)
+        {
+            std::vector<double> p = planar(a, n, k);
+            upload(b, p.data(), p.size(), st);
+            hip_check(hipStreamSynchronize(st), "upload");
+        };
+        up(ctx->pos, d->pos, 3);
+        up(ctx->vel, d->vel, 3);
+        up(ctx->ang, d->angvel, 3);
+        up(ctx->quat, d->quat, 4);
+        up(ctx->mass, mass.data(), 1);
+        up(ctx->I_local, d->inertia, 9);
+        up(ctx->rest, d->restitution, 1);
+        up(ctx->fric, d->friction, 1);
+        upload(ctx->shape, d->shape, n, st);
+        upload(ctx->is_static, d->is_static, n, st);
+        for (auto *b : {&ctx->acc, &ctx->torque}) b->reserve(static_cast<size_t>(3) * n);
+        ctx->inv_mass.reserve(n);
+        for (auto *b : {&ctx->Iinv_local, &ctx->Iw, &ctx->Iw_inv}) b->reserve(static_cast<size_t>(9) * n);
+        ctx->fat.reserve(static_cast<size_t>(6) * n);
+        ctx->moved.reserve(n + 4);
+        // broadphase arrays
+        ctx->morton.reserve(n);
+        ctx->morton_alt.reserve(n);
+        ctx->ids.reserve(n);
+        ctx->ids_alt.reserve(n);
+        ctx->left.reserve(n);
+        ctx->right.reserve(n);
+        ctx->parent.reserve(2 * static_cast<size_t>(n));
+        ctx->escape.reserve(2 * static_cast<size_t>(n));
+        ctx->leaf_body.reserve(n);
+        ctx->visit.reserve(n);
+        ctx->node_box.reserve(12 * static_cast<size_t>(n));
+        launch_body_init(ctx->B(), ctx->S(), st);
+        ctx->P[0] = ctx->P[1] = 0;
+        ctx->cur = 0;
+        ctx->have_bodies = true;
+        hip_check(hipStreamSynchronize(st), "body upload");
+    })
+}
+
+pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids, const double *pos,
+                              const double *vel, const double *angvel, const double *quat)
+{
+    if (!ctx || (count && !ids)) return PKG_INVALID_ARG;
+    for (uint32_t i = 0; i < count; ++i)
+        if (ids[i] >= ctx->n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        if (!count) return PKG_OK;
+        hipStream_t st = ctx->st;
+        dbuf<uint32_t> di;
+        dbuf<double> dp, dv, da, dq;
+        upload(di, ids, count, st);
+        if (pos) upload(dp, pos, 3 * static_cast<size_t>(count), st);
+        if (vel) upload(dv, vel, 3 * static_cast<size_t>(count), st);
+        if (angvel) upload(da, angvel, 3 * static_cast<size_t>(count), st);
+        if (quat) upload(dq, quat, 4 * static_cast<size_t>(count), st);
+        k_set_state<<<(count + 255) / 256, 256, 0, st>>>(ctx->B(), count, di.p, pos ? dp.p : nullptr,
+                                                         vel ? dv.p : nullptr, angvel ? da.p : nullptr,
+                                                         quat ? dq.p : nullptr);
+        hip_check(hipGetLastError(), "set state launch");
+        hip_check(hipStreamSynchronize(st), "set state");
+        di.release();
+        dp.release();
+        dv.release();
+        da.release();
+        dq.release();
+    })
+}
+
+pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps)
+{
+    if (!ctx || nsteps < 0 || !(dt >= 0.0)) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        for (int i = 0; i < nsteps; ++i) ctx->step_once(dt, nullptr);
+        hip_check(hipGetLastError(), "step");
+    })
+}
+
+pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms, int n_stages)
+{
+    if (!ctx || nsteps < 0 || !stage_ms || n_stages < 1) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        hipEvent_t ev[10];
+        for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
+        // stage k spans events (from[k], to[k]); event 9 follows the GJK hit-count read
+        const int from[8] = {0, 1, 2, 3, 9, 5, 6, 7};
+        const int to[8] = {1, 2, 3, 4, 5, 6, 7, 8};
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < nsteps; ++i)
+        {
+            ctx->step_once(dt, ev);
+            hip_check(hipEventSynchronize(ev[8]), "event sync");
+            for (int k = 0; k < 8; ++k)
+            {
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]), "elapsed");
+                acc[k] += ms;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < n_stages && k < 8; ++k) stage_ms[k] = acc[k];
+    })
+}
+
+pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out)
+{
+    if (!ctx || !out) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        uint32_t n = ctx->n;
+        hipStream_t st = ctx->st;
+        auto get = [&](dbuf<double> &b, int k, double *dst)
+        {
+            if (!dst) return;
+            std::vector<double> h = download(b, static_cast<size_t>(n) * k, st);
+            aos(h, n, k, dst);
+        };
+        get(ctx->pos, 3, out->pos);
+        get(ctx->vel, 3, out->vel);
+        get(ctx->ang, 3, out->angvel);
+        get(ctx->quat, 4, out->quat);
+        get(ctx->fat, 6, out->fat);
+        if (out->moved)
+        {
+            std::vector<uint8_t> m = download(ctx->moved, n, st);
+            std::memcpy(out->moved, m.data(), n);
+        }
+    })
+}
+
+pkg_status pkg_get_pairs(pkg_ctx *ctx, uint64_t *keys, size_t cap, size_t *n)
+{
+    if (!ctx || !n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        uint32_t P = ctx->P[ctx->cur];
+        *n = P;
+        if (keys && cap)
+        {
+            std::vector<uint64_t> k = download(ctx->keys[ctx->cur], std::min<size_t>(cap, P), ctx->st);
+            std::memcpy(keys, k.data(), k.size() * sizeof(uint64_t));
+        }
+    })
+}
+
+pkg_status pkg_get_manifolds(pkg_ctx *ctx, int32_t *counts, double *data, size_t cap, size_t *n)
+{
+    if (!ctx || !n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        int c = ctx->cur;
+        uint32_t P = ctx->P[c];
+        *n = P;
+        size_t m = std::min<size_t>(cap, P);
+        if (m && (counts || data))
+        {
+            std::vector<int32_t> cnt = download(ctx->mcount[c], P, ctx->st);
+            std::vector<double> raw = download(ctx->mdata[c], static_cast<size_t>(ctx->mcap[c]) * 4 * kMF, ctx->st);
+            for (size_t p = 0; p < m; ++p)
+            {
+                if (counts) counts[p] = cnt[p];
+                if (data)
+                    for (int k = 0; k < 4; ++k)
+                        for (int f = 0; f < kMF; ++f)
+                            data[(p * 4 + k) * kMF + f] =
+                                k < cnt[p] ? raw[static_cast<size_t>(k * kMF + f) * ctx->mcap[c] + p] : 0.0;
+            }
+        }
+    })
+}
+
+pkg_status pkg_get_events(pkg_ctx *ctx, uint64_t *keys, int32_t *kinds, size_t cap, size_t *n)
+{
+    if (!ctx || !n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        int c = ctx->cur;
+        uint32_t P = ctx->P[c];
+        std::vector<uint8_t> ev = download(ctx->events, P, ctx->st);
+        std::vector<uint64_t> k = download(ctx->keys[c], P, ctx->st);
+        size_t cnt = 0;
+        for (uint32_t p = 0; p < P; ++p)
+            if (ev[p])
+            {
+                if (cnt < cap)
+                {
+                    if (keys) keys[cnt] = k[p];
+                    if (kinds) kinds[cnt] = ev[p];
+                }
+                ++cnt;
+            }
+        *n = cnt;
+    })
+}
+
+pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out)
+{
+    if (!ctx || !out) return PKG_INVALID_ARG;
+    *out = ctx->last;
+    return PKG_OK;
+}
+
+pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, size_t cap, size_t *n)
+{
+    if (!ctx || !n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        uint32_t P = ctx->P[ctx->cur];
+        size_t cnt = 0;
+        if (ctx->n_colours && P)
+        {
+            std::vector<uint32_t> col = download(ctx->colour, P, ctx->st);
+            for (uint32_t p = 0; p < P; ++p)
+                if (col[p] != kNone)
+                {
+                    if (colours && cnt < cap) colours[cnt] = col[p];
+                    ++cnt;
+                }
+        }
+        *n = cnt;
+    })
+}
+
+pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a, const double *prm_a,
+                             const uint32_t *vbeg_a, const uint32_t *vend_a, const int32_t *kind_b,
+                             const double *prm_b, const uint32_t *vbeg_b, const uint32_t *vend_b,
+                             const double *verts, uint32_t n_verts, double *out, int32_t *hit)
+{
+    if (!ctx || !kind_a || !prm_a || !vbeg_a || !vend_a || !kind_b || !prm_b || !vbeg_b || !vend_b || !out || !hit)
+        return PKG_INVALID_ARG;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        for (int side = 0; side < 2; ++side)
+        {
+            int k = side ? kind_b[i] : kind_a[i];
+            uint32_t vb = side ? vbeg_b[i] : vbeg_a[i], ve = side ? vend_b[i] : vend_a[i];
+            if (k < 0 || k > 2) return PKG_INVALID_ARG;
+            if (k == 2 && (ve <= vb || ve > n_verts || !verts)) return PKG_INVALID_ARG;
+        }
+    }
+    PKG_GUARD(ctx, {
+        if (!n) return PKG_OK;
+        hipStream_t st = ctx->st;
+        dbuf<int32_t> ka, kb, dh;
+        dbuf<double> pa, pb, dv, dout;
+        dbuf<uint32_t> ba, ea, bb, eb;
+        dbuf<char> pool;
+        upload(ka, kind_a, n, st);
+        upload(kb, kind_b, n, st);
+        upload(pa, prm_a, 10 * static_cast<size_t>(n), st);
+        upload(pb, prm_b, 10 * static_cast<size_t>(n), st);
+        upload(ba, vbeg_a, n, st);
+        upload(ea, vend_a, n, st);
+        upload(bb, vbeg_b, n, st);
+        upload(eb, vend_b, n, st);
+        std::vector<double> vz(3, 0.0);
+        if (n_verts && verts)
+            upload(dv, verts, 3 * static_cast<size_t>(n_verts), st);
+        else
+            upload(dv, vz.data(), 3, st);
+        dout.reserve(10 * static_cast<size_t>(n));
+        dh.reserve(n);
+        pool.reserve(epa_big_bytes());
+        shape_pair_dev sp;
+        sp.kind_a = ka.p;
+        sp.kind_b = kb.p;
+        sp.prm_a = pa.p;
+        sp.prm_b = pb.p;
+        sp.vbeg_a = ba.p;
+        sp.vend_a = ea.p;
+        sp.vbeg_b = bb.p;
+        sp.vend_b = eb.p;
+        sp.verts = dv.p;
+        launch_gjk_epa_batch(sp, n, dout.p, dh.p, pool.p, st);
+        hip_check(hipGetLastError(), "batch launch");
+        std::vector<double> o = download(dout, 10 * static_cast<size_t>(n), st);
+        std::vector<int32_t> h = download(dh, n, st);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            hit[i] = h[i];
+            if (h[i] == 1) std::memcpy(out + 10 * static_cast<size_t>(i), o.data() + 10 * static_cast<size_t>(i), 10 * sizeof(double));
+        }
+        for (auto *b : {&ka, &kb, &dh}) b->release();
+        for (auto *b : {&pa, &pb, &dv, &dout}) b->release();
+        for (auto *b : {&ba, &ea, &bb, &eb}) b->release();
+        pool.release();
+        for (uint32_t i = 0; i < n; ++i)
+            if (h[i] == 3) throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
+    })
+}
+
+} // extern "C"

@@ -1,0 +1,431 @@
+// Narrowphase: narrow_phase::calculate (include/physkit/collision/collision_phases.h:
+// 244-320) split into three passes so each runs dense:
+//   1. GJK, one thread per active pair; hits are compacted (wavefront-aggregated
+//      append) with their final simplex;
+//   2. EPA, one thread per hit, polytope/faces/heap in a private 192-face store;
+//      a hit that needs more faces is rerun by a second kernel with a 4096-face store
+//      in global memory, so capacity never changes a result;
+//   3. manifold update (warm-start transfer, persistence, add/reduce, events), one
+//      thread per pair, reading the previous step's manifold by source index.
+#include "pk_access.hpp"
+#include "pk_gjk.hpp"
+
+namespace pk
+{
+namespace
+{
+constexpr int kBlock = 256;
+constexpr int kEpaBlock = 64;
+constexpr int kFastCap = 192;
+constexpr int kBigCap = 4096;
+constexpr int kSimplexF = 37; // n, then 4 x (p, pa, pb)
+unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
+
+__device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_dev &s, uint32_t id)
+{
+    gshape g;
+    g.kind = 2;
+    g.a = ld3(b.pos, b.n, id);
+    g.b = mk(0.0, 0.0, 0.0);
+    g.q = ldq(b.quat, b.n, id);
+    uint32_t sh = b.shape[id];
+    uint32_t o = s.off[sh];
+    g.v = s.verts + 3 * static_cast<size_t>(o);
+    g.nv = s.off[sh + 1] - o;
+    return g;
+}
+
+__device__ __forceinline__ void st_spt(double *d, uint32_t cap, uint32_t slot, int k, const spt &p)
+{
+    double *b = d + static_cast<size_t>(1 + 9 * k) * cap + slot;
+    b[0] = p.p.x;
+    b[cap] = p.p.y;
+    b[2 * static_cast<size_t>(cap)] = p.p.z;
+    b[3 * static_cast<size_t>(cap)] = p.pa.x;
+    b[4 * static_cast<size_t>(cap)] = p.pa.y;
+    b[5 * static_cast<size_t>(cap)] = p.pa.z;
+    b[6 * static_cast<size_t>(cap)] = p.pb.x;
+    b[7 * static_cast<size_t>(cap)] = p.pb.y;
+    b[8 * static_cast<size_t>(cap)] = p.pb.z;
+}
+__device__ __forceinline__ spt ld_spt(const double *d, uint32_t cap, uint32_t slot, int k)
+{
+    const double *b = d + static_cast<size_t>(1 + 9 * k) * cap + slot;
+    spt p;
+    p.p = mk(b[0], b[cap], b[2 * static_cast<size_t>(cap)]);
+    p.pa = mk(b[3 * static_cast<size_t>(cap)], b[4 * static_cast<size_t>(cap)], b[5 * static_cast<size_t>(cap)]);
+    p.pb = mk(b[6 * static_cast<size_t>(cap)], b[7 * static_cast<size_t>(cap)], b[8 * static_cast<size_t>(cap)]);
+    return p;
+}
+__device__ __forceinline__ void save_simplex(double *d, uint32_t cap, uint32_t slot, const simplex &x)
+{
+    d[slot] = static_cast<double>(x.n);
+    st_spt(d, cap, slot, 0, x.s0);
+    if (x.n > 1) st_spt(d, cap, slot, 1, x.s1);
+    if (x.n > 2) st_spt(d, cap, slot, 2, x.s2);
+    if (x.n > 3) st_spt(d, cap, slot, 3, x.s3);
+}
+__device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint32_t slot, simplex &x)
+{
+    x.n = static_cast<int>(d[slot]);
+    x.s0 = ld_spt(d, cap, slot, 0);
+    if (x.n > 1) x.s1 = ld_spt(d, cap, slot, 1);
+    if (x.n > 2) x.s2 = ld_spt(d, cap, slot, 2);
+    if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_gjk(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, gjk_out_dev g,
+          unsigned long long *counters)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    simplex x;
+    x.n = 0;
+    if (p < n_pairs)
+    {
+        uint64_t key = keys[p];
+        gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
+        gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        hit = gjk(A, B, x);
+        g.status[p] = 0;
+    }
+    unsigned long long slot = wave_append(&counters[C_HITS], hit);
+    if (hit)
+    {
+        g.hit_list[slot] = p;
+        save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
+    }
+}
+
+__device__ __forceinline__ void write_info(double *info, uint32_t cap, uint32_t p, const double *o)
+{
+    for (int k = 0; k < 10; ++k) info[static_cast<size_t>(k) * cap + p] = o[k];
+}
+
+__global__ void __launch_bounds__(kEpaBlock)
+    k_epa(const uint64_t *keys, uint32_t n_hits, bodies_dev b, shapes_dev s, gjk_out_dev g,
+          uint32_t *ovf_list, unsigned long long *counters)
+{
+    uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ovf = false;
+    if (h < n_hits)
+    {
+        uint32_t p = g.hit_list[h];
+        uint64_t key = keys[p];
+        gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
+        gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        simplex x;
+        load_simplex(g.simplex, g.cap, h, x);
+        epa_store<kFastCap> store;
+        double out[10];
+        int r = epa_solve(A, B, x, store, out);
+        if (r == 1) write_info(g.info, g.pair_cap, p, out);
+        g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
+        ovf = r == 3;
+    }
+    unsigned long long slot = wave_append(&counters[C_EPA_OVF], ovf);
+    if (ovf) ovf_list[slot] = h;
+}
+
+__global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b,
+                          shapes_dev s, gjk_out_dev g, epa_store<kBigCap> *pool,
+                          unsigned long long *counters)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_ovf) return;
+    uint32_t h = ovf_list[t];
+    uint32_t p = g.hit_list[h];
+    uint64_t key = keys[p];
+    gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
+    gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+    simplex x;
+    load_simplex(g.simplex, g.cap, h, x);
+    double out[10];
+    int r = epa_solve(A, B, x, pool[t], out);
+    if (r == 1) write_info(g.info, g.pair_cap, p, out);
+    g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
+    if (r == 3) atomicAdd(&counters[C_EPA_BIG_FAIL], 1ull);
+}
+
+// ---- manifolds (collision_phases.h:75-198, 244-320) ----
+struct contact
+{
+    v3 n, la, lb;
+    double depth, ln, lt1, lt2;
+};
+
+__device__ __forceinline__ contact ld_contact(const manifolds_dev &m, uint32_t p, int c)
+{
+    const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+    size_t s = m.cap;
+    contact k;
+    k.n = mk(d[0], d[s], d[2 * s]);
+    k.la = mk(d[3 * s], d[4 * s], d[5 * s]);
+    k.lb = mk(d[6 * s], d[7 * s], d[8 * s]);
+    k.depth = d[9 * s];
+    k.ln = d[10 * s];
+    k.lt1 = d[11 * s];
+    k.lt2 = d[12 * s];
+    return k;
+}
+__device__ __forceinline__ void st_contact(const manifolds_dev &m, uint32_t p, int c, const contact &k)
+{
+    double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+    size_t s = m.cap;
+    d[0] = k.n.x;
+    d[s] = k.n.y;
+    d[2 * s] = k.n.z;
+    d[3 * s] = k.la.x;
+    d[4 * s] = k.la.y;
+    d[5 * s] = k.la.z;
+    d[6 * s] = k.lb.x;
+    d[7 * s] = k.lb.y;
+    d[8 * s] = k.lb.z;
+    d[9 * s] = k.depth;
+    d[10 * s] = k.ln;
+    d[11 * s] = k.lt1;
+    d[12 * s] = k.lt2;
+}
+
+struct mani
+{
+    contact c[4];
+    int n;
+};
+
+__device__ void add_reduce(mani &m, const contact &k) // collision_phases.h:139-198
+{
+    contact pool[5] = {m.c[0], m.c[1], m.c[2], m.c[3], k};
+    int best[4] = {0, 1, 2, 3};
+    for (int i = 1; i < 5; ++i)
+        if (pool[i].depth > pool[best[0]].depth) best[0] = i;
+    double md = -1.0;
+    for (int i = 0; i < 5; ++i)
+    {
+        if (i == best[0]) continue;
+        double d2 = sqn(pool[i].la - pool[best[0]].la);
+        if (d2 > md)
+        {
+            md = d2;
+            best[1] = i;
+        }
+    }
+    double ma = -1.0;
+    v3 e0 = pool[best[1]].la - pool[best[0]].la;
+    for (int i = 0; i < 5; ++i)
+    {
+        if (i == best[0] || i == best[1]) continue;
+        v3 e1 = pool[i].la - pool[best[0]].la;
+        double a2 = sqn(cross(e0, e1));
+        if (a2 > ma)
+        {
+            ma = a2;
+            best[2] = i;
+        }
+    }
+    md = -1.0;
+    for (int i = 0; i < 5; ++i)
+    {
+        if (i == best[0] || i == best[1] || i == best[2]) continue;
+        double d2 = sqn(pool[i].la - pool[best[2]].la);
+        if (d2 > md)
+        {
+            md = d2;
+            best[3] = i;
+        }
+    }
+    for (int i = 0; i < 4; ++i) m.c[i] = pool[best[i]];
+    m.n = 4;
+}
+
+__device__ __forceinline__ void add_contact(mani &m, const contact &k) // :124-130
+{
+    if (m.n < 4)
+        m.c[m.n++] = k;
+    else
+        add_reduce(m, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_manifold(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, bodies_dev b, gjk_out_dev g,
+               manifolds_dev prev, manifolds_dev next, uint8_t *events, unsigned long long *counters)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t count = 0;
+    if (p < n_pairs)
+    {
+        const double distance2_eps = 0.005 * 0.005; // :212
+        const double breaking = 0.05;               // :213
+        const double drift2_eps = 0.06;             // :267
+        uint64_t key = keys[p];
+        uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
+        qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+        bool has_new = g.status[p] == 1;
+        contact nc{};
+        if (has_new)
+        {
+            const double *inf = g.info + p;
+            size_t s = g.pair_cap;
+            nc.n = mk(inf[0], inf[s], inf[2 * s]);
+            v3 wa = mk(inf[3 * s], inf[4 * s], inf[5 * s]);
+            v3 wb = mk(inf[6 * s], inf[7 * s], inf[8 * s]);
+            nc.la = rotate(conj(qa), wa - pa); // particle.h:107-108
+            nc.lb = rotate(conj(qb), wb - pb);
+            nc.depth = inf[9 * s];
+            nc.ln = 0.0;
+            nc.lt1 = 0.0;
+            nc.lt2 = 0.0;
+        }
+        uint32_t sp = src[p];
+        int oldn = sp != kNone ? prev.count[sp] : 0;
+        mani nm;
+        nm.n = 0;
+        for (int i = 0; i < oldn; ++i)
+        {
+            contact old = ld_contact(prev, sp, i);
+            if (has_new)
+            {
+                if (sqn(nc.la - old.la) < distance2_eps || sqn(nc.lb - old.lb) < distance2_eps)
+                {
+                    nc.ln = old.ln;
+                    nc.lt1 = old.lt1;
+                    nc.lt2 = old.lt2;
+                    continue;
+                }
+            }
+            v3 wa = rotate(qa, old.la) + pa; // particle.h:110-111
+            v3 wb = rotate(qb, old.lb) + pb;
+            v3 nrm = has_new ? nc.n : old.n;
+            v3 rel = wb - wa;
+            double depth = dot(rel, nrm);
+            v3 pja = wa + nrm * depth;
+            double drift2 = sqn(pja - wb);
+            if (depth > -breaking && drift2 < drift2_eps)
+            {
+                old.depth = depth;
+                old.n = nrm;
+                add_contact(nm, old);
+            }
+        }
+        if (has_new) add_contact(nm, nc);
+        for (int i = 0; i < nm.n; ++i) st_contact(next, p, i, nm.c[i]);
+        next.count[p] = nm.n;
+        bool was = oldn > 0, is = nm.n > 0;
+        events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
+        count = static_cast<uint32_t>(nm.n);
+    }
+    wave_sum3(&counters[C_CONTACTS], count);
+}
+
+// ---- standalone batch ----
+__device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint32_t *vb, const uint32_t *ve,
+                             const double *verts, uint32_t i)
+{
+    const double *q = prm + 10 * static_cast<size_t>(i);
+    gshape g;
+    g.kind = kind[i];
+    g.v = nullptr;
+    g.nv = 0;
+    g.b = mk(0.0, 0.0, 0.0);
+    g.q = mkq(1.0, 0.0, 0.0, 0.0);
+    if (g.kind == 0)
+    {
+        g.a = mk(q[0], q[1], q[2]);
+        g.b = mk(q[3], q[4], q[5]);
+    }
+    else if (g.kind == 1)
+    {
+        g.a = mk(q[0], q[1], q[2]);
+        g.q = mkq(q[3], q[4], q[5], q[6]);
+        g.b = mk(q[7], q[8], q[9]);
+    }
+    else
+    {
+        g.a = mk(q[0], q[1], q[2]);
+        g.q = mkq(q[3], q[4], q[5], q[6]);
+        g.v = verts + 3 * static_cast<size_t>(vb[i]);
+        g.nv = ve[i] - vb[i];
+    }
+    return g;
+}
+
+template <class S>
+__device__ void batch_one(const shape_pair_dev &sp, uint32_t i, double *out, int32_t *hit, S &store)
+{
+    gshape A = pair_shape(sp.kind_a, sp.prm_a, sp.vbeg_a, sp.vend_a, sp.verts, i);
+    gshape B = pair_shape(sp.kind_b, sp.prm_b, sp.vbeg_b, sp.vend_b, sp.verts, i);
+    simplex x;
+    if (!gjk(A, B, x))
+    {
+        hit[i] = 0;
+        return;
+    }
+    double o[10];
+    int r = epa_solve(A, B, x, store, o);
+    if (r == 1)
+        for (int k = 0; k < 10; ++k) out[10 * static_cast<size_t>(i) + k] = o[k];
+    hit[i] = r == 1 ? 1 : (r == 0 ? 2 : 3);
+}
+
+__global__ void __launch_bounds__(kEpaBlock) k_batch(shape_pair_dev sp, uint32_t n, double *out, int32_t *hit)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    epa_store<kFastCap> store;
+    batch_one(sp, i, out, hit, store);
+}
+
+__global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t *hit,
+                            epa_store<kBigCap> *pool)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 1) return;
+    // single thread walks the overflowed pairs one by one through the large store
+    for (uint32_t i = 0; i < n; ++i)
+        if (hit[i] == 3) batch_one(sp, i, out, hit, pool[0]);
+}
+} // namespace
+
+size_t epa_big_bytes() { return sizeof(epa_store<kBigCap>); }
+
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
+                gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
+{
+    if (n_pairs) k_gjk<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+}
+
+void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
+                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
+{
+    if (n_hits) k_epa<<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+}
+
+void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
+                    const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,
+                    unsigned long long *counters, hipStream_t st)
+{
+    if (n_ovf)
+        k_epa_big<<<grid(n_ovf, 64), 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g,
+                                                  static_cast<epa_store<kBigCap> *>(pool), counters);
+}
+
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
+                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
+                            manifolds_dev next, uint8_t *events, unsigned long long *counters,
+                            hipStream_t st)
+{
+    if (n_pairs)
+        k_manifold<<<grid(n_pairs), kBlock, 0, st>>>(keys, src, n_pairs, b, g, prev, next, events, counters);
+}
+
+void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,
+                          void *big_pool, hipStream_t st)
+{
+    if (!n) return;
+    k_batch<<<grid(n, kEpaBlock), kEpaBlock, 0, st>>>(sp, n, out, hit);
+    if (big_pool) k_batch_big<<<1, 64, 0, st>>>(sp, n, out, hit, static_cast<epa_store<kBigCap> *>(big_pool));
+}
+
+} // namespace pk

@@ -1,0 +1,426 @@
+// Contact solver: constraint_solver::setup_contacts / solve_constraints
+// (include/physkit/collision/constraint.h:1052-1201, build_contact_jacobian :874-953,
+// jacobian_row :29-87). The reference runs projected Gauss-Seidel sequentially over
+// all contact points. Here the manifolds are coloured so that no two manifolds of a
+// colour share a dynamic body; within a colour every manifold is independent, so one
+// thread per manifold reproduces, bit for bit, a sequential sweep in colour order.
+// Colouring: round c colours every uncoloured manifold whose priority
+// (mix32(p) << 32 | p) is the minimum claim at each of its dynamic bodies.
+#include "pk_access.hpp"
+
+#include <cfloat>
+
+namespace pk
+{
+namespace
+{
+constexpr int kBlock = 256;
+constexpr uint32_t kUncoloured = 0xFFFFFFFEu;
+unsigned grid(size_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+struct rowv
+{
+    v3 Jv, Jwa, Jwb;
+    double Meff, bias, acc;
+};
+struct body_state
+{
+    v3 vel, ang;
+    double im;
+    m3 Iwi;
+    bool stat;
+};
+
+__device__ __forceinline__ double row_jv(const rowv &r, const body_state &A, const body_state &B)
+{ return ((dot(r.Jv, A.vel) + dot(r.Jwa, A.ang)) - dot(r.Jv, B.vel)) + dot(r.Jwb, B.ang); }
+
+__device__ __forceinline__ void row_apply(rowv &r, double lo, double hi, body_state &A, body_state &B) // :49-72
+{
+    double jv = row_jv(r, A, B);
+    double cfp = 0.0 * r.acc;
+    double lambda = -((jv + r.bias) + cfp) * r.Meff;
+    double old = r.acc;
+    double v = old + lambda;
+    r.acc = (v < lo) ? lo : ((hi < v) ? hi : v);
+    double ap = r.acc - old;
+    if (!A.stat)
+    {
+        A.vel = A.vel + (r.Jv * ap) * A.im;
+        A.ang = A.ang + mul(A.Iwi, r.Jwa * ap);
+    }
+    if (!B.stat)
+    {
+        B.vel = B.vel + ((-r.Jv) * ap) * B.im;
+        B.ang = B.ang + mul(B.Iwi, r.Jwb * ap);
+    }
+}
+
+__device__ __forceinline__ void row_warm(const rowv &r, body_state &A, body_state &B) // :74-87
+{
+    if (r.acc == 0.0) return;
+    if (!A.stat)
+    {
+        A.vel = A.vel + (r.Jv * r.acc) * A.im;
+        A.ang = A.ang + mul(A.Iwi, r.Jwa * r.acc);
+    }
+    if (!B.stat)
+    {
+        B.vel = B.vel + ((-r.Jv) * r.acc) * B.im;
+        B.ang = B.ang + mul(B.Iwi, r.Jwb * r.acc);
+    }
+}
+
+__device__ __forceinline__ void friction_block(rowv &t1, rowv &t2, double mu, double nacc, double i11,
+                                               double i12, double i22, body_state &A, body_state &B) // :1131-1187
+{
+    double maxf = mu * nacc;
+    double jv1 = row_jv(t1, A, B);
+    double jv2 = row_jv(t2, A, B);
+    double b1 = jv1 + 0.0;
+    double b2 = jv2 + 0.0;
+    double l1 = -(i11 * b1 + i12 * b2);
+    double l2 = -(i12 * b1 + i22 * b2);
+    double o1 = t1.acc, o2 = t2.acc;
+    double n1 = o1 + l1, n2 = o2 + l2;
+    double mag = hypot_pk(n1, n2);
+    if (mag > maxf)
+    {
+        n1 = n1 * (maxf / mag);
+        n2 = n2 * (maxf / mag);
+    }
+    t1.acc = n1;
+    t2.acc = n2;
+    double a1 = n1 - o1, a2 = n2 - o2;
+    if (!A.stat)
+    {
+        A.vel = A.vel + (t1.Jv * a1 + t2.Jv * a2) * A.im;
+        A.ang = A.ang + mul(A.Iwi, t1.Jwa * a1 + t2.Jwa * a2);
+    }
+    if (!B.stat)
+    {
+        B.vel = B.vel + ((-t1.Jv) * a1 - t2.Jv * a2) * B.im;
+        B.ang = B.ang + mul(B.Iwi, t1.Jwb * a1 + t2.Jwb * a2);
+    }
+}
+
+__device__ __forceinline__ void basis(v3 n, v3 &t1, v3 &t2) // :104-113
+{
+    double sign = copysign(1.0, n.z);
+    double a = -1.0 / (sign + n.z);
+    double b = n.x * n.y * a;
+    t1 = mk(1.0 + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    t2 = mk(b, sign + n.y * n.y * a, -n.y);
+}
+
+__device__ __forceinline__ rowv make_row(const body_state &A, const body_state &B, v3 dir, v3 ra, v3 rb)
+{
+    rowv r;
+    r.Jv = dir;
+    r.Jwa = cross(ra, dir);
+    r.Jwb = -cross(rb, dir);
+    v3 ka = mul(A.Iwi, r.Jwa);
+    v3 kb = mul(B.Iwi, r.Jwb);
+    r.Meff = 1.0 / (((A.im + B.im) + dot(r.Jwa, ka)) + dot(r.Jwb, kb));
+    r.bias = 0.0;
+    r.acc = 0.0;
+    return r;
+}
+
+__device__ __forceinline__ body_state ld_body(const bodies_dev &b, uint32_t i)
+{
+    body_state s;
+    s.vel = ld3(b.vel, b.n, i);
+    s.ang = ld3(b.ang, b.n, i);
+    s.im = b.inv_mass[i];
+    s.Iwi = ld9(b.Iw_inv, b.n, i);
+    s.stat = b.is_static[i] != 0;
+    return s;
+}
+
+// row field planes
+enum
+{
+    F_NJV = 0, F_NJWA = 3, F_NJWB = 6, F_NMEFF = 9, F_NBIAS = 10, F_NACC = 11,
+    F_T1JV = 12, F_T1JWA = 15, F_T1JWB = 18, F_T1MEFF = 21, F_T1ACC = 22,
+    F_T2JV = 23, F_T2JWA = 26, F_T2JWB = 29, F_T2MEFF = 32, F_T2ACC = 33,
+    F_MU = 34, F_I11 = 35, F_I12 = 36, F_I22 = 37
+};
+__device__ __forceinline__ double &rf(const rows_dev &r, int f, uint32_t slot)
+{ return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
+__device__ __forceinline__ v3 rf3(const rows_dev &r, int f, uint32_t slot)
+{ return mk(rf(r, f, slot), rf(r, f + 1, slot), rf(r, f + 2, slot)); }
+__device__ __forceinline__ void wf3(const rows_dev &r, int f, uint32_t slot, v3 v)
+{
+    rf(r, f, slot) = v.x;
+    rf(r, f + 1, slot) = v.y;
+    rf(r, f + 2, slot) = v.z;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_setup(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, manifolds_dev m, rows_dev r, double dt,
+            double thr, uint32_t *colour, unsigned long long *counters)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nrows = 0;
+    if (p < n_pairs)
+    {
+        for (int c = 0; c < 4; ++c) r.valid[4 * p + c] = 0;
+        int cnt = m.count[p];
+        if (cnt > 0)
+        {
+            uint64_t key = keys[p];
+            uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+            body_state A = ld_body(b, ia), B = ld_body(b, ib);
+            v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
+            qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+            double e = smax(b.restitution[ia], b.restitution[ib]);
+            double mu = sqrt(b.friction[ia] * b.friction[ib]);
+            for (int c = 0; c < cnt; ++c)
+            {
+                const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+                size_t s = m.cap;
+                v3 nw = normalized(mk(d[0], d[s], d[2 * s]));
+                v3 la = mk(d[3 * s], d[4 * s], d[5 * s]);
+                v3 lb = mk(d[6 * s], d[7 * s], d[8 * s]);
+                // build_contact_jacobian, :874-953
+                v3 lnb = rotate(conj(qb), nw);
+                v3 ra = rotate(qa, la);
+                v3 rb = rotate(qb, lb);
+                v3 n = rotate(qb, lnb);
+                double pen = dot((pb + rb) - (pa + ra), n);
+                if (pen <= 0.0) continue;
+                rowv nr = make_row(A, B, n, ra, rb);
+                v3 vca = A.vel + cross(A.ang, ra);
+                v3 vcb = B.vel + cross(B.ang, rb);
+                double vrel = dot(vca - vcb, n);
+                double rbias = 0.0;
+                if (vrel < -thr) rbias = e * vrel;
+                double baum = (-0.1 / dt) * smax(0.0, pen - 0.005);
+                nr.bias = smin(baum, rbias);
+                v3 t1, t2;
+                basis(n, t1, t2);
+                rowv r1 = make_row(A, B, t1, ra, rb);
+                rowv r2 = make_row(A, B, t2, ra, rb);
+                // setup_contacts, :1075-1102
+                double m11 = 1.0 / r1.Meff, m22 = 1.0 / r2.Meff;
+                double m12 = dot(r1.Jwa, mul(A.Iwi, r2.Jwa)) + dot(r1.Jwb, mul(B.Iwi, r2.Jwb));
+                double det = m11 * m22 - m12 * m12;
+                double i11, i12, i22;
+                if (det > 0.0)
+                {
+                    i11 = m22 / det;
+                    i22 = m11 / det;
+                    i12 = -m12 / det;
+                }
+                else
+                {
+                    i11 = r1.Meff;
+                    i22 = r2.Meff;
+                    i12 = 0.0;
+                }
+                uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+                wf3(r, F_NJV, slot, nr.Jv);
+                wf3(r, F_NJWA, slot, nr.Jwa);
+                wf3(r, F_NJWB, slot, nr.Jwb);
+                rf(r, F_NMEFF, slot) = nr.Meff;
+                rf(r, F_NBIAS, slot) = nr.bias;
+                rf(r, F_NACC, slot) = d[10 * s];
+                wf3(r, F_T1JV, slot, r1.Jv);
+                wf3(r, F_T1JWA, slot, r1.Jwa);
+                wf3(r, F_T1JWB, slot, r1.Jwb);
+                rf(r, F_T1MEFF, slot) = r1.Meff;
+                rf(r, F_T1ACC, slot) = d[11 * s];
+                wf3(r, F_T2JV, slot, r2.Jv);
+                wf3(r, F_T2JWA, slot, r2.Jwa);
+                wf3(r, F_T2JWB, slot, r2.Jwb);
+                rf(r, F_T2MEFF, slot) = r2.Meff;
+                rf(r, F_T2ACC, slot) = d[12 * s];
+                rf(r, F_MU, slot) = mu;
+                rf(r, F_I11, slot) = i11;
+                rf(r, F_I12, slot) = i12;
+                rf(r, F_I22, slot) = i22;
+                r.valid[slot] = 1;
+                ++nrows;
+            }
+        }
+        colour[p] = nrows ? kUncoloured : kNone;
+    }
+    wave_sum3(&counters[C_ROWS], nrows);
+}
+
+__device__ __forceinline__ uint64_t prio(uint32_t p)
+{ return (static_cast<uint64_t>(mix32(p)) << 32) | p; }
+
+__global__ void k_claim(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, const uint32_t *colour,
+                        unsigned long long *claim)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs || colour[p] != kUncoloured) return;
+    uint64_t key = keys[p];
+    uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    unsigned long long pr = prio(p);
+    if (!b.is_static[ia]) atomicMin(&claim[ia], pr);
+    if (!b.is_static[ib]) atomicMin(&claim[ib], pr);
+}
+
+__global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, uint32_t *colour,
+                         const unsigned long long *claim, uint32_t c, unsigned long long *counters)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool remain = false;
+    if (p < n_pairs && colour[p] == kUncoloured)
+    {
+        uint64_t key = keys[p];
+        uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        unsigned long long pr = prio(p);
+        bool ok = (b.is_static[ia] || claim[ia] == pr) && (b.is_static[ib] || claim[ib] == pr);
+        if (ok)
+            colour[p] = c;
+        else
+            remain = true;
+    }
+    wave_append(&counters[C_REMAIN], remain);
+}
+
+__global__ void k_colour_hist(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint32_t *count)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    uint32_t c = colour[p];
+    if (c < n_colours) atomicAdd(&count[c], 1u);
+}
+
+__global__ void k_colour_scan(const uint32_t *count, uint32_t n_colours, uint32_t *start, uint32_t *cursor)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t s = 0;
+    for (uint32_t c = 0; c < n_colours; ++c)
+    {
+        start[c] = s;
+        cursor[c] = s;
+        s += count[c];
+    }
+    start[n_colours] = s;
+}
+
+__global__ void k_colour_scatter(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint32_t *cursor,
+                                 uint32_t *order)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    uint32_t c = colour[p];
+    if (c < n_colours) order[atomicAdd(&cursor[c], 1u)] = p;
+}
+
+__device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, int f_bias, int f_acc, uint32_t slot)
+{
+    rowv v;
+    v.Jv = rf3(r, f_jv, slot);
+    v.Jwa = rf3(r, f_jv + 3, slot);
+    v.Jwb = rf3(r, f_jv + 6, slot);
+    v.Meff = rf(r, f_meff, slot);
+    v.bias = f_bias >= 0 ? rf(r, f_bias, slot) : 0.0;
+    v.acc = rf(r, f_acc, slot);
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_solve(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows_dev r,
+            int warm)
+{
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= end) return;
+    uint32_t p = order[t];
+    uint64_t key = keys[p];
+    uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    body_state A = ld_body(b, ia), B = ld_body(b, ib);
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+        if (!r.valid[slot]) continue;
+        rowv nr = ld_row(r, F_NJV, F_NMEFF, F_NBIAS, F_NACC, slot);
+        rowv t1 = ld_row(r, F_T1JV, F_T1MEFF, -1, F_T1ACC, slot);
+        rowv t2 = ld_row(r, F_T2JV, F_T2MEFF, -1, F_T2ACC, slot);
+        if (warm)
+        {
+            row_warm(nr, A, B);
+            row_warm(t1, A, B);
+            row_warm(t2, A, B);
+        }
+        else
+        {
+            row_apply(nr, 0.0, DBL_MAX, A, B);
+            friction_block(t1, t2, rf(r, F_MU, slot), nr.acc, rf(r, F_I11, slot), rf(r, F_I12, slot),
+                           rf(r, F_I22, slot), A, B);
+            rf(r, F_NACC, slot) = nr.acc;
+            rf(r, F_T1ACC, slot) = t1.acc;
+            rf(r, F_T2ACC, slot) = t2.acc;
+        }
+    }
+    if (!A.stat)
+    {
+        st3(b.vel, b.n, ia, A.vel);
+        st3(b.ang, b.n, ia, A.ang);
+    }
+    if (!B.stat)
+    {
+        st3(b.vel, b.n, ib, B.vel);
+        st3(b.ang, b.n, ib, B.ang);
+    }
+}
+
+__global__ void k_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r) // :1191-1196
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+        if (!r.valid[slot]) continue;
+        double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+        size_t s = m.cap;
+        d[10 * s] = rf(r, F_NACC, slot);
+        d[11 * s] = rf(r, F_T1ACC, slot);
+        d[12 * s] = rf(r, F_T2ACC, slot);
+    }
+}
+} // namespace
+
+void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                          manifolds_dev m, rows_dev r, double dt, double threshold,
+                          uint32_t *colour, unsigned long long *counters, hipStream_t st)
+{
+    if (n_pairs) k_setup<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, r, dt, threshold, colour, counters);
+}
+void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                         const uint32_t *colour, unsigned long long *claim, hipStream_t st)
+{
+    if (n_pairs) k_claim<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, claim);
+}
+void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                          uint32_t *colour, const unsigned long long *claim, uint32_t c,
+                          unsigned long long *counters, hipStream_t st)
+{
+    if (n_pairs) k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, claim, c, counters);
+}
+void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                          uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,
+                          uint32_t *order, hipStream_t st)
+{
+    if (!n_pairs || !n_colours) return;
+    launch_fill_u32(colour_count, 0u, n_colours, st);
+    k_colour_hist<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, colour_count);
+    k_colour_scan<<<1, 64, 0, st>>>(colour_count, n_colours, colour_start, cursor);
+    k_colour_scatter<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, cursor, order);
+}
+void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
+                         const bodies_dev &b, rows_dev r, bool warm, hipStream_t st)
+{
+    if (end > begin) k_solve<<<grid(end - begin), kBlock, 0, st>>>(keys, order, begin, end, b, r, warm ? 1 : 0);
+}
+void launch_impulse_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r, hipStream_t st)
+{
+    if (n_pairs) k_writeback<<<grid(n_pairs), kBlock, 0, st>>>(n_pairs, m, r);
+}
+
+} // namespace pk

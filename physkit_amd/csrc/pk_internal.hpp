@@ -1,0 +1,192 @@
+// Internal interfaces between the host orchestration (api.cpp) and the kernel
+// translation units. Device pointers only; every array is structure-of-arrays
+// ("planar": component c of element i lives at ptr[c * stride + i]) so that a
+// wavefront reading one component of 64 consecutive elements issues one
+// coalesced 512-byte access.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace pk
+{
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// per-body state, n bodies, body id = index
+struct bodies_dev
+{
+    uint32_t n;
+    double *pos;        // 3 planes
+    double *vel;        // 3
+    double *ang;        // 3
+    double *acc;        // 3 (external force accumulator / m, cleared every step)
+    double *torque;     // 3
+    double *quat;       // 4 planes: w, x, y, z
+    double *mass;       // 1
+    double *inv_mass;   // 1
+    double *I_local;    // 9 planes, row-major element order
+    double *Iinv_local; // 9
+    double *Iw;         // 9
+    double *Iw_inv;     // 9
+    double *restitution;
+    double *friction;
+    double *fat; // 6 planes: min xyz, max xyz
+    uint32_t *shape;
+    uint8_t *is_static;
+    uint8_t *moved;
+};
+
+struct shapes_dev
+{
+    uint32_t n;
+    const uint32_t *off; // n+1 vertex offsets
+    const double *verts; // xyz interleaved (a support scan reads one shape's vertices in order)
+    double *local_box;   // 6 planes over shapes
+};
+
+// manifold cache: up to 4 contacts x 13 fields per pair, planar with stride `cap`
+// field f of contact c of pair p at data[(c * 13 + f) * cap + p]
+struct manifolds_dev
+{
+    uint32_t cap;
+    int32_t *count;
+    double *data;
+};
+constexpr int kMF = 13; // normal3 local_a3 local_b3 depth ln lt1 lt2
+
+// device counters (one 64-bit slot each)
+enum counter_slot
+{
+    C_MOVED = 0,
+    C_CAND = 1,
+    C_COMBINED = 2,
+    C_PAIRS = 3,
+    C_HITS = 4,
+    C_CONTACTS = 5,
+    C_ROWS = 6,
+    C_REMAIN = 7,
+    C_EPA_OVF = 8,
+    C_EPA_BIG_FAIL = 9,
+    C_EVENTS = 10,
+    C_TREE_FAIL = 11,
+    C_NCOUNTERS = 16
+};
+
+// ---------------- k_body.hip ----------------
+void launch_body_init(const bodies_dev &b, const shapes_dev &s, hipStream_t st);
+void launch_shape_boxes(const shapes_dev &s, hipStream_t st);
+void launch_integrate_vel(const bodies_dev &b, const shapes_dev &s, double gx, double gy, double gz,
+                          double dt, unsigned long long *counters, hipStream_t st);
+void launch_integrate_pos(const bodies_dev &b, double dt, hipStream_t st);
+void launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t st);
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+
+// ---------------- k_sort.hip ----------------
+// LSD radix sort of (key, value) pairs on 8-bit digits at the given bit shifts
+// (ascending). Buffers ping-pong; returns true when the result is in (k_alt, v_alt).
+struct sort_scratch
+{
+    uint32_t *hist = nullptr; // 256 * max_blocks
+    size_t hist_cap = 0;
+};
+bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, uint32_t n,
+                    const int *shifts, int n_shifts, sort_scratch &sc, hipStream_t st);
+size_t radix_hist_entries(uint32_t n);
+// exclusive scan of n uint32 flags into out, total written to *total (device)
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *block_sums,
+                        uint32_t *total, hipStream_t st);
+size_t scan_block_sums(uint32_t n);
+
+// ---------------- k_broad.hip ----------------
+struct lbvh_dev
+{
+    uint32_t n; // leaves
+    uint64_t *morton;
+    uint32_t *ids;
+    uint64_t *morton_alt;
+    uint32_t *ids_alt;
+    int32_t *left;       // n-1 internal
+    int32_t *right;      // n-1
+    uint32_t *parent;    // 2n-1 (unified node index: internal [0,n-1), leaf n-1+i)
+    uint32_t *escape;    // 2n-1
+    double *node_box;    // 6 per node, node-major (one node test reads 48 contiguous bytes)
+    uint32_t *leaf_body; // n: body id of sorted leaf i
+    uint32_t *visit;     // n-1 refit arrival counters
+    unsigned long long *scene; // 6 ordered-integer bounds of leaf centroids
+};
+void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, sort_scratch &sc, hipStream_t st);
+// candidate pairs of moved leaves appended (key, kNone) into out; the append cursor
+// is counters[C_COMBINED], a malformed tree is reported in counters[C_TREE_FAIL]
+void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, bool all_moved, uint64_t *out_keys,
+                       uint32_t *out_vals, uint32_t cap, unsigned long long *counters,
+                       hipStream_t st);
+// previous pairs whose leaves still overlap appended (key, prev index)
+void launch_prev_filter(const uint64_t *prev_keys, uint32_t n_prev, const bodies_dev &b,
+                        uint64_t *out_keys, uint32_t *out_vals, uint32_t cap,
+                        unsigned long long *counter, hipStream_t st);
+void launch_dedupe_flags(const uint64_t *keys, const uint32_t *vals, uint32_t n, uint32_t *flag,
+                         uint32_t *src, hipStream_t st);
+void launch_compact_pairs(const uint64_t *keys, const uint32_t *src, const uint32_t *flag,
+                          const uint32_t *pos, uint32_t n, uint64_t *out_keys, uint32_t *out_src,
+                          hipStream_t st);
+
+// ---------------- k_narrow.hip ----------------
+struct gjk_out_dev
+{
+    uint32_t *hit_list;   // compacted pair indices whose GJK found an intersection
+    double *simplex;      // per hit-list entry: n, then 4 x 9 doubles (p, pa, pb), planar stride cap
+    uint32_t cap;
+    double *info;         // per pair: normal3 wa3 wb3 depth, planar stride pair_cap
+    uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
+    uint32_t pair_cap;
+};
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
+                gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
+void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
+                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
+size_t epa_big_bytes();
+void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
+                    const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,
+                    unsigned long long *counters, hipStream_t st);
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
+                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
+                            manifolds_dev next, uint8_t *events, unsigned long long *counters,
+                            hipStream_t st);
+
+// standalone narrowphase over shape pairs (pkg_gjk_epa_batch)
+struct shape_pair_dev
+{
+    const int32_t *kind_a, *kind_b;
+    const double *prm_a, *prm_b; // 10 doubles per shape
+    const uint32_t *vbeg_a, *vend_a, *vbeg_b, *vend_b;
+    const double *verts;
+};
+void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,
+                          void *big_pool, hipStream_t st);
+
+// ---------------- k_solver.hip ----------------
+struct rows_dev
+{
+    uint32_t cap4;   // 4 * pair capacity
+    double *d;       // kRowF planes of cap4
+    uint8_t *valid;  // cap4
+};
+constexpr int kRowF = 38;
+void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                          manifolds_dev m, rows_dev r, double dt, double threshold,
+                          uint32_t *colour, unsigned long long *counters, hipStream_t st);
+void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                         const uint32_t *colour, unsigned long long *claim, hipStream_t st);
+void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                          uint32_t *colour, const unsigned long long *claim, uint32_t c,
+                          unsigned long long *counters, hipStream_t st);
+void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                          uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,
+                          uint32_t *order, hipStream_t st);
+void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
+                         const bodies_dev &b, rows_dev r, bool warm, hipStream_t st);
+void launch_impulse_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r, hipStream_t st);
+
+} // namespace pk
