@@ -43,6 +43,11 @@ typedef struct pkg_ctx pkg_ctx;
 #define PKG_FLAG_ALL_MOVED 1u    /* every dynamic leaf counts as moved each step       */
 #define PKG_FLAG_NO_SOLVER 2u    /* skip contact setup and solve                        */
 #define PKG_FLAG_NO_INTEGRATE 4u /* skip velocity/position integration and leaf update */
+/* Discard an EPA result whose normal has zero length or whose witness points or depth
+ * are not finite, as if GJK had found no contact. Not in the reference: its EPA can
+ * return such a result from a face spanned by three collinear support points
+ * (collision.cpp:284-285, 442-445), which then propagates NaN through the solver. */
+#define PKG_FLAG_EPA_GUARD 8u
 
 /* Mirrors world_desc (include/physkit/core/world.h:23-49): gravity and solver
  * iterations; defaults there are (0,-9.81,0) and 10. */

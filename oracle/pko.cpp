@@ -747,6 +747,14 @@ void world::stage_narrow() // collision_phases.h:244-320
         auto col = gjk_epa(ia, ib, &st);
         if (st.hit) counters.gjk_hits++;
         std::optional<contact> nc;
+        if (col && epa_guard)
+        {
+            const collision_info &ci = *col;
+            bool ok = sqn(ci.normal) > 0.0 && std::isfinite(ci.world_a.x) && std::isfinite(ci.world_a.y) &&
+                      std::isfinite(ci.world_a.z) && std::isfinite(ci.world_b.x) && std::isfinite(ci.world_b.y) &&
+                      std::isfinite(ci.world_b.z) && std::isfinite(ci.depth);
+            if (!ok) col.reset();
+        }
         if (col)
         {
             contact k;

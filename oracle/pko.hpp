@@ -199,6 +199,7 @@ struct world
     bool all_moved = false;
     bool run_solver = true;
     bool run_integrate = true;
+    bool epa_guard = false; // discard zero-normal / non-finite EPA results (not in the reference)
 
     std::vector<mesh> shapes;
     std::vector<body> bodies;

@@ -40,7 +40,7 @@ extern "C"
 {
 
 // flags: 1 = every dynamic leaf counts as moved, 2 = skip the contact solver,
-//        4 = skip velocity/position integration
+//        4 = skip velocity/position integration, 8 = discard degenerate EPA results
 void *pko_world_new(const double *gravity, int iterations, int solver_mode, int flags)
 {
     auto *w = new world();
@@ -50,6 +50,7 @@ void *pko_world_new(const double *gravity, int iterations, int solver_mode, int 
     w->all_moved = (flags & 1) != 0;
     w->run_solver = (flags & 2) == 0;
     w->run_integrate = (flags & 4) == 0;
+    w->epa_guard = (flags & 8) != 0;
     w->finalize_world();
     return w;
 }

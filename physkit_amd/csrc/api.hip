@@ -342,7 +342,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
         }
         record(ev, 5, st);
-        launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p, counters.p, st);
+        launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p,
+                               (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, counters.p, st);
     }
     else
     {

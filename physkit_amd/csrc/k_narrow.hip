@@ -249,7 +249,8 @@ __device__ __forceinline__ void add_contact(mani &m, const contact &k) // :124-1
 
 __global__ void __launch_bounds__(kBlock)
     k_manifold(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, bodies_dev b, gjk_out_dev g,
-               manifolds_dev prev, manifolds_dev next, uint8_t *events, unsigned long long *counters)
+               manifolds_dev prev, manifolds_dev next, uint8_t *events, int epa_guard,
+               unsigned long long *counters)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t count = 0;
@@ -277,6 +278,12 @@ __global__ void __launch_bounds__(kBlock)
             nc.ln = 0.0;
             nc.lt1 = 0.0;
             nc.lt2 = 0.0;
+            if (epa_guard)
+            {
+                bool ok = sqn(nc.n) > 0.0 && isfinite(wa.x) && isfinite(wa.y) && isfinite(wa.z) &&
+                          isfinite(wb.x) && isfinite(wb.y) && isfinite(wb.z) && isfinite(nc.depth);
+                has_new = ok;
+            }
         }
         uint32_t sp = src[p];
         int oldn = sp != kNone ? prev.count[sp] : 0;
@@ -413,11 +420,12 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
 
 void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
                             const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, unsigned long long *counters,
-                            hipStream_t st)
+                            manifolds_dev next, uint8_t *events, bool epa_guard,
+                            unsigned long long *counters, hipStream_t st)
 {
     if (n_pairs)
-        k_manifold<<<grid(n_pairs), kBlock, 0, st>>>(keys, src, n_pairs, b, g, prev, next, events, counters);
+        k_manifold<<<grid(n_pairs), kBlock, 0, st>>>(keys, src, n_pairs, b, g, prev, next, events,
+                                                     epa_guard ? 1 : 0, counters);
 }
 
 void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,

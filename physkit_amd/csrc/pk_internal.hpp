@@ -152,8 +152,8 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                     unsigned long long *counters, hipStream_t st);
 void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
                             const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, unsigned long long *counters,
-                            hipStream_t st);
+                            manifolds_dev next, uint8_t *events, bool epa_guard,
+                            unsigned long long *counters, hipStream_t st);
 
 // standalone narrowphase over shape pairs (pkg_gjk_epa_batch)
 struct shape_pair_dev
