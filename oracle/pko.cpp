@@ -679,9 +679,18 @@ void world::stage_integrate_vel(double dt) // world.cpp:22-34
 //                && (active(a,b) || moved_a || moved_b)
 void world::stage_pairs()
 {
-    const std::size_t n = bodies.size();
-    std::vector<uint32_t> order(n);
-    for (std::size_t i = 0; i < n; ++i) order[i] = static_cast<uint32_t>(i);
+    // a box with a NaN coordinate fails every comparison of aabb::intersects, so it
+    // overlaps nothing; leaving it out keeps the sort below a strict weak ordering
+    auto has_nan = [](const aabb &b)
+    {
+        return std::isnan(b.min.x) || std::isnan(b.min.y) || std::isnan(b.min.z) || std::isnan(b.max.x) ||
+               std::isnan(b.max.y) || std::isnan(b.max.z);
+    };
+    std::vector<uint32_t> order;
+    order.reserve(bodies.size());
+    for (std::size_t i = 0; i < bodies.size(); ++i)
+        if (!has_nan(bodies[i].fat)) order.push_back(static_cast<uint32_t>(i));
+    const std::size_t n = order.size();
     std::sort(order.begin(), order.end(),
               [&](uint32_t x, uint32_t y)
               {
