@@ -1011,6 +1011,8 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
     // processing order over contributing manifolds
     std::vector<std::size_t> order;
     colours.assign(M, 0);
+    colour_keys.resize(M);
+    for (std::size_t i = 0; i < M; ++i) colour_keys[i] = active[midx[i]];
     if (solver_mode == 0)
     {
         for (std::size_t i = 0; i < M; ++i) order.push_back(i);

@@ -207,6 +207,7 @@ struct world
     std::vector<manifold> manifolds;  // parallel to `active`
     std::vector<event> events;        // begin/end events of the last step, manifold order
     std::vector<uint32_t> colours;    // colour of each manifold that produced rows (last step)
+    std::vector<uint64_t> colour_keys; // pair key of each of those manifolds
     step_counters counters;
 
     void finalize_world();            // |g| as world.h:383 computes it

@@ -202,11 +202,12 @@ void pko_get_counters(void *h, uint64_t *out)
 }
 
 // colours of the manifolds that produced solver rows in the last step
-size_t pko_get_colours(void *h, uint32_t *out, size_t cap)
+size_t pko_get_colours(void *h, uint32_t *out, uint64_t *keys, size_t cap)
 {
     auto *w = static_cast<world *>(h);
     size_t n = std::min(cap, w->colours.size());
     std::copy(w->colours.begin(), w->colours.begin() + static_cast<std::ptrdiff_t>(n), out);
+    if (keys) std::copy(w->colour_keys.begin(), w->colour_keys.begin() + static_cast<std::ptrdiff_t>(n), keys);
     return w->colours.size();
 }
 

@@ -52,7 +52,7 @@ def lib():
         L.pko_get_events.argtypes = [C.c_void_p, U64, I32, C.c_size_t]
         L.pko_get_counters.argtypes = [C.c_void_p, U64]
         L.pko_get_colours.restype = C.c_size_t
-        L.pko_get_colours.argtypes = [C.c_void_p, U32, C.c_size_t]
+        L.pko_get_colours.argtypes = [C.c_void_p, U32, U64, C.c_size_t]
         L.pko_gjk_epa.restype = C.c_int
         L.pko_gjk_epa.argtypes = [C.c_int, D, D, C.c_int, C.c_int, D, D, C.c_int, D, I32]
         L.pko_mesh_box.restype = C.c_int
@@ -245,9 +245,10 @@ class OracleWorld:
         return dict(zip(["moved", "pairs", "gjk_hits", "contacts", "rows", "colours"],
                         [int(x) for x in out]))
 
-    def colours(self):
-        n = self.L.pko_get_colours(self.h, None, 0)
+    def colours(self, with_keys=False):
+        n = self.L.pko_get_colours(self.h, None, None, 0)
         out = np.zeros(n, dtype=np.uint32)
+        keys = np.zeros(n, dtype=np.uint64)
         if n:
-            self.L.pko_get_colours(self.h, out.ctypes.data_as(U32), n)
-        return out
+            self.L.pko_get_colours(self.h, out.ctypes.data_as(U32), keys.ctypes.data_as(U64), n)
+        return (out, keys) if with_keys else out
