@@ -31,10 +31,15 @@ void hip_check(hipError_t e, const char *what)
     if (e != hipSuccess) throw pk_error(PKG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// device buffer owned by the context; freed when the context is destroyed
 template <class T> struct dbuf
 {
     T *p = nullptr;
     size_t cap = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
     void reserve(size_t n) // contents are not kept
     {
         if (n <= cap) return;
@@ -101,13 +106,15 @@ struct pkg_ctx
 
     // broadphase
     dbuf<uint64_t> morton, morton_alt;
-    dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, visit;
+    dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, visit, range_first, range_last, lastf, defer;
     dbuf<int32_t> left, right;
-    dbuf<double> node_box;
+    dbuf<float4> inode_lo, inode_hi;
+    dbuf<double> leaf_box;
+    dbuf<uint8_t> leaf_moved, covered;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
-    dbuf<uint64_t> comb_keys, comb_keys_alt;
-    dbuf<uint32_t> comb_vals, comb_vals_alt, flag, fpos, bsums, total;
+    dbuf<uint64_t> comb_keys, comb_keys_alt, akeys;
+    dbuf<uint32_t> comb_vals, comb_vals_alt, flag, fpos, bsums, total, srcb, asrc;
 
     // pair list and manifold cache, double-buffered (cur = last step's result)
     dbuf<uint64_t> keys[2];
@@ -224,13 +231,20 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     t.ids_alt = ids_alt.p;
     t.left = left.p;
     t.right = right.p;
+    t.first = range_first.p;
+    t.last = range_last.p;
+    t.lastf = lastf.p;
+    t.inode_lo = inode_lo.p;
+    t.inode_hi = inode_hi.p;
     t.parent = parent.p;
     t.escape = escape.p;
-    t.node_box = node_box.p;
+    t.leaf_box = leaf_box.p;
     t.leaf_body = leaf_body.p;
+    t.leaf_moved = leaf_moved.p;
     t.visit = visit.p;
+    t.defer = defer.p;
     t.scene = scene.p;
-    launch_lbvh_build(t, b, sc, st);
+    launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
     record(ev, 2, st);
     if (t.morton != morton.p)
     {
@@ -239,16 +253,15 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     }
     const uint32_t Pprev = P[cur];
     const int nxt = 1 - cur;
+    // candidates: pairs with at least one moved leaf, each once
     uint32_t M = 0;
     for (int attempt = 0;; ++attempt)
     {
-        size_t want = std::max<size_t>(comb_keys.cap, static_cast<size_t>(Pprev) + 4 * static_cast<size_t>(n) + 1024);
+        size_t want = std::max<size_t>(comb_keys.cap, 4 * static_cast<size_t>(n) + 1024);
         comb_keys.reserve(want);
-        comb_vals.reserve(want);
         uint32_t cap = static_cast<uint32_t>(std::min<size_t>(comb_keys.cap, 0xFFFFFFF0u));
         reset_counter(C_COMBINED);
-        launch_prev_filter(keys[cur].p, Pprev, b, comb_keys.p, comb_vals.p, cap, counters.p + C_COMBINED, st);
-        launch_pair_query(t, b, all_moved, comb_keys.p, comb_vals.p, cap, counters.p, st);
+        launch_pair_query(t, b, comb_keys.p, cap, counters.p, st);
         unsigned long long cnt = read_counter(C_COMBINED);
         if (read_counter(C_TREE_FAIL)) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
         if (cnt <= cap)
@@ -258,16 +271,13 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         }
         if (cnt > 0xFFFFFFF0ull || attempt > 2)
             throw pk_error(PKG_OVERFLOW_PAIRS, "candidate pair count exceeds 32-bit indexing");
-        size_t grow = static_cast<size_t>(cnt + cnt / 4 + 1024);
         comb_keys.release();
-        comb_vals.release();
-        comb_keys.reserve(grow);
-        comb_vals.reserve(grow);
+        comb_keys.reserve(static_cast<size_t>(cnt + cnt / 4 + 1024));
     }
+    comb_vals.reserve(M);
     comb_keys_alt.reserve(M);
     comb_vals_alt.reserve(M);
     uint64_t *ck = comb_keys.p;
-    uint32_t *cv = comb_vals.p;
     if (M > 1)
     {
         int bits = id_bits(n);
@@ -276,30 +286,36 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
         for (int sft = 32; sft < 32 + bits; sft += 8) shifts[ns++] = sft;
         if (radix_sort_u64(comb_keys.p, comb_vals.p, comb_keys_alt.p, comb_vals_alt.p, M, shifts, ns, sc, st))
-        {
             ck = comb_keys_alt.p;
-            cv = comb_vals_alt.p;
-        }
     }
-    flag.reserve(M);
-    fpos.reserve(M);
-    bsums.reserve(scan_block_sums(M));
-    total.reserve(1);
-    src.reserve(std::max<uint32_t>(M, 1));
-    keys[nxt].reserve(std::max<uint32_t>(M, 1));
-    uint32_t Pn = 0;
-    if (M > 0)
+    // carry-over: a candidate found in the previous sorted list keeps its manifold;
+    // previous pairs no candidate covers stay while their leaves overlap
+    srcb.reserve(std::max<uint32_t>(M, 1));
+    covered.reserve(static_cast<size_t>(Pprev) + 4);
+    if (Pprev) launch_fill_u32(reinterpret_cast<uint32_t *>(covered.p), 0u, (Pprev + 3) / 4, st);
+    launch_pair_match(ck, M, keys[cur].p, Pprev, srcb.p, covered.p, st);
+    uint32_t K = 0;
+    if (!all_moved && Pprev)
     {
-        // per-key source index staged in the value buffer the sort left free
-        uint32_t *staged = (cv == comb_vals_alt.p) ? comb_vals.p : comb_vals_alt.p;
-        launch_dedupe_flags(ck, cv, M, flag.p, staged, st);
-        exclusive_scan_u32(flag.p, fpos.p, M, bsums.p, total.p, st);
-        launch_compact_pairs(ck, staged, flag.p, fpos.p, M, keys[nxt].p, src.p, st);
+        flag.reserve(Pprev);
+        fpos.reserve(Pprev);
+        bsums.reserve(scan_block_sums(Pprev));
+        total.reserve(1);
+        launch_prev_keep(keys[cur].p, Pprev, covered.p, b, flag.p, st);
+        exclusive_scan_u32(flag.p, fpos.p, Pprev, bsums.p, total.p, st);
         uint32_t h_total = 0;
         hip_check(hipMemcpyAsync(&h_total, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total copy");
         hip_check(hipStreamSynchronize(st), "stream sync");
-        Pn = h_total;
+        K = h_total;
+        akeys.reserve(std::max<uint32_t>(K, 1));
+        asrc.reserve(std::max<uint32_t>(K, 1));
+        launch_prev_compact(keys[cur].p, Pprev, flag.p, fpos.p, akeys.p, asrc.p, st);
     }
+    const uint32_t Pn = M + K;
+    keys[nxt].reserve(std::max<uint32_t>(Pn, 1));
+    src.reserve(std::max<uint32_t>(Pn, 1));
+    launch_merge(ck, srcb.p, M, akeys.p, K, keys[nxt].p, src.p, st);
+    launch_merge(akeys.p, asrc.p, K, ck, M, keys[nxt].p, src.p, st);
     record(ev, 3, st);
 
     // ---- 3. narrowphase (collision_phases.h:244-320)
@@ -499,23 +515,7 @@ pkg_status pkg_destroy(pkg_ctx *ctx)
 {
     if (!ctx) return PKG_INVALID_ARG;
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
-    // every dbuf member frees in release(); the ctx owns nothing else on the device
-    ctx->shape_off.release(); ctx->verts.release(); ctx->shape_box.release();
-    for (auto *d : {&ctx->pos, &ctx->vel, &ctx->ang, &ctx->acc, &ctx->torque, &ctx->quat, &ctx->mass,
-                    &ctx->inv_mass, &ctx->I_local, &ctx->Iinv_local, &ctx->Iw, &ctx->Iw_inv, &ctx->rest,
-                    &ctx->fric, &ctx->fat, &ctx->node_box, &ctx->simplex, &ctx->info, &ctx->rows,
-                    &ctx->mdata[0], &ctx->mdata[1]})
-        d->release();
-    for (auto *d : {&ctx->shape, &ctx->ids, &ctx->ids_alt, &ctx->parent, &ctx->escape, &ctx->leaf_body,
-                    &ctx->visit, &ctx->comb_vals, &ctx->comb_vals_alt, &ctx->flag, &ctx->fpos, &ctx->bsums,
-                    &ctx->total, &ctx->src, &ctx->hit_list, &ctx->ovf_list, &ctx->colour, &ctx->colour_count,
-                    &ctx->colour_start, &ctx->cursor, &ctx->order})
-        d->release();
-    for (auto *d : {&ctx->is_static, &ctx->moved, &ctx->events, &ctx->status, &ctx->row_valid}) d->release();
-    for (auto *d : {&ctx->morton, &ctx->morton_alt, &ctx->comb_keys, &ctx->comb_keys_alt, &ctx->keys[0], &ctx->keys[1]})
-        d->release();
-    ctx->left.release(); ctx->right.release(); ctx->mcount[0].release(); ctx->mcount[1].release();
-    ctx->counters.release(); ctx->scene.release(); ctx->claim.release(); ctx->big_pool.release();
+    // the dbuf members free their device memory in their destructors
     if (ctx->sc.hist) (void)hipFree(ctx->sc.hist);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -595,8 +595,13 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->parent.reserve(2 * static_cast<size_t>(n));
         ctx->escape.reserve(2 * static_cast<size_t>(n));
         ctx->leaf_body.reserve(n);
+        ctx->leaf_moved.reserve(n);
         ctx->visit.reserve(n);
-        ctx->node_box.reserve(12 * static_cast<size_t>(n));
+        for (auto *b : {&ctx->range_first, &ctx->range_last, &ctx->lastf}) b->reserve(n);
+        ctx->defer.reserve(2 * static_cast<size_t>(n));
+        ctx->inode_lo.reserve(n);
+        ctx->inode_hi.reserve(n);
+        ctx->leaf_box.reserve(6 * static_cast<size_t>(n));
         launch_body_init(ctx->B(), ctx->S(), st);
         ctx->P[0] = ctx->P[1] = 0;
         ctx->cur = 0;

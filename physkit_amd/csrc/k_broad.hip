@@ -6,8 +6,10 @@
 // The reference keeps two incremental SAH trees (src/bvh.cpp:235-514); the pair SET
 // does not depend on tree shape, so the device rebuilds a linear BVH every step:
 //   63-bit Morton keys of leaf centroids -> radix sort -> Karras hierarchy ->
-//   bottom-up refit with arrival counters -> escape indices for stackless traversal.
-// Node boxes are exact FP64 unions, so every overlap test is the reference's test.
+//   two-phase bottom-up refit -> escape indices for stackless traversal.
+// Internal nodes hold FP32 boxes rounded outward (conservative, 32 bytes per node
+// with the child and escape links); every leaf test uses the exact FP64 leaf box, so
+// each overlap decision is the reference's aabb::intersects.
 #include "pk_access.hpp"
 
 namespace pk
@@ -15,7 +17,8 @@ namespace pk
 namespace
 {
 constexpr int kBlock = 256;
-unsigned grid(size_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+constexpr int kChunk = 1024; // leaves per refit workgroup (phase 1)
+unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 __device__ __forceinline__ unsigned long long ord(double x)
 {
@@ -26,6 +29,19 @@ __device__ __forceinline__ double unord(unsigned long long o)
 {
     unsigned long long u = (o >> 63) ? (o & 0x7fffffffffffffffull) : ~o;
     return __longlong_as_double(static_cast<long long>(u));
+}
+// float bounds of a double, rounded toward -inf / +inf
+__device__ __forceinline__ float f_down(double x)
+{
+    float f = static_cast<float>(x);
+    if (static_cast<double>(f) > x) f = nextafterf(f, -__builtin_inff());
+    return f;
+}
+__device__ __forceinline__ float f_up(double x)
+{
+    float f = static_cast<float>(x);
+    if (static_cast<double>(f) < x) f = nextafterf(f, __builtin_inff());
+    return f;
 }
 
 __global__ void k_scene_init(unsigned long long *scene)
@@ -46,34 +62,47 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
-__global__ void k_scene_bounds(bodies_dev b, unsigned long long *scene)
+// centroid bounds: grid-stride per thread, then wavefront and workgroup reductions,
+// then one ordered-integer atomic per workgroup and component
+__global__ void __launch_bounds__(kBlock) k_scene_bounds(bodies_dev b, unsigned long long *scene)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[2][3][kBlock / 64];
     double c[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
     double d[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-    if (i < b.n)
-    {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x)
         for (int k = 0; k < 3; ++k)
         {
             double v = (b.fat[static_cast<size_t>(k) * b.n + i] + b.fat[static_cast<size_t>(k + 3) * b.n + i]) * 0.5;
-            c[k] = v;
-            d[k] = v;
+            c[k] = fmin(c[k], v);
+            d[k] = fmax(d[k], v);
         }
-    }
+    int w = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k)
     {
         c[k] = wave_min(c[k]);
         d[k] = wave_max(d[k]);
-    }
-    if (__lane_id() == 0)
-        for (int k = 0; k < 3; ++k)
+        if (__lane_id() == 0)
         {
-            if (c[k] <= d[k])
-            {
-                atomicMin(&scene[k], ord(c[k]));
-                atomicMax(&scene[3 + k], ord(d[k]));
-            }
+            red[0][k][w] = c[k];
+            red[1][k][w] = d[k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+    {
+        int k = threadIdx.x;
+        double lo = red[0][k][0], hi = red[1][k][0];
+        for (int j = 1; j < kBlock / 64; ++j)
+        {
+            lo = fmin(lo, red[0][k][j]);
+            hi = fmax(hi, red[1][k][j]);
+        }
+        if (lo <= hi)
+        {
+            atomicMin(&scene[k], ord(lo));
+            atomicMax(&scene[3 + k], ord(hi));
+        }
+    }
 }
 
 __device__ __forceinline__ uint64_t split3(uint64_t x)
@@ -116,7 +145,7 @@ __device__ __forceinline__ int delta(const uint64_t *k, int n, int i, int j)
     return __clzll(static_cast<long long>(a ^ c));
 }
 
-__global__ void k_build_internal(const uint64_t *k, int n, int32_t *left, int32_t *right, uint32_t *parent)
+__global__ void k_build_internal(const uint64_t *k, int n, lbvh_dev t)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -125,111 +154,187 @@ __global__ void k_build_internal(const uint64_t *k, int n, int32_t *left, int32_
     int lmax = 2;
     while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
     int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    for (int s = lmax / 2; s >= 1; s /= 2)
+        if (delta(k, n, i, i + (l + s) * d) > dmin) l += s;
     int j = i + l * d;
     int dnode = delta(k, n, i, j);
     int s = 0;
     int div = 2;
     while (true)
     {
-        int t = (l + div - 1) / div;
-        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
-        if (t == 1) break;
+        int st = (l + div - 1) / div;
+        if (delta(k, n, i, i + (s + st) * d) > dnode) s += st;
+        if (st == 1) break;
         div *= 2;
     }
     int gamma = i + s * d + (d < 0 ? d : 0);
     int lo = i < j ? i : j, hi = i < j ? j : i;
     int lc = (lo == gamma) ? (n - 1 + gamma) : gamma;
     int rc = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
-    left[i] = lc;
-    right[i] = rc;
-    parent[lc] = static_cast<uint32_t>(i);
-    parent[rc] = static_cast<uint32_t>(i);
-    if (i == 0) parent[0] = kNone;
+    t.left[i] = lc;
+    t.right[i] = rc;
+    t.first[i] = static_cast<uint32_t>(lo);
+    t.last[i] = static_cast<uint32_t>(hi);
+    t.parent[lc] = static_cast<uint32_t>(i);
+    t.parent[rc] = static_cast<uint32_t>(i);
+    if (i == 0) t.parent[0] = kNone;
 }
 
-__global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, uint32_t n, double *node_box,
-                              uint32_t *leaf_body, uint32_t *parent)
+__global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, lbvh_dev t)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = t.n;
     if (i >= n) return;
     uint32_t a = ids[i];
-    leaf_body[i] = a;
-    size_t node = static_cast<size_t>(n - 1 + i);
-    for (int k = 0; k < 6; ++k) node_box[node * 6 + k] = b.fat[static_cast<size_t>(k) * b.n + a];
-    if (n == 1) parent[0] = kNone;
-}
-
-__global__ void k_refit(uint32_t n, const int32_t *left, const int32_t *right, const uint32_t *parent,
-                        double *node_box, uint32_t *visit)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t node = parent[n - 1 + i];
-    for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
-    {
-        __threadfence();
-        uint32_t old = atomicAdd(&visit[node], 1u);
-        __threadfence();
-        if (old == 0) return; // the sibling subtree is not finished yet
-        const double *lb = node_box + static_cast<size_t>(left[node]) * 6;
-        const double *rb = node_box + static_cast<size_t>(right[node]) * 6;
-        double *o = node_box + static_cast<size_t>(node) * 6;
-        for (int k = 0; k < 3; ++k) o[k] = smin(lb[k], rb[k]);
-        for (int k = 3; k < 6; ++k) o[k] = smax(lb[k], rb[k]);
-        node = parent[node];
-    }
+    t.leaf_body[i] = a;
+    t.leaf_moved[i] = (!b.is_static[a] && (all_moved || b.moved[a])) ? 1 : 0;
+    for (int k = 0; k < 6; ++k) t.leaf_box[static_cast<size_t>(i) * 6 + k] = b.fat[static_cast<size_t>(k) * b.n + a];
+    if (n == 1) t.parent[0] = kNone;
 }
 
 // next node in left-first depth-first order after the subtree of u
-__global__ void k_escape(uint32_t n, const int32_t *left, const int32_t *right, const uint32_t *parent,
-                         uint32_t *escape)
+__global__ void k_escape(lbvh_dev t)
 {
+    uint32_t n = t.n;
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= 2 * n - 1) return;
     uint32_t x = u, esc = kNone;
     for (uint32_t guard = 0; guard < 2 * n; ++guard)
     {
-        uint32_t p = parent[x];
+        uint32_t p = t.parent[x];
         if (p == kNone) break;
-        if (static_cast<uint32_t>(left[p]) == x)
+        if (static_cast<uint32_t>(t.left[p]) == x)
         {
-            esc = static_cast<uint32_t>(right[p]);
+            esc = static_cast<uint32_t>(t.right[p]);
             break;
         }
         x = p;
     }
-    escape[u] = esc;
+    t.escape[u] = esc;
 }
 
-__device__ __forceinline__ box3 node_box_at(const double *nb, uint32_t node)
+struct fbox
 {
-    const double *p = nb + static_cast<size_t>(node) * 6;
-    box3 b;
-    b.min = mk(p[0], p[1], p[2]);
-    b.max = mk(p[3], p[4], p[5]);
-    return b;
+    float lo[3], hi[3];
+    uint32_t moved; // 1 when every leaf below is a moved leaf
+};
+
+__device__ __forceinline__ fbox child_box(const lbvh_dev &t, uint32_t c)
+{
+    fbox f;
+    uint32_t n = t.n;
+    if (c >= n - 1)
+    {
+        uint32_t j = c - (n - 1);
+        const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
+        for (int k = 0; k < 3; ++k)
+        {
+            f.lo[k] = f_down(p[k]);
+            f.hi[k] = f_up(p[3 + k]);
+        }
+        f.moved = t.leaf_moved[j];
+    }
+    else
+    {
+        float4 a = t.inode_lo[c], b = t.inode_hi[c];
+        f.lo[0] = a.x;
+        f.lo[1] = a.y;
+        f.lo[2] = a.z;
+        f.hi[0] = b.x;
+        f.hi[1] = b.y;
+        f.hi[2] = b.z;
+        f.moved = t.lastf[c] >> 31;
+    }
+    return f;
 }
 
-// one thread per sorted leaf i; a moved dynamic leaf emits its pair with leaf j
-// when j > i, or when j is not a moved leaf (j then never queries): each pair once.
-__global__ void k_pair_query(lbvh_dev t, bodies_dev b, int all_moved, uint64_t *out_keys,
-                             uint32_t *out_vals, uint32_t cap, unsigned long long *counters)
+__device__ __forceinline__ void finish_node(const lbvh_dev &t, uint32_t node)
+{
+    uint32_t l = static_cast<uint32_t>(t.left[node]), r = static_cast<uint32_t>(t.right[node]);
+    fbox a = child_box(t, l), b = child_box(t, r);
+    t.inode_lo[node] = make_float4(fminf(a.lo[0], b.lo[0]), fminf(a.lo[1], b.lo[1]), fminf(a.lo[2], b.lo[2]),
+                                   __uint_as_float(l));
+    t.inode_hi[node] = make_float4(fmaxf(a.hi[0], b.hi[0]), fmaxf(a.hi[1], b.hi[1]), fmaxf(a.hi[2], b.hi[2]),
+                                   __uint_as_float(t.escape[node]));
+    t.lastf[node] = t.last[node] | ((a.moved & b.moved) << 31);
+}
+
+__device__ __forceinline__ uint32_t wg_arrive(uint32_t *p)
+{ return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// phase 1: a workgroup owns leaves [b*kChunk, (b+1)*kChunk) and finishes every
+// internal node whose leaf range lies inside that chunk; the second child to arrive
+// computes the node (workgroup-scope arrival counters, no device-wide fence). A walk
+// that reaches a node spanning chunks records it for phase 2.
+__global__ void __launch_bounds__(kChunk) k_refit_local(lbvh_dev t, uint32_t *defer, unsigned long long *counters)
+{
+    uint32_t n = t.n;
+    uint32_t c_lo = blockIdx.x * kChunk;
+    uint32_t c_hi = min(c_lo + kChunk, n) - 1;
+    uint32_t i = c_lo + threadIdx.x;
+    if (i >= n) return;
+    uint32_t node = t.parent[n - 1 + i];
+    for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+    {
+        if (t.first[node] < c_lo || t.last[node] > c_hi)
+        {
+            uint32_t slot = static_cast<uint32_t>(atomicAdd(&counters[C_DEFER], 1ull));
+            defer[slot] = node;
+            return;
+        }
+        if (wg_arrive(&t.visit[node]) == 0) return;
+        finish_node(t, node);
+        node = t.parent[node];
+    }
+}
+
+// phase 2: one workgroup finishes the nodes spanning chunks, walking up from every
+// recorded arrival with the same arrival protocol
+__global__ void __launch_bounds__(1024) k_refit_top(lbvh_dev t, const uint32_t *defer,
+                                                    const unsigned long long *counters)
+{
+    uint32_t nd = static_cast<uint32_t>(counters[C_DEFER]);
+    uint32_t n = t.n;
+    for (uint32_t base = 0; base < nd; base += blockDim.x)
+    {
+        uint32_t e = base + threadIdx.x;
+        if (e >= nd) continue;
+        uint32_t node = defer[e];
+        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+        {
+            if (wg_arrive(&t.visit[node]) == 0) break;
+            finish_node(t, node);
+            node = t.parent[node];
+        }
+    }
+}
+
+// one thread per sorted leaf i; a moved dynamic leaf emits its pair with leaf j when
+// j > i, or when j is not a moved leaf (j then never queries): each pair once. A
+// subtree whose leaves all lie at or below i and are all moved holds no such j.
+__global__ void __launch_bounds__(kBlock)
+    k_pair_query(lbvh_dev t, bodies_dev b, uint64_t *out_keys, uint32_t cap, unsigned long long *counters)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = t.n;
     bool active = false;
     uint32_t a = 0;
     box3 q{};
-    if (i < n)
+    float qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    if (i < n && t.leaf_moved[i])
     {
+        active = true;
         a = t.leaf_body[i];
-        active = !b.is_static[a] && (all_moved || b.moved[a]);
-        if (active) q = node_box_at(t.node_box, n - 1 + i);
+        const double *p = t.leaf_box + static_cast<size_t>(i) * 6;
+        q.min = mk(p[0], p[1], p[2]);
+        q.max = mk(p[3], p[4], p[5]);
+        for (int k = 0; k < 3; ++k)
+        {
+            qlo[k] = f_down(p[k]);
+            qhi[k] = f_up(p[3 + k]);
+        }
     }
     uint32_t node = active ? 0u : kNone;
-    // a depth-first walk visits each of the 2n-1 nodes at most once
     for (uint32_t guard = 0; node != kNone; ++guard)
     {
         if (guard >= 2 * n)
@@ -237,94 +342,120 @@ __global__ void k_pair_query(lbvh_dev t, bodies_dev b, int all_moved, uint64_t *
             atomicAdd(&counters[C_TREE_FAIL], 1ull); // malformed tree: reported, never loops
             break;
         }
-        box3 nb = node_box_at(t.node_box, node);
         bool emit = false;
         uint64_t key = 0;
         uint32_t next;
-        if (box_intersects(nb, q))
+        if (node < n - 1)
         {
-            if (node >= n - 1)
+            float4 lo = t.inode_lo[node], hi = t.inode_hi[node];
+            bool ov = lo.x <= qhi[0] && hi.x >= qlo[0] && lo.y <= qhi[1] && hi.y >= qlo[1] && lo.z <= qhi[2] &&
+                      hi.z >= qlo[2];
+            next = __float_as_uint(hi.w);
+            if (ov)
             {
-                uint32_t j = node - (n - 1);
-                if (j != i)
-                {
-                    uint32_t bb = t.leaf_body[j];
-                    bool moved_b = !b.is_static[bb] && (all_moved || b.moved[bb]);
-                    if (j > i || !moved_b)
-                    {
-                        emit = true;
-                        key = pair_key(a, bb);
-                    }
-                }
-                next = t.escape[node];
+                uint32_t lf = t.lastf[node];
+                bool skip = (lf & 0x7fffffffu) <= i && (lf >> 31);
+                if (!skip) next = __float_as_uint(lo.w);
             }
-            else
-                next = static_cast<uint32_t>(t.left[node]);
         }
         else
-            next = t.escape[node];
-        unsigned long long slot = wave_append(&counters[C_COMBINED], emit);
-        if (emit && slot < cap)
         {
-            out_keys[slot] = key;
-            out_vals[slot] = kNone;
+            uint32_t j = node - (n - 1);
+            next = t.escape[node];
+            if (j != i)
+            {
+                const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
+                box3 lb;
+                lb.min = mk(p[0], p[1], p[2]);
+                lb.max = mk(p[3], p[4], p[5]);
+                if (box_intersects(lb, q) && (j > i || !t.leaf_moved[j]))
+                {
+                    emit = true;
+                    key = pair_key(a, t.leaf_body[j]);
+                }
+            }
         }
+        unsigned long long slot = wave_append(&counters[C_COMBINED], emit);
+        if (emit && slot < cap) out_keys[slot] = key;
         node = next;
     }
 }
 
-__global__ void k_prev_filter(const uint64_t *prev, uint32_t n_prev, bodies_dev b, uint64_t *out_keys,
-                              uint32_t *out_vals, uint32_t cap, unsigned long long *counter)
+__device__ __forceinline__ uint32_t lower_bound(const uint64_t *a, uint32_t n, uint64_t key)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi)
+    {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// candidate e found in the previous (sorted) pair list: its manifold carries over
+__global__ void k_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,
+                        uint8_t *covered)
+{
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    uint64_t k = cand[e];
+    uint32_t j = lower_bound(prev, n_prev, k);
+    uint32_t s = kNone;
+    if (j < n_prev && prev[j] == k)
+    {
+        s = j;
+        covered[j] = 1;
+    }
+    src[e] = s;
+}
+
+// previous pairs that no candidate covers stay when their leaves still overlap
+// (collision_phases.h:383-403)
+__global__ void k_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered, bodies_dev b,
+                            uint32_t *flag)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = false;
-    uint64_t key = 0;
-    if (p < n_prev)
-    {
-        key = prev[p];
-        uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        keep = box_intersects(ldbox(b.fat, b.n, a), ldbox(b.fat, b.n, c)); // collision_phases.h:396
-    }
-    unsigned long long slot = wave_append(counter, keep);
-    if (keep && slot < cap)
-    {
-        out_keys[slot] = key;
-        out_vals[slot] = p;
-    }
+    if (p >= n_prev) return;
+    uint64_t key = prev[p];
+    uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    flag[p] = (!covered[p] && box_intersects(ldbox(b.fat, b.n, a), ldbox(b.fat, b.n, c))) ? 1u : 0u;
 }
 
-__global__ void k_dedupe(const uint64_t *keys, const uint32_t *vals, uint32_t n, uint32_t *flag, uint32_t *src)
+__global__ void k_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag, const uint32_t *pos,
+                               uint64_t *akeys, uint32_t *asrc)
 {
-    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    uint64_t k = keys[e];
-    bool first = e == 0 || keys[e - 1] != k;
-    flag[e] = first ? 1u : 0u;
-    if (first)
-    {
-        uint32_t s = vals[e];
-        if (s == kNone && e + 1 < n && keys[e + 1] == k) s = vals[e + 1];
-        src[e] = s;
-    }
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_prev || !flag[p]) return;
+    akeys[pos[p]] = prev[p];
+    asrc[pos[p]] = p;
 }
 
-__global__ void k_compact(const uint64_t *keys, const uint32_t *src, const uint32_t *flag, const uint32_t *pos,
-                          uint32_t n, uint64_t *out_keys, uint32_t *out_src)
+// merge of two sorted, disjoint key lists: an element's output slot is its own index
+// plus the number of keys of the other list below it
+__global__ void k_merge(const uint64_t *xk, const uint32_t *xs, uint32_t nx, const uint64_t *yk, uint32_t ny,
+                        uint64_t *out_keys, uint32_t *out_src)
 {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n || !flag[e]) return;
-    out_keys[pos[e]] = keys[e];
-    out_src[pos[e]] = src[e];
+    if (e >= nx) return;
+    uint64_t k = xk[e];
+    uint32_t pos = e + lower_bound(yk, ny, k);
+    out_keys[pos] = k;
+    out_src[pos] = xs[e];
 }
 } // namespace
 
-void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, sort_scratch &sc, hipStream_t st)
+void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
+                       unsigned long long *counters, hipStream_t st)
 {
     uint32_t n = b.n;
     t.n = n;
     if (n == 0) return;
     k_scene_init<<<1, 64, 0, st>>>(t.scene);
-    k_scene_bounds<<<grid(n), kBlock, 0, st>>>(b, t.scene);
+    unsigned sb = grid(n) < 512 ? grid(n) : 512;
+    k_scene_bounds<<<sb, kBlock, 0, st>>>(b, t.scene);
     k_morton<<<grid(n), kBlock, 0, st>>>(b, t.scene, t.morton, t.ids);
     static const int shifts[8] = {0, 8, 16, 24, 32, 40, 48, 56};
     if (radix_sort_u64(t.morton, t.ids, t.morton_alt, t.ids_alt, n, shifts, 8, sc, st))
@@ -336,41 +467,47 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, sort_scratch &sc, hipSt
         t.ids = t.ids_alt;
         t.ids_alt = ti;
     }
-    if (n >= 2) k_build_internal<<<grid(n - 1), kBlock, 0, st>>>(t.morton, static_cast<int>(n), t.left, t.right, t.parent);
-    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, n, t.node_box, t.leaf_body, t.parent);
+    if (n >= 2) k_build_internal<<<grid(n - 1), kBlock, 0, st>>>(t.morton, static_cast<int>(n), t);
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    k_escape<<<grid(2 * n - 1), kBlock, 0, st>>>(t);
     if (n >= 2)
     {
         launch_fill_u32(t.visit, 0u, n - 1, st);
-        k_refit<<<grid(n), kBlock, 0, st>>>(n, t.left, t.right, t.parent, t.node_box, t.visit);
+        launch_fill_u64(counters + C_DEFER, 0ull, 1, st);
+        k_refit_local<<<grid(n, kChunk), kChunk, 0, st>>>(t, t.defer, counters);
+        k_refit_top<<<1, 1024, 0, st>>>(t, t.defer, counters);
     }
-    k_escape<<<grid(2 * n - 1), kBlock, 0, st>>>(n, t.left, t.right, t.parent, t.escape);
 }
 
-void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, bool all_moved, uint64_t *out_keys,
-                       uint32_t *out_vals, uint32_t cap, unsigned long long *counters, hipStream_t st)
+void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_keys, uint32_t cap,
+                       unsigned long long *counters, hipStream_t st)
 {
     if (t.n < 2) return;
-    k_pair_query<<<grid(t.n), kBlock, 0, st>>>(t, b, all_moved ? 1 : 0, out_keys, out_vals, cap, counters);
+    k_pair_query<<<grid(t.n), kBlock, 0, st>>>(t, b, out_keys, cap, counters);
 }
 
-void launch_prev_filter(const uint64_t *prev_keys, uint32_t n_prev, const bodies_dev &b,
-                        uint64_t *out_keys, uint32_t *out_vals, uint32_t cap,
-                        unsigned long long *counter, hipStream_t st)
+void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,
+                       uint8_t *covered, hipStream_t st)
 {
-    if (n_prev) k_prev_filter<<<grid(n_prev), kBlock, 0, st>>>(prev_keys, n_prev, b, out_keys, out_vals, cap, counter);
+    if (m) k_match<<<grid(m), kBlock, 0, st>>>(cand, m, prev, n_prev, src, covered);
 }
 
-void launch_dedupe_flags(const uint64_t *keys, const uint32_t *vals, uint32_t n, uint32_t *flag,
-                         uint32_t *src, hipStream_t st)
+void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered, const bodies_dev &b,
+                      uint32_t *flag, hipStream_t st)
 {
-    if (n) k_dedupe<<<grid(n), kBlock, 0, st>>>(keys, vals, n, flag, src);
+    if (n_prev) k_prev_keep<<<grid(n_prev), kBlock, 0, st>>>(prev, n_prev, covered, b, flag);
 }
 
-void launch_compact_pairs(const uint64_t *keys, const uint32_t *src, const uint32_t *flag,
-                          const uint32_t *pos, uint32_t n, uint64_t *out_keys, uint32_t *out_src,
-                          hipStream_t st)
+void launch_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag, const uint32_t *pos,
+                         uint64_t *akeys, uint32_t *asrc, hipStream_t st)
 {
-    if (n) k_compact<<<grid(n), kBlock, 0, st>>>(keys, src, flag, pos, n, out_keys, out_src);
+    if (n_prev) k_prev_compact<<<grid(n_prev), kBlock, 0, st>>>(prev, n_prev, flag, pos, akeys, asrc);
+}
+
+void launch_merge(const uint64_t *xk, const uint32_t *xs, uint32_t nx, const uint64_t *yk, uint32_t ny,
+                  uint64_t *out_keys, uint32_t *out_src, hipStream_t st)
+{
+    if (nx) k_merge<<<grid(nx), kBlock, 0, st>>>(xk, xs, nx, yk, ny, out_keys, out_src);
 }
 
 } // namespace pk

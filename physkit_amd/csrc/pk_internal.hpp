@@ -71,6 +71,8 @@ enum counter_slot
     C_EPA_BIG_FAIL = 9,
     C_EVENTS = 10,
     C_TREE_FAIL = 11,
+    C_DEFER = 12,
+    C_KEEP = 13,
     C_NCOUNTERS = 16
 };
 
@@ -100,6 +102,8 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
 size_t scan_block_sums(uint32_t n);
 
 // ---------------- k_broad.hip ----------------
+// Node indices are unified: internal nodes [0, n-1), leaf i is node n-1+i; node 0 is
+// the root when n >= 2. Leaves are in Morton order.
 struct lbvh_dev
 {
     uint32_t n; // leaves
@@ -107,30 +111,39 @@ struct lbvh_dev
     uint32_t *ids;
     uint64_t *morton_alt;
     uint32_t *ids_alt;
-    int32_t *left;       // n-1 internal
+    int32_t *left;       // n-1: children (unified)
     int32_t *right;      // n-1
-    uint32_t *parent;    // 2n-1 (unified node index: internal [0,n-1), leaf n-1+i)
-    uint32_t *escape;    // 2n-1
-    double *node_box;    // 6 per node, node-major (one node test reads 48 contiguous bytes)
-    uint32_t *leaf_body; // n: body id of sorted leaf i
+    uint32_t *first;     // n-1: leaf range of the node
+    uint32_t *last;      // n-1
+    uint32_t *lastf;     // n-1: last | (every leaf below is moved) << 31
+    float4 *inode_lo;    // n-1: FP32 box min rounded down, w = left child bits
+    float4 *inode_hi;    // n-1: FP32 box max rounded up, w = escape bits
+    uint32_t *parent;    // 2n-1
+    uint32_t *escape;    // 2n-1: next node after the subtree in left-first order
+    double *leaf_box;    // 6n: exact FP64 leaf bounds, leaf-major
+    uint32_t *leaf_body; // n: body id of leaf i
+    uint8_t *leaf_moved; // n: leaf queries this step (dynamic and moved)
     uint32_t *visit;     // n-1 refit arrival counters
+    uint32_t *defer;     // 2n: refit arrivals at nodes spanning workgroup chunks
     unsigned long long *scene; // 6 ordered-integer bounds of leaf centroids
 };
-void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, sort_scratch &sc, hipStream_t st);
-// candidate pairs of moved leaves appended (key, kNone) into out; the append cursor
-// is counters[C_COMBINED], a malformed tree is reported in counters[C_TREE_FAIL]
-void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, bool all_moved, uint64_t *out_keys,
-                       uint32_t *out_vals, uint32_t cap, unsigned long long *counters,
-                       hipStream_t st);
-// previous pairs whose leaves still overlap appended (key, prev index)
-void launch_prev_filter(const uint64_t *prev_keys, uint32_t n_prev, const bodies_dev &b,
-                        uint64_t *out_keys, uint32_t *out_vals, uint32_t cap,
-                        unsigned long long *counter, hipStream_t st);
-void launch_dedupe_flags(const uint64_t *keys, const uint32_t *vals, uint32_t n, uint32_t *flag,
-                         uint32_t *src, hipStream_t st);
-void launch_compact_pairs(const uint64_t *keys, const uint32_t *src, const uint32_t *flag,
-                          const uint32_t *pos, uint32_t n, uint64_t *out_keys, uint32_t *out_src,
-                          hipStream_t st);
+void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
+                       unsigned long long *counters, hipStream_t st);
+// candidate keys (pairs with at least one moved leaf) appended at counters[C_COMBINED];
+// a malformed tree is reported in counters[C_TREE_FAIL]
+void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_keys, uint32_t cap,
+                       unsigned long long *counters, hipStream_t st);
+// src[e] = index of candidate e in the sorted previous list (kNone if new); marks it covered
+void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev,
+                       uint32_t *src, uint8_t *covered, hipStream_t st);
+// flag previous pairs no candidate covers whose leaves still overlap, then compact them
+void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered,
+                      const bodies_dev &b, uint32_t *flag, hipStream_t st);
+void launch_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag,
+                         const uint32_t *pos, uint64_t *akeys, uint32_t *asrc, hipStream_t st);
+// writes list x (sorted) into its slots of the merge of x and y (sorted, disjoint)
+void launch_merge(const uint64_t *xk, const uint32_t *xs, uint32_t nx, const uint64_t *yk,
+                  uint32_t ny, uint64_t *out_keys, uint32_t *out_src, hipStream_t st);
 
 // ---------------- k_narrow.hip ----------------
 struct gjk_out_dev
