@@ -17,7 +17,8 @@ namespace pk
 namespace
 {
 constexpr int kBlock = 256;
-constexpr int kChunk = 1024; // leaves per refit workgroup (phase 1)
+constexpr int kChunk = 4096;       // leaves per refit workgroup (phase 1)
+constexpr int kRefitThreads = 1024;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 __device__ __forceinline__ unsigned long long ord(double x)
@@ -266,25 +267,27 @@ __device__ __forceinline__ uint32_t wg_arrive(uint32_t *p)
 // internal node whose leaf range lies inside that chunk; the second child to arrive
 // computes the node (workgroup-scope arrival counters, no device-wide fence). A walk
 // that reaches a node spanning chunks records it for phase 2.
-__global__ void __launch_bounds__(kChunk) k_refit_local(lbvh_dev t, uint32_t *defer, unsigned long long *counters)
+__global__ void __launch_bounds__(kRefitThreads) k_refit_local(lbvh_dev t, uint32_t *defer,
+                                                               unsigned long long *counters)
 {
     uint32_t n = t.n;
     uint32_t c_lo = blockIdx.x * kChunk;
     uint32_t c_hi = min(c_lo + kChunk, n) - 1;
-    uint32_t i = c_lo + threadIdx.x;
-    if (i >= n) return;
-    uint32_t node = t.parent[n - 1 + i];
-    for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+    for (uint32_t i = c_lo + threadIdx.x; i <= c_hi; i += blockDim.x)
     {
-        if (t.first[node] < c_lo || t.last[node] > c_hi)
+        uint32_t node = t.parent[n - 1 + i];
+        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
         {
-            uint32_t slot = static_cast<uint32_t>(atomicAdd(&counters[C_DEFER], 1ull));
-            defer[slot] = node;
-            return;
+            if (t.first[node] < c_lo || t.last[node] > c_hi)
+            {
+                uint32_t slot = static_cast<uint32_t>(atomicAdd(&counters[C_DEFER], 1ull));
+                defer[slot] = node;
+                break;
+            }
+            if (wg_arrive(&t.visit[node]) == 0) break;
+            finish_node(t, node);
+            node = t.parent[node];
         }
-        if (wg_arrive(&t.visit[node]) == 0) return;
-        finish_node(t, node);
-        node = t.parent[node];
     }
 }
 
@@ -312,9 +315,24 @@ __global__ void __launch_bounds__(1024) k_refit_top(lbvh_dev t, const uint32_t *
 // one thread per sorted leaf i; a moved dynamic leaf emits its pair with leaf j when
 // j > i, or when j is not a moved leaf (j then never queries): each pair once. A
 // subtree whose leaves all lie at or below i and are all moved holds no such j.
+// Pairs collect in a per-thread buffer; the output space is reserved once per
+// wavefront after the walk (a full buffer is flushed on its own), so the shared
+// cursor sees a few thousand atomics instead of one per emitting iteration.
+constexpr int kEmitBuf = 32;
+
+__device__ __forceinline__ void flush_own(uint64_t *buf, uint32_t cnt, uint64_t *out_keys, uint32_t cap,
+                                          unsigned long long *cursor)
+{
+    unsigned long long base = atomicAdd(cursor, static_cast<unsigned long long>(cnt));
+    for (uint32_t k = 0; k < cnt; ++k)
+        if (base + k < cap) out_keys[base + k] = buf[k];
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_pair_query(lbvh_dev t, bodies_dev b, uint64_t *out_keys, uint32_t cap, unsigned long long *counters)
 {
+    uint64_t buf[kEmitBuf];
+    uint32_t nbuf = 0;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = t.n;
     bool active = false;
@@ -342,8 +360,6 @@ __global__ void __launch_bounds__(kBlock)
             atomicAdd(&counters[C_TREE_FAIL], 1ull); // malformed tree: reported, never loops
             break;
         }
-        bool emit = false;
-        uint64_t key = 0;
         uint32_t next;
         if (node < n - 1)
         {
@@ -370,15 +386,31 @@ __global__ void __launch_bounds__(kBlock)
                 lb.max = mk(p[3], p[4], p[5]);
                 if (box_intersects(lb, q) && (j > i || !t.leaf_moved[j]))
                 {
-                    emit = true;
-                    key = pair_key(a, t.leaf_body[j]);
+                    if (nbuf == kEmitBuf)
+                    {
+                        flush_own(buf, nbuf, out_keys, cap, &counters[C_COMBINED]);
+                        nbuf = 0;
+                    }
+                    buf[nbuf++] = pair_key(a, t.leaf_body[j]);
                 }
             }
         }
-        unsigned long long slot = wave_append(&counters[C_COMBINED], emit);
-        if (emit && slot < cap) out_keys[slot] = key;
         node = next;
     }
+    // converged again: one reservation per wavefront for the buffered pairs
+    int lane = __lane_id();
+    uint32_t incl = nbuf;
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        uint32_t v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    uint32_t total = __shfl(incl, 63);
+    unsigned long long base = 0;
+    if (lane == 63 && total) base = atomicAdd(&counters[C_COMBINED], static_cast<unsigned long long>(total));
+    base = __shfl(base, 63) + (incl - nbuf);
+    for (uint32_t k = 0; k < nbuf; ++k)
+        if (base + k < cap) out_keys[base + k] = buf[k];
 }
 
 __device__ __forceinline__ uint32_t lower_bound(const uint64_t *a, uint32_t n, uint64_t key)
@@ -474,7 +506,7 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     {
         launch_fill_u32(t.visit, 0u, n - 1, st);
         launch_fill_u64(counters + C_DEFER, 0ull, 1, st);
-        k_refit_local<<<grid(n, kChunk), kChunk, 0, st>>>(t, t.defer, counters);
+        k_refit_local<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t, t.defer, counters);
         k_refit_top<<<1, 1024, 0, st>>>(t, t.defer, counters);
     }
 }
