@@ -17,7 +17,7 @@ namespace pk
 namespace
 {
 constexpr int kBlock = 256;
-constexpr int kChunk = 4096;       // leaves per refit workgroup (phase 1)
+constexpr int kChunk = 1024; // leaves per refit workgroup
 constexpr int kRefitThreads = 1024;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
@@ -263,12 +263,15 @@ __device__ __forceinline__ void finish_node(const lbvh_dev &t, uint32_t node)
 __device__ __forceinline__ uint32_t wg_arrive(uint32_t *p)
 { return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
-// phase 1: a workgroup owns leaves [b*kChunk, (b+1)*kChunk) and finishes every
-// internal node whose leaf range lies inside that chunk; the second child to arrive
-// computes the node (workgroup-scope arrival counters, no device-wide fence). A walk
-// that reaches a node spanning chunks records it for phase 2.
-__global__ void __launch_bounds__(kRefitThreads) k_refit_local(lbvh_dev t, uint32_t *defer,
-                                                               unsigned long long *counters)
+__device__ __forceinline__ uint32_t dev_arrive(uint32_t *p)
+{ return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Bottom-up refit: the second child to arrive at a node computes it. A workgroup owns
+// leaves [b*kChunk, (b+1)*kChunk); arrivals at nodes whose leaf range lies inside the
+// chunk only synchronise within the workgroup (workgroup-scope acquire/release, no
+// cache write-back), and only the few nodes spanning chunks pay a device-scope
+// acquire/release.
+__global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
 {
     uint32_t n = t.n;
     uint32_t c_lo = blockIdx.x * kChunk;
@@ -278,34 +281,9 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit_local(lbvh_dev t, uint3
         uint32_t node = t.parent[n - 1 + i];
         for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
         {
-            if (t.first[node] < c_lo || t.last[node] > c_hi)
-            {
-                uint32_t slot = static_cast<uint32_t>(atomicAdd(&counters[C_DEFER], 1ull));
-                defer[slot] = node;
-                break;
-            }
-            if (wg_arrive(&t.visit[node]) == 0) break;
-            finish_node(t, node);
-            node = t.parent[node];
-        }
-    }
-}
-
-// phase 2: one workgroup finishes the nodes spanning chunks, walking up from every
-// recorded arrival with the same arrival protocol
-__global__ void __launch_bounds__(1024) k_refit_top(lbvh_dev t, const uint32_t *defer,
-                                                    const unsigned long long *counters)
-{
-    uint32_t nd = static_cast<uint32_t>(counters[C_DEFER]);
-    uint32_t n = t.n;
-    for (uint32_t base = 0; base < nd; base += blockDim.x)
-    {
-        uint32_t e = base + threadIdx.x;
-        if (e >= nd) continue;
-        uint32_t node = defer[e];
-        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
-        {
-            if (wg_arrive(&t.visit[node]) == 0) break;
+            bool local = t.first[node] >= c_lo && t.last[node] <= c_hi;
+            uint32_t before = local ? wg_arrive(&t.visit[node]) : dev_arrive(&t.visit[node]);
+            if (before == 0) break;
             finish_node(t, node);
             node = t.parent[node];
         }
@@ -505,9 +483,7 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     if (n >= 2)
     {
         launch_fill_u32(t.visit, 0u, n - 1, st);
-        launch_fill_u64(counters + C_DEFER, 0ull, 1, st);
-        k_refit_local<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t, t.defer, counters);
-        k_refit_top<<<1, 1024, 0, st>>>(t, t.defer, counters);
+        k_refit<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t);
     }
 }
 

@@ -73,13 +73,14 @@ struct simplex
 };
 __device__ __forceinline__ void sx_push(simplex &x, const spt &p)
 {
-    switch (x.n)
-    {
-    case 0: x.s0 = p; break;
-    case 1: x.s1 = p; break;
-    case 2: x.s2 = p; break;
-    default: x.s3 = p; break;
-    }
+    if (x.n == 0)
+        x.s0 = p;
+    else if (x.n == 1)
+        x.s1 = p;
+    else if (x.n == 2)
+        x.s2 = p;
+    else
+        x.s3 = p;
     ++x.n;
 }
 
@@ -188,7 +189,8 @@ __device__ __forceinline__ bool handle_tetrahedron(simplex &x, v3 &dir) // colli
 }
 
 // boolean GJK, collision.cpp:165-189; returns true with the final simplex on a hit
-__device__ bool gjk(const gshape &A, const gshape &B, simplex &x)
+// (inlined so the simplex stays in registers)
+__device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x)
 {
     const double eps = 1e-12;
     x.n = 0;
@@ -213,7 +215,7 @@ __device__ bool gjk(const gshape &A, const gshape &B, simplex &x)
 }
 
 // collision.cpp:191-248
-__device__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x)
+__device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x)
 {
     int start = x.n;
     if (start <= 1)
@@ -252,22 +254,35 @@ __device__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x)
     return fabs(triple) > 1e-12;
 }
 
+
 // ---------------- EPA (collision.cpp:251-509) ----------------
 constexpr uint16_t kFNull = 0xFFFF;
 constexpr int kPolyCap = 68; // 4 simplex points + at most one per iteration (64)
 
+// one face per 64-byte record, so every face access touches a single cache line
+struct alignas(64) epa_face
+{
+    v3 n;        // unit normal (zero for a degenerate triangle)
+    double d;    // distance of the face plane from the origin
+    uint16_t v[3];
+    uint16_t a[3]; // adjacent face across edge (v[i], v[i+1])
+    uint8_t obsolete;
+};
+struct epa_edge
+{
+    uint16_t s, e, adj;
+};
+
+// A store is private to one EPA solve and contiguous, so the working set of a solve
+// is compact in memory.
 template <int FCAP> struct epa_store
 {
     static constexpr int cap = FCAP;
+    epa_face f[FCAP];
     spt poly[kPolyCap];
-    v3 fn[FCAP];
-    double fd[FCAP];
-    uint16_t fv[FCAP][3];
-    uint16_t fa[FCAP][3];
-    uint8_t fo[FCAP];
     uint16_t heap[FCAP];
     uint16_t stack[FCAP];
-    uint16_t hs[FCAP], he[FCAP], hadj[FCAP];
+    epa_edge h[FCAP];
     uint16_t nf[FCAP];
 };
 
@@ -275,51 +290,58 @@ template <class S> struct epa_ctx
 {
     S &m;
     int npoly = 0, nfaces = 0, nheap = 0;
-    bool overflow = false;
 
     __device__ explicit epa_ctx(S &store) : m(store) {}
 
-    __device__ void init_face(int f, int i, int j, int k, int opp) // :273-297
+    __device__ void init_face(int fi, int i, int j, int k, int opp) // :273-297
     {
-        m.fv[f][0] = static_cast<uint16_t>(i);
-        m.fv[f][1] = static_cast<uint16_t>(j);
-        m.fv[f][2] = static_cast<uint16_t>(k);
-        m.fa[f][0] = kFNull;
-        m.fa[f][1] = kFNull;
-        m.fa[f][2] = kFNull;
-        m.fo[f] = 0;
-        v3 ab = m.poly[j].p - m.poly[i].p;
-        v3 ac = m.poly[k].p - m.poly[i].p;
+        epa_face &f = m.f[fi];
+        uint16_t v1 = static_cast<uint16_t>(j), v2 = static_cast<uint16_t>(k);
+        v3 pi = m.poly[i].p;
+        v3 ab = m.poly[j].p - pi;
+        v3 ac = m.poly[k].p - pi;
         v3 nrm = cross(ab, ac);
         if (sqn(nrm) < 1e-12)
             nrm = mk(0.0, 0.0, 0.0);
         else
             nrm = normalized(nrm);
-        if (opp >= 0 && dot(nrm, m.poly[opp].p - m.poly[i].p) > 0.0)
+        if (opp >= 0 && dot(nrm, m.poly[opp].p - pi) > 0.0)
         {
-            uint16_t t = m.fv[f][1];
-            m.fv[f][1] = m.fv[f][2];
-            m.fv[f][2] = t;
+            uint16_t t = v1;
+            v1 = v2;
+            v2 = t;
             nrm = -nrm;
         }
-        m.fn[f] = nrm;
-        m.fd[f] = dot(nrm, m.poly[i].p);
+        f.v[0] = static_cast<uint16_t>(i);
+        f.v[1] = v1;
+        f.v[2] = v2;
+        f.a[0] = kFNull;
+        f.a[1] = kFNull;
+        f.a[2] = kFNull;
+        f.obsolete = 0;
+        f.n = nrm;
+        f.d = dot(nrm, pi);
     }
     __device__ void link(int f1, int f2, int va, int vb) // :305-313
     {
-        int e1 = (m.fv[f1][0] == va) ? 0 : (m.fv[f1][1] == va ? 1 : 2);
-        int e2 = (m.fv[f2][0] == vb) ? 0 : (m.fv[f2][1] == vb ? 1 : 2);
-        m.fa[f1][e1] = static_cast<uint16_t>(f2);
-        m.fa[f2][e2] = static_cast<uint16_t>(f1);
+        epa_face &a = m.f[f1];
+        epa_face &b = m.f[f2];
+        int e1 = (a.v[0] == va) ? 0 : (a.v[1] == va ? 1 : 2);
+        int e2 = (b.v[0] == vb) ? 0 : (b.v[1] == vb ? 1 : 2);
+        a.a[e1] = static_cast<uint16_t>(f2);
+        b.a[e2] = static_cast<uint16_t>(f1);
     }
     // binary min-heap on face distance; sift order of libstdc++'s push_heap/pop_heap
-    __device__ bool before(int x, int y) const { return m.fd[x] > m.fd[y]; }
+    __device__ bool before(int x, int y) const { return m.f[x].d > m.f[y].d; }
     __device__ void sift_up(int hole, int top, uint16_t val)
     {
+        double dv = m.f[val].d;
         int parent = (hole - 1) / 2;
-        while (hole > top && before(m.heap[parent], val))
+        while (hole > top)
         {
-            m.heap[hole] = m.heap[parent];
+            uint16_t pv = m.heap[parent];
+            if (!(m.f[pv].d > dv)) break;
+            m.heap[hole] = pv;
             hole = parent;
             parent = (hole - 1) / 2;
         }
@@ -346,8 +368,13 @@ template <class S> struct epa_ctx
                 while (child < (n - 1) / 2)
                 {
                     child = 2 * (child + 1);
-                    if (before(m.heap[child], m.heap[child - 1])) child--;
-                    m.heap[hole] = m.heap[child];
+                    uint16_t r = m.heap[child], l = m.heap[child - 1];
+                    if (before(r, l))
+                    {
+                        child--;
+                        r = l;
+                    }
+                    m.heap[hole] = r;
                     hole = child;
                 }
                 if ((n & 1) == 0 && child == (n - 2) / 2)
@@ -359,17 +386,13 @@ template <class S> struct epa_ctx
                 sift_up(hole, 0, val);
             }
             --nheap;
-            if (!m.fo[top]) return top;
+            if (!m.f[top].obsolete) return top;
         }
         return -1;
     }
     __device__ int alloc_face()
     {
-        if (nfaces >= S::cap)
-        {
-            overflow = true;
-            return -1;
-        }
+        if (nfaces >= S::cap) return -1;
         return nfaces++;
     }
     // :315-353; returns the number of horizon edges, -1 on overflow
@@ -378,27 +401,31 @@ template <class S> struct epa_ctx
         const double tol = 1e-6;
         int sp = 0, nh = 0;
         m.stack[sp++] = static_cast<uint16_t>(f0);
-        m.fo[f0] = 1;
+        m.f[f0].obsolete = 1;
         while (sp > 0)
         {
             int cur = m.stack[--sp];
+            uint16_t cv[3] = {m.f[cur].v[0], m.f[cur].v[1], m.f[cur].v[2]};
+            uint16_t ca[3] = {m.f[cur].a[0], m.f[cur].a[1], m.f[cur].a[2]};
             for (int i = 0; i < 3; ++i)
             {
-                int nb = m.fa[cur][i];
+                int nb = ca[i];
                 if (nb == kFNull) continue;
-                if (m.fo[nb]) continue;
-                if (dot(m.fn[nb], p) > m.fd[nb] + tol)
+                epa_face &g = m.f[nb];
+                if (g.obsolete) continue;
+                if (dot(g.n, p) > g.d + tol)
                 {
-                    m.fo[nb] = 1;
+                    g.obsolete = 1;
                     m.stack[sp++] = static_cast<uint16_t>(nb); // each face is pushed at most once
                 }
                 else
                 {
                     if (nh >= S::cap) return -1;
-                    m.hs[nh] = m.fv[cur][i];
-                    m.he[nh] = m.fv[cur][(i + 1) % 3];
-                    m.hadj[nh] = static_cast<uint16_t>(nb);
-                    ++nh;
+                    epa_edge ed;
+                    ed.s = cv[i];
+                    ed.e = cv[(i + 1) % 3];
+                    ed.adj = static_cast<uint16_t>(nb);
+                    m.h[nh++] = ed;
                 }
             }
         }
@@ -417,26 +444,27 @@ template <class S> struct epa_ctx
             for (int j = i + 1; j < 4; ++j)
                 for (int e1 = 0; e1 < 3; ++e1)
                 {
-                    int u1 = m.fv[i][e1], v1 = m.fv[i][(e1 + 1) % 3];
+                    int u1 = m.f[i].v[e1], v1 = m.f[i].v[(e1 + 1) % 3];
                     for (int e2 = 0; e2 < 3; ++e2)
                     {
-                        int u2 = m.fv[j][e2], v2 = m.fv[j][(e2 + 1) % 3];
+                        int u2 = m.f[j].v[e2], v2 = m.f[j].v[(e2 + 1) % 3];
                         if (u1 == v2 && v1 == u2)
                         {
-                            m.fa[i][e1] = static_cast<uint16_t>(j);
-                            m.fa[j][e2] = static_cast<uint16_t>(i);
+                            m.f[i].a[e1] = static_cast<uint16_t>(j);
+                            m.f[j].a[e2] = static_cast<uint16_t>(i);
                         }
                     }
                 }
     }
     // :424-454; out = normal3, world_a3, world_b3, depth
-    __device__ void barycentric(int f, double *out) const
+    __device__ void barycentric(int fi, double *out) const
     {
-        const spt &p0 = m.poly[m.fv[f][0]];
-        const spt &p1 = m.poly[m.fv[f][1]];
-        const spt &p2 = m.poly[m.fv[f][2]];
-        v3 nrm = m.fn[f];
-        double fd = m.fd[f];
+        const epa_face &f = m.f[fi];
+        const spt &p0 = m.poly[f.v[0]];
+        const spt &p1 = m.poly[f.v[1]];
+        const spt &p2 = m.poly[f.v[2]];
+        v3 nrm = f.n;
+        double fd = f.d;
         v3 pm = nrm * fd;
         v3 v0 = p1.p - p0.p, v1 = p2.p - p0.p, v2 = pm - p0.p;
         double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
@@ -479,8 +507,8 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
     {
         int mf = e.pop();
         if (mf < 0) break;
-        v3 nrm = store.fn[mf];
-        double fd = store.fd[mf];
+        v3 nrm = store.f[mf].n;
+        double fd = store.f[mf].d;
         spt p = msupport(A, B, nrm);
         double pd = dot(nrm, p.p);
         if (pd - fd < tol)
@@ -497,18 +525,20 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         {
             int f = e.alloc_face();
             if (f < 0) return 3;
-            e.init_face(f, store.hs[h], store.he[h], pi, -1);
-            e.link(f, store.hadj[h], store.hs[h], store.he[h]);
+            epa_edge ed = store.h[h];
+            e.init_face(f, ed.s, ed.e, pi, -1);
+            e.link(f, ed.adj, ed.s, ed.e);
             e.push(f);
             store.nf[h] = static_cast<uint16_t>(f);
         }
         for (int i = 0; i < nh; ++i)
             for (int j = i + 1; j < nh; ++j)
             {
-                if (store.he[i] == store.hs[j])
-                    e.link(store.nf[i], store.nf[j], store.he[i], pi);
-                else if (store.hs[i] == store.he[j])
-                    e.link(store.nf[j], store.nf[i], store.he[j], pi);
+                epa_edge hi = store.h[i], hj = store.h[j];
+                if (hi.e == hj.s)
+                    e.link(store.nf[i], store.nf[j], hi.e, pi);
+                else if (hi.s == hj.e)
+                    e.link(store.nf[j], store.nf[i], hj.e, pi);
             }
     }
     int mf = e.pop();
