@@ -62,7 +62,8 @@ typedef struct pkg_config
  * inertia is the local 3x3 tensor row-major (object_desc, include/physkit/core/
  * object.h:15-109). A static body (is_static=1) has infinite mass, as
  * object_desc::mass() returns for body_type::stat (object.h:81-85); its mass entry
- * is ignored. */
+ * is ignored. is_static=2 marks a disabled slot (a removed body's arena slot kept so
+ * that the other ids do not change): it never moves and never pairs. */
 typedef struct pkg_body_desc
 {
     const uint8_t *is_static;

@@ -638,6 +638,12 @@ uint32_t world::add_body(const body &in) // world.h:202-208, particle.h:16-28
     b.update_derived();
     mesh_instance inst{&shapes[b.shape], b.pos, b.q};
     b.fat = inst.bounds(); // dynamic_bvh::add stores exact bounds
+    if (b.disabled) // a removed body's slot has no leaf (dynamic_bvh::remove_leaf, collision/bvh.h:307)
+    {
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        b.fat.min = {nan, nan, nan};
+        b.fat.max = b.fat.min;
+    }
     b.moved = false;
     bodies.push_back(b);
     return static_cast<uint32_t>(bodies.size() - 1);

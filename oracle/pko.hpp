@@ -141,6 +141,7 @@ std::optional<collision_info> gjk_epa(const mesh_instance &a, const mesh_instanc
 struct body // particle.h:13-147 + object.h:111-144
 {
     bool is_static = false;
+    bool disabled = false; // a removed body's arena slot: static, never paired
     uint32_t shape = 0;
     v3 pos, vel, acc, angvel, torque;
     qt q;

@@ -76,6 +76,7 @@ int pko_add_bodies(void *h, int n, const uint8_t *is_static, const uint32_t *sha
     {
         body b;
         b.is_static = is_static[i] != 0;
+        b.disabled = is_static[i] == 2;
         b.shape = shape[i];
         if (b.shape >= w->shapes.size()) return -1;
         b.pos = ld3(pos + 3 * i);

@@ -355,10 +355,16 @@ class world_desc:
     def __init__(self):
         self.gravity = (0.0, -9.81, 0.0)
         self.solver_iterations = 10
+        self.flags = 0
 
     @staticmethod
     def make():
         return world_desc()
+
+    def with_epa_guard(self, on=True):
+        """Opt-in FLAG_EPA_GUARD (not in the reference): drop degenerate EPA results."""
+        self.flags = (self.flags | FLAG_EPA_GUARD) if on else (self.flags & ~FLAG_EPA_GUARD)
+        return self
 
     def with_gravity(self, g):
         self.gravity = tuple(float(x) for x in g)
@@ -417,7 +423,7 @@ class world:
         self._descs = []
         self._gpu = None
         self._dirty = True
-        self.on_collision = None       # callback(a, b, manifold) at collision begin
+        self.on_collision = None       # callback(a, b) at collision begin
         self.on_collision_exit = None  # callback(a, b) at collision end
 
     def create_rigid(self, d: object_desc) -> int:
@@ -448,7 +454,7 @@ class world:
             "friction": np.array([d.friction for d in self._descs], float),
         }
         if self._gpu is None:
-            self._gpu = GpuWorld(self.desc.gravity, self.desc.solver_iterations)
+            self._gpu = GpuWorld(self.desc.gravity, self.desc.solver_iterations, self.desc.flags)
         self._gpu.load_scene(s)
         self._dirty = False
 

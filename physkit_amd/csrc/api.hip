@@ -130,7 +130,7 @@ struct pkg_ctx
     dbuf<uint32_t> hit_list, ovf_list;
     dbuf<double> simplex, info;
     dbuf<uint8_t> status;
-    dbuf<char> big_pool, epa_pool;
+    dbuf<char> big_pool;
 
     // solver
     dbuf<double> rows;
@@ -348,8 +348,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);
-        epa_pool.reserve(epa_pool_bytes(hits));
-        launch_epa(keys[nxt].p, hits, b, s, g, epa_pool.p, ovf_list.p, counters.p, st);
+        launch_epa(keys[nxt].p, hits, b, s, g, ovf_list.p, counters.p, st);
         uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
         if (novf)
         {

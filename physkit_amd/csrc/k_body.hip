@@ -53,7 +53,14 @@ __global__ void k_body_init(bodies_dev b, shapes_dev s) // particle.h:16-28, bvh
     qt q = ldq(b.quat, b.n, i);
     derived(b, i, q);
     uint32_t sh = b.shape[i];
-    stbox(b.fat, b.n, i, instance_bounds(ldbox(s.local_box, s.n, sh), q, ld3(b.pos, b.n, i)));
+    box3 fat = instance_bounds(ldbox(s.local_box, s.n, sh), q, ld3(b.pos, b.n, i));
+    if (b.is_static[i] == 2) // a disabled (freed) slot: NaN bounds overlap nothing
+    {
+        double nan = __builtin_nan("");
+        fat.min = mk(nan, nan, nan);
+        fat.max = fat.min;
+    }
+    stbox(b.fat, b.n, i, fat);
     st3(b.acc, b.n, i, mk(0.0, 0.0, 0.0));
     st3(b.torque, b.n, i, mk(0.0, 0.0, 0.0));
     b.moved[i] = 0;

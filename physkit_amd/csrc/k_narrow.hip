@@ -18,7 +18,6 @@ constexpr int kBlock = 256;
 constexpr int kEpaBlock = 64;
 constexpr int kFastCap = 192;
 constexpr int kBigCap = 4096;
-constexpr size_t kEpaMaxThreads = 131072; // 131072 stores of ~19.5 KB = 2.5 GB of pool
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 __device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_dev &s, uint32_t id)
@@ -103,36 +102,30 @@ __device__ __forceinline__ void write_info(double *info, uint32_t cap, uint32_t 
     for (int k = 0; k < 10; ++k) info[static_cast<size_t>(k) * cap + p] = o[k];
 }
 
-// grid-stride over the hits; thread t owns store pool[t], contiguous in memory, so a
-// solve's faces, heap and polytope stay within a few cache lines of one another
+// one thread per hit, solve store in private memory
 __global__ void __launch_bounds__(kEpaBlock)
     k_epa(const uint64_t *keys, uint32_t n_hits, bodies_dev b, shapes_dev s, gjk_out_dev g,
-          epa_store<kFastCap> *pool, uint32_t *ovf_list, unsigned long long *counters)
+          uint32_t *ovf_list, unsigned long long *counters)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t rounds = (n_hits + stride - 1) / stride;
-    for (uint32_t r0 = 0; r0 < rounds; ++r0)
+    uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ovf = false;
+    if (h < n_hits)
     {
-        uint32_t h = r0 * stride + t;
-        bool ovf = false;
-        if (h < n_hits)
-        {
-            uint32_t p = g.hit_list[h];
-            uint64_t key = keys[p];
-            gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
-            gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-            simplex x;
-            load_simplex(g.simplex, g.cap, h, x);
-            double out[10];
-            int r = epa_solve(A, B, x, pool[t], out);
-            if (r == 1) write_info(g.info, g.pair_cap, p, out);
-            g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
-            ovf = r == 3;
-        }
-        unsigned long long slot = wave_append(&counters[C_EPA_OVF], ovf);
-        if (ovf) ovf_list[slot] = h;
+        uint32_t p = g.hit_list[h];
+        uint64_t key = keys[p];
+        gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
+        gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        simplex x;
+        load_simplex(g.simplex, g.cap, h, x);
+        epa_store<kFastCap> store;
+        double out[10];
+        int r = epa_solve(A, B, x, store, out);
+        if (r == 1) write_info(g.info, g.pair_cap, p, out);
+        g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
+        ovf = r == 3;
     }
+    unsigned long long slot = wave_append(&counters[C_EPA_OVF], ovf);
+    if (ovf) ovf_list[slot] = h;
 }
 
 __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b,
@@ -404,14 +397,6 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 
 size_t epa_big_bytes() { return sizeof(epa_store<kBigCap>); }
 
-// EPA threads in flight (each owns one store of the pool)
-size_t epa_pool_threads(uint32_t n_hits)
-{
-    size_t t = grid(n_hits, kEpaBlock) * static_cast<size_t>(kEpaBlock);
-    return t < kEpaMaxThreads ? t : kEpaMaxThreads;
-}
-size_t epa_pool_bytes(uint32_t n_hits) { return epa_pool_threads(n_hits) * sizeof(epa_store<kFastCap>); }
-
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
 {
@@ -419,13 +404,9 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
 }
 
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
-                gjk_out_dev &g, void *pool, uint32_t *ovf_list, unsigned long long *counters,
-                hipStream_t st)
+                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
 {
-    if (!n_hits) return;
-    unsigned blocks = static_cast<unsigned>(epa_pool_threads(n_hits) / kEpaBlock);
-    k_epa<<<blocks, kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, static_cast<epa_store<kFastCap> *>(pool),
-                                        ovf_list, counters);
+    if (n_hits) k_epa<<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
 }
 
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,

@@ -259,8 +259,8 @@ __device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, si
 constexpr uint16_t kFNull = 0xFFFF;
 constexpr int kPolyCap = 68; // 4 simplex points + at most one per iteration (64)
 
-// one face per 64-byte record, so every face access touches a single cache line
-struct alignas(64) epa_face
+// one record per face: a face test reads one 48-byte record
+struct epa_face
 {
     v3 n;        // unit normal (zero for a degenerate triangle)
     double d;    // distance of the face plane from the origin

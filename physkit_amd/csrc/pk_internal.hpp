@@ -157,11 +157,8 @@ struct gjk_out_dev
 };
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
-// EPA on the compacted hits; pool holds epa_pool_bytes(n_hits) bytes of solve stores
-size_t epa_pool_bytes(uint32_t n_hits);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
-                gjk_out_dev &g, void *pool, uint32_t *ovf_list, unsigned long long *counters,
-                hipStream_t st);
+                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 size_t epa_big_bytes();
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
                     const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,

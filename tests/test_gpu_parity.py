@@ -147,6 +147,8 @@ def test_edge_worlds(pk, oracle):
         bodies([[0, 0, 0], [0.5, 0, 0]], [1, 1]),          # only static bodies (no pairs)
         bodies([[0, 3, 0], [0, 0, 0]], [0, 1]),            # a box falling on a static box
         bodies([[0, 0, 0], [0, 0, 0], [0, 0, 0]], [0, 0, 0]),  # coincident boxes
+        # slot 2 disabled (a removed body's arena slot) inside the falling box's path
+        bodies([[0, 3, 0], [0, 0, 0], [0, 2.5, 0], [0.2, 1.6, 0]], [0, 1, 2, 0]),
     ]
     for ci, s in enumerate(cases):
         o = oracle.OracleWorld(iterations=10, solver_mode=1)
@@ -157,6 +159,11 @@ def test_edge_worlds(pk, oracle):
             o.step(1 / 60)
             g.step(1 / 60)
             _compare_world((ci, k), g, o)
+            if ci == 4:
+                ids = np.concatenate([g.pairs() >> 32, g.pairs() & 0xFFFFFFFF])
+                assert not np.any(ids == 2), "a disabled slot paired"
+        if ci == 4:
+            assert np.array_equal(g.bodies()["pos"][2], [0, 2.5, 0])
         g.close()
 
 

@@ -329,6 +329,36 @@ def test_pair_set_closed_form_brute_force(oracle, pk):
         prev = w.pairs()
 
 
+def test_disabled_slot_never_pairs(oracle, pk):
+    """A removed body's slot (is_static=2) has no broadphase leaf (dynamic_bvh::remove_leaf,
+    collision/bvh.h:307): with every other body identical, the pair sets equal those of
+    the world without that body after renumbering, and the slot never moves."""
+    s, _ = _world(oracle, pk, 0, 200)
+    d = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in s.items()}
+    victim = 17
+    d["is_static"][victim] = 2
+    keep = np.array([i for i in range(len(s["shape"])) if i != victim])
+    r = dict(s)
+    for k in ("is_static", "shape", "pos", "vel", "angvel", "quat", "mass", "inertia", "restitution", "friction"):
+        r[k] = s[k][keep]
+    wd = oracle.OracleWorld(iterations=s["iterations"], solver_mode=1, flags=s["flags"])
+    wd.load_scene(d)
+    wr = oracle.OracleWorld(iterations=s["iterations"], solver_mode=1, flags=s["flags"])
+    wr.load_scene(r)
+    remap = np.where(np.arange(len(s["shape"])) > victim, np.arange(len(s["shape"])) - 1, np.arange(len(s["shape"])))
+    p0 = d["pos"][victim].copy()
+    for k in range(40):
+        wd.step(s["dt"])
+        wr.step(s["dt"])
+        pd = wd.pairs()
+        a, b = pd >> 32, pd & 0xFFFFFFFF
+        assert not np.any(a == victim) and not np.any(b == victim)
+        mapped = np.sort((remap[a].astype(np.uint64) << 32) | remap[b].astype(np.uint64))
+        np.testing.assert_array_equal(mapped, wr.pairs())
+        np.testing.assert_array_equal(wd.bodies()["pos"][keep], wr.bodies()["pos"])
+    assert np.array_equal(wd.bodies()["pos"][victim], p0)
+
+
 def test_all_moved_pairs_are_all_overlaps(oracle, pk):
     s, w = _world(oracle, pk, 1, 1500)
     w.step(s["dt"])
