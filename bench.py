@@ -82,10 +82,13 @@ def main():
     ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run a plain device copy of MIB MiB three times "
+                         "(the reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(each GPU would run its own worlds; see DESIGN.md on sharding)", file=sys.stderr)
+              "(each GPU would run its own worlds; multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
     import physkit_amd as pk
@@ -149,6 +152,16 @@ def main():
         "roofline": roof,
     }
     g.close()
+    if args.copy_check:
+        import torch
+        src = torch.empty(args.copy_check << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_check_bytes"] = args.copy_check << 20
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)

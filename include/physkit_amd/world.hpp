@@ -215,11 +215,12 @@ public:
             M_free.pop_back();
             M_slots[idx].desc = d;
             M_slots[idx].live = true;
+            M_slots[idx].fresh = true;
         }
         else
         {
             idx = static_cast<uint32_t>(M_slots.size());
-            M_slots.push_back({d, 0, true});
+            M_slots.push_back({d, 0, true, true});
         }
         M_dirty = true;
         return {idx, M_slots[idx].gen};
@@ -276,6 +277,7 @@ private:
         object_desc desc;
         uint32_t gen;
         bool live;
+        bool fresh; // created since the last upload: starts from its object_desc
     };
     world_desc M_desc;
     pkg_ctx *M_ctx = nullptr;
@@ -283,6 +285,7 @@ private:
     std::vector<uint32_t> M_free;
     std::vector<object_state> M_state;
     bool M_dirty = true, M_state_valid = false, M_uploaded = false;
+    size_t M_n_dev = 0; // bodies on the device (slot count at the last upload)
 
     void check(pkg_status st, const char *what) const
     {
@@ -295,7 +298,7 @@ private:
 
     void upload()
     {
-        if (M_uploaded) sync(); // keep the state reached so far
+        if (M_uploaded && !M_state_valid) download_state(); // keep the state reached so far
         const size_t n = M_slots.size();
         std::vector<uint32_t> offs{0};
         std::vector<double> verts;
@@ -319,7 +322,7 @@ private:
             is_static[i] = !live ? 2 : (d.type() == body_type::stat ? 1 : 0);
             vec3 p = d.pos(), v = d.vel(), w = d.ang_vel();
             quat o = d.orientation();
-            if (M_uploaded && i < M_state.size() && live)
+            if (M_uploaded && i < M_state.size() && live && !s.fresh)
             {
                 p = M_state[i].p;
                 v = M_state[i].v;
@@ -346,16 +349,23 @@ private:
         pkg_body_desc bd{is_static.data(), shape.data(), pos.data(), vel.data(), ang.data(), q.data(),
                          mass.data(), I.data(), e.data(), mu.data()};
         check(pkg_upload_bodies(M_ctx, static_cast<uint32_t>(n), &bd), "pkg_upload_bodies");
+        for (slot &s : M_slots) s.fresh = false;
         M_dirty = false;
         M_uploaded = true;
+        M_n_dev = n;
         M_state_valid = false;
     }
 
     void sync()
     {
-        if (M_state_valid) return;
         if (M_dirty) upload();
-        const size_t n = M_slots.size();
+        if (!M_state_valid) download_state();
+    }
+
+    // poses of the uploaded bodies (the slot count of the last upload)
+    void download_state()
+    {
+        const size_t n = M_n_dev;
         std::vector<double> pos(3 * n), vel(3 * n), ang(3 * n), q(4 * n);
         pkg_body_state out{pos.data(), vel.data(), ang.data(), q.data(), nullptr, nullptr};
         check(pkg_download_bodies(M_ctx, &out), "pkg_download_bodies");

@@ -434,6 +434,9 @@ class world:
         return len(self._descs) - 1
 
     def _upload(self):
+        # bodies already on the device keep the state they reached; new ones start
+        # from their object_desc
+        kept = self._gpu.bodies() if self._gpu is not None and self._gpu.n > 0 else None
         offs = [0]
         verts = []
         for d in self._descs:
@@ -453,6 +456,10 @@ class world:
             "restitution": np.array([d.restitution for d in self._descs], float),
             "friction": np.array([d.friction for d in self._descs], float),
         }
+        if kept is not None:
+            m = len(kept["pos"])
+            for k in ("pos", "vel", "angvel", "quat"):
+                s[k][:m] = kept[k]
         if self._gpu is None:
             self._gpu = GpuWorld(self.desc.gravity, self.desc.solver_iterations, self.desc.flags)
         self._gpu.load_scene(s)

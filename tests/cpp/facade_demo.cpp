@@ -19,6 +19,7 @@ static int fail(const char *what)
 
 int main()
 {
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
     {
         world w(world_desc::make().with_gravity({0, 0, 0}));
         auto box = mesh::box({0.5, 0.5, 0.5});

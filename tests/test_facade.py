@@ -7,7 +7,6 @@ resting box and arena slot reuse; the Python world API steps the same head-on sc
 import os
 import subprocess
 
-import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -22,6 +21,18 @@ def _build():
 def test_facade_compiles_and_links(pk):
     exe = _build()
     assert os.path.exists(exe)
+
+
+def test_facade_host_logic_with_test_double(tmp_path):
+    """Handles, slot reuse and re-upload keeping the state reached so far, with the
+    facade linked against a host-memory test double of the C ABI (tests/cpp/stub_pkg.cpp)."""
+    exe = str(tmp_path / "facade_logic")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-Wall", "-Wextra", "-fsanitize=address,undefined",
+                    "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "facade_logic.cpp"),
+                    os.path.join(ROOT, "tests", "cpp", "stub_pkg.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "facade logic ok" in r.stdout
 
 
 @pytest.mark.gpu
