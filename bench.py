@@ -56,6 +56,36 @@ def algorithmic_bytes(stage, s, pairs, hits_frac):
     raise ValueError(stage)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hulls_262k_fp64_hbm_bytes.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the default workload, from the committed
+    FETCH_SIZE/WRITE_SIZE counter passes (scaled by a copy kernel of known size)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
+
+
+def step_bytes(s, c, unique_vertices, scalar=8):
+    """Compulsory HBM bytes of one whole step (every persistent array read and written
+    once; grid-wide intermediates written and read once):
+    N(54s + 157 + 3s V_u) + S(6s + 4) + 24 P + 10s P_c + C(20s + 48)."""
+    n_static = int(s["n_static"])
+    n_dyn = len(s["shape"]) - n_static
+    vu = 0.0
+    if unique_vertices:
+        off = s["shape_offsets"].astype(np.int64)
+        vu = float(np.mean((off[1:] - off[:-1])[s["shape"]]))
+    return (n_dyn * (54 * scalar + 157 + 3 * scalar * vu) + n_static * (6 * scalar + 4) + 24 * c["pairs"]
+            + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
+
+
 def cpu_baseline(s, budget_s=12.0):
     """The oracle (FP64 CPU restatement, one thread) on the same workload, stepped until
     about budget_s seconds have elapsed (at least one step)."""
@@ -82,9 +112,9 @@ def main():
     ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
-                    help="after the timed steps, run a plain device copy of MIB MiB three times "
-                         "(the reference kernel for scaling HBM byte counters)")
+    ap.add_argument("--copy-check", type=int, default=1024, metavar="MIB",
+                    help="after the timed steps, time a plain device copy of MIB MiB (measured "
+                         "copy bandwidth; the calibration kernel for HBM byte counters); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -101,14 +131,10 @@ def main():
     n_dyn = len(s["shape"]) - int(s["n_static"])
     print(f"[bench] workload={args.workload} bodies={len(s['shape'])} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
+    # pkg_step / pkg_step_timed return only when the step's device work is complete, so
+    # the timed region is bracketed by device synchronisation on both sides
     for _ in range(args.warmup):
         g.step(dt)
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
     pairs_total = 0
     hits_total = 0
     stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
@@ -126,6 +152,8 @@ def main():
     pairs = g.pairs()
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
 
+    step_b = step_bytes(s, c, unique_vertices=(args.workload == "hulls"))
+
     # dominant kernel of the narrowphase and its roofline
     dom = max(("gjk", "epa"), key=lambda k: stage[k])
     t_launch = stage[dom] / 1e3 / args.steps
@@ -133,7 +161,8 @@ def main():
     achieved = alg / t_launch / 1e9 if t_launch > 0 else 0.0
     roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None, "algorithmic_bytes_per_launch": alg,
+            "traffic": measured_traffic("k_" + dom) if (args.workload == "hulls" and not args.n) else None,
+            "algorithmic_bytes_per_launch": alg,
             "avg_launch_ms": round(t_launch * 1e3, 4)}
     out = {
         "metric": METRIC,
@@ -148,20 +177,23 @@ def main():
                    "contact_pairs_per_s": round(pairs_total / wall, 1),
                    "colliding_pairs_per_s": round(hits_total / wall, 1),
                    "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stage.items()},
+                   "step_algorithmic_bytes": step_b,
+                   "step_hbm_frac": round(step_b / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 5),
                    "parallelism": "1 GPU, one world"},
         "roofline": roof,
     }
     g.close()
     if args.copy_check:
-        import torch
-        src = torch.empty(args.copy_check << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        src.fill_(1)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_check_bytes"] = args.copy_check << 20
+        import ctypes
+        from physkit_amd import build as pk_build
+        cc = ctypes.CDLL(pk_build.COPY_LIB)
+        cc.pkc_copy_check.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        ms = ctypes.c_double(0.0)
+        nbytes = args.copy_check << 20
+        if cc.pkc_copy_check(nbytes, 3, ctypes.byref(ms)) != 0:
+            raise RuntimeError("copy check failed")
+        out["copy_check"] = {"bytes": nbytes, "ms": round(ms.value, 4),
+                             "GB/s": round(2 * nbytes / ms.value / 1e6, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)

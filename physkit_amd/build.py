@@ -16,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libphyskit_amd.so")
+COPY_LIB = os.path.join(OUT_DIR, "libpk_copy_check.so")
 
 SOURCES = ["api.hip", "k_body.hip", "k_sort.hip", "k_broad.hip", "k_narrow.hip", "k_solver.hip",
            "scenes.cpp"]
@@ -49,6 +50,13 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr.strip():
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    # the copy-calibration kernel for HBM byte counters (bench.py --copy-check)
+    cmd = [hipcc(), *FLAGS, "-o", COPY_LIB + ".tmp", os.path.join(CSRC, "copy_check.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed (copy_check.hip)")
+    os.replace(COPY_LIB + ".tmp", COPY_LIB)
     return LIB
 
 
