@@ -40,7 +40,8 @@ def algorithmic_bytes(stage, s, pairs, hits_frac):
     """Compulsory HBM bytes of one launch of the narrowphase kernels (FP64).
     Per pair, GJK reads the key (8 B), each body's pose (pos 24 + quat 32), shape id and
     vertex offsets (12) and every vertex of both shapes once (24 B each), and writes a
-    status byte; a hit also writes its hit-list entry (4) and simplex (37 x 8).
+    status byte; a hit also writes its hit-list entry (4) and simplex (17 x 8: count, then
+    per point p.xyz and the packed support features).
     EPA re-reads, per hit, the hit-list entry, key, poses, offsets, vertices and
     simplex, and writes 10 doubles of contact data and a status byte."""
     off = s["shape_offsets"].astype(np.int64)
@@ -50,9 +51,9 @@ def algorithmic_bytes(stage, s, pairs, hits_frac):
     verts = 24.0 * (nv[a] + nv[b]).astype(np.float64)
     per_pair_fixed = 8 + 2 * (24 + 32 + 12)
     if stage == "gjk":
-        return float(np.sum(per_pair_fixed + verts + 1) + hits_frac * len(pairs) * (4 + 37 * 8))
+        return float(np.sum(per_pair_fixed + verts + 1) + hits_frac * len(pairs) * (4 + 17 * 8))
     if stage == "epa":
-        return float(hits_frac * np.sum(4 + per_pair_fixed + verts + 37 * 8 + 80 + 1))
+        return float(hits_frac * np.sum(4 + per_pair_fixed + verts + 17 * 8 + 80 + 1))
     raise ValueError(stage)
 
 

@@ -331,14 +331,14 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     uint32_t ncap = std::max<uint32_t>(mcap[nxt], 1);
     hit_list.reserve(ncap);
     ovf_list.reserve(ncap);
-    simplex.reserve(static_cast<size_t>(ncap) * 37);
+    simplex.reserve(static_cast<size_t>(ncap) * kSimplexWords);
     info.reserve(static_cast<size_t>(ncap) * 10);
     status.reserve(ncap);
     events.reserve(ncap);
     gjk_out_dev g;
     g.hit_list = hit_list.p;
     g.simplex = simplex.p;
-    g.cap = static_cast<uint32_t>(std::min(hit_list.cap, simplex.cap / 37));
+    g.cap = static_cast<uint32_t>(std::min(hit_list.cap, simplex.cap / kSimplexWords));
     g.info = info.p;
     g.status = status.p;
     g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
@@ -350,7 +350,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         record(ev, 9, st);
         launch_epa(keys[nxt].p, hits, b, s, g, ovf_list.p, counters.p, st);
         uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
-        if (novf)
+        if (novf) // rerun with the 4096-face store; capacity never changes a result
         {
             big_pool.reserve(static_cast<size_t>(novf) * epa_big_bytes());
             launch_epa_big(keys[nxt].p, ovf_list.p, novf, b, s, g, big_pool.p, counters.p, st);

@@ -16,8 +16,6 @@ namespace
 {
 constexpr int kBlock = 256;
 constexpr int kEpaBlock = 64;
-constexpr int kFastCap = 192;
-constexpr int kBigCap = 4096;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 __device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_dev &s, uint32_t id)
@@ -34,26 +32,25 @@ __device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_d
     return g;
 }
 
+// GJK simplex hand-off, planar: slot 0 = point count, then per point p.xyz and the
+// two support features packed in one 64-bit word (kSimplexWords doubles per hit).
 __device__ __forceinline__ void st_spt(double *d, uint32_t cap, uint32_t slot, int k, const spt &p)
 {
-    double *b = d + static_cast<size_t>(1 + 9 * k) * cap + slot;
+    double *b = d + static_cast<size_t>(1 + 4 * k) * cap + slot;
     b[0] = p.p.x;
     b[cap] = p.p.y;
     b[2 * static_cast<size_t>(cap)] = p.p.z;
-    b[3 * static_cast<size_t>(cap)] = p.pa.x;
-    b[4 * static_cast<size_t>(cap)] = p.pa.y;
-    b[5 * static_cast<size_t>(cap)] = p.pa.z;
-    b[6 * static_cast<size_t>(cap)] = p.pb.x;
-    b[7 * static_cast<size_t>(cap)] = p.pb.y;
-    b[8 * static_cast<size_t>(cap)] = p.pb.z;
+    b[3 * static_cast<size_t>(cap)] =
+        __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(p.ib) << 32) | p.ia));
 }
 __device__ __forceinline__ spt ld_spt(const double *d, uint32_t cap, uint32_t slot, int k)
 {
-    const double *b = d + static_cast<size_t>(1 + 9 * k) * cap + slot;
+    const double *b = d + static_cast<size_t>(1 + 4 * k) * cap + slot;
     spt p;
     p.p = mk(b[0], b[cap], b[2 * static_cast<size_t>(cap)]);
-    p.pa = mk(b[3 * static_cast<size_t>(cap)], b[4 * static_cast<size_t>(cap)], b[5 * static_cast<size_t>(cap)]);
-    p.pb = mk(b[6 * static_cast<size_t>(cap)], b[7 * static_cast<size_t>(cap)], b[8 * static_cast<size_t>(cap)]);
+    uint64_t f = static_cast<uint64_t>(__double_as_longlong(b[3 * static_cast<size_t>(cap)]));
+    p.ia = static_cast<uint32_t>(f);
+    p.ib = static_cast<uint32_t>(f >> 32);
     return p;
 }
 __device__ __forceinline__ void save_simplex(double *d, uint32_t cap, uint32_t slot, const simplex &x)
@@ -73,11 +70,17 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-__global__ void __launch_bounds__(kBlock)
+// GJK, kGjkGroup lanes per pair: the lanes of a group read consecutive vertices of
+// the same shape (coalesced) and reduce the support search; the simplex logic runs
+// identically in every lane of the group, and lane 0 writes the results.
+constexpr int kGjkGroup = 8;
+__global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
     k_gjk(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, gjk_out_dev g,
           unsigned long long *counters)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t p = t / kGjkGroup;
+    int gl = static_cast<int>(t % kGjkGroup);
     bool hit = false;
     simplex x;
     x.n = 0;
@@ -86,11 +89,12 @@ __global__ void __launch_bounds__(kBlock)
         uint64_t key = keys[p];
         gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
         gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-        hit = gjk(A, B, x);
-        g.status[p] = 0;
+        hit = gjk<kGjkGroup>(A, B, x, gl);
+        if (gl == 0) g.status[p] = 0;
     }
-    unsigned long long slot = wave_append(&counters[C_HITS], hit);
-    if (hit)
+    bool lead = hit && gl == 0;
+    unsigned long long slot = wave_append(&counters[C_HITS], lead);
+    if (lead)
     {
         g.hit_list[slot] = p;
         save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
@@ -102,7 +106,8 @@ __device__ __forceinline__ void write_info(double *info, uint32_t cap, uint32_t 
     for (int k = 0; k < 10; ++k) info[static_cast<size_t>(k) * cap + p] = o[k];
 }
 
-// one thread per hit, solve store in private memory
+// one thread per hit, 192-face store in private memory; a solve that needs more is
+// listed for the 4096-face kernel below
 __global__ void __launch_bounds__(kEpaBlock)
     k_epa(const uint64_t *keys, uint32_t n_hits, bodies_dev b, shapes_dev s, gjk_out_dev g,
           uint32_t *ovf_list, unsigned long long *counters)
@@ -117,7 +122,7 @@ __global__ void __launch_bounds__(kEpaBlock)
         gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
         simplex x;
         load_simplex(g.simplex, g.cap, h, x);
-        epa_store<kFastCap> store;
+        epa_store_fast store;
         double out[10];
         int r = epa_solve(A, B, x, store, out);
         if (r == 1) write_info(g.info, g.pair_cap, p, out);
@@ -129,8 +134,7 @@ __global__ void __launch_bounds__(kEpaBlock)
 }
 
 __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b,
-                          shapes_dev s, gjk_out_dev g, epa_store<kBigCap> *pool,
-                          unsigned long long *counters)
+                          shapes_dev s, gjk_out_dev g, epa_store_big *pool, unsigned long long *counters)
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_ovf) return;
@@ -364,7 +368,7 @@ __device__ void batch_one(const shape_pair_dev &sp, uint32_t i, double *out, int
     gshape A = pair_shape(sp.kind_a, sp.prm_a, sp.vbeg_a, sp.vend_a, sp.verts, i);
     gshape B = pair_shape(sp.kind_b, sp.prm_b, sp.vbeg_b, sp.vend_b, sp.verts, i);
     simplex x;
-    if (!gjk(A, B, x))
+    if (!gjk<1>(A, B, x, 0))
     {
         hit[i] = 0;
         return;
@@ -380,12 +384,12 @@ __global__ void __launch_bounds__(kEpaBlock) k_batch(shape_pair_dev sp, uint32_t
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    epa_store<kFastCap> store;
+    epa_store_fast store;
     batch_one(sp, i, out, hit, store);
 }
 
 __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t *hit,
-                            epa_store<kBigCap> *pool)
+                            epa_store_big *pool)
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 1) return;
@@ -395,12 +399,12 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 }
 } // namespace
 
-size_t epa_big_bytes() { return sizeof(epa_store<kBigCap>); }
+size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs) k_gjk<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    if (n_pairs) k_gjk<<<grid(static_cast<size_t>(n_pairs) * kGjkGroup), kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
 }
 
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
@@ -415,7 +419,7 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
 {
     if (n_ovf)
         k_epa_big<<<grid(n_ovf, 64), 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g,
-                                                  static_cast<epa_store<kBigCap> *>(pool), counters);
+                                                  static_cast<epa_store_big *>(pool), counters);
 }
 
 void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
@@ -433,7 +437,7 @@ void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int
 {
     if (!n) return;
     k_batch<<<grid(n, kEpaBlock), kEpaBlock, 0, st>>>(sp, n, out, hit);
-    if (big_pool) k_batch_big<<<1, 64, 0, st>>>(sp, n, out, hit, static_cast<epa_store<kBigCap> *>(big_pool));
+    if (big_pool) k_batch_big<<<1, 64, 0, st>>>(sp, n, out, hit, static_cast<epa_store_big *>(big_pool));
 }
 
 } // namespace pk

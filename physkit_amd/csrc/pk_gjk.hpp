@@ -1,18 +1,23 @@
 // GJK + EPA on the device (src/collision.cpp:12-518, include/physkit/collision/
 // collision.h:27-59). Same branch structure and operation order as the reference, so
 // verdicts, depths, normals and witness points are reproducible bit for bit.
-// The GJK simplex lives in registers (only constant indices). The EPA polytope,
-// faces and min-heap live in a caller-provided store of compile-time capacity;
-// running out of capacity is reported (status 3) and never truncates silently.
+// The GJK simplex lives in registers (only constant indices); a group of lanes can
+// share one GJK, splitting its support searches. The EPA polytope, faces and min-heap
+// live in a caller-provided store of compile-time capacity; running out of capacity
+// is reported (status 3) and never truncates silently.
 #pragma once
 #include "pk_math.hpp"
 
 namespace pk
 {
 
+// A support point keeps the Minkowski point and the support feature of each shape
+// (a mesh vertex index, or the corner code of a box); the witness points pa, pb
+// are recomputed from the features when needed, bit for bit what support() returned.
 struct spt
 {
-    v3 p, pa, pb;
+    v3 p;
+    uint32_t ia, ib;
 };
 
 // shape kinds: 0 aabb (a = min, b = max), 1 obb (a = center, b = half extents, q),
@@ -28,41 +33,75 @@ struct gshape
 
 __device__ __forceinline__ v3 ldv(const double *v, uint32_t i) { return mk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
 
-__device__ __forceinline__ v3 support(const gshape &s, v3 d)
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+
+// Support feature of s in direction d. G lanes of a group (G = 1: one thread) share
+// a shape: lane gl scans vertices gl, gl+G, ... and the group reduces. The result is
+// the reference's (mesh::support, mesh.cpp:341-358): the first vertex with a strictly
+// greater dot product, i.e. the lowest index among the maxima, except that a NaN dot
+// at vertex 0 keeps vertex 0 (no later value compares greater) and NaN dots elsewhere
+// are never taken.
+template <int G> __device__ __forceinline__ uint32_t support_idx(const gshape &s, v3 d, int gl)
 {
-    if (s.kind == 2) // mesh::instance::support, mesh.cpp:442-448 -> mesh::support :341-358
+    if (s.kind == 2) // mesh::instance::support, mesh.cpp:442-448
     {
         v3 ld = rotate(conj(s.q), d);
-        uint32_t best = 0;
-        double bd = dot(ldv(s.v, 0), ld);
-        for (uint32_t i = 1; i < s.nv; ++i)
+        double bv = 0.0;
+        uint32_t bi = kNoIdx;
+        bool nan0 = false;
+        for (uint32_t i = static_cast<uint32_t>(gl); i < s.nv; i += G)
         {
             double t = dot(ldv(s.v, i), ld);
-            if (t > bd)
+            if (i == 0) nan0 = t != t;
+            if (t == t && (bi == kNoIdx || t > bv))
             {
-                bd = t;
-                best = i;
+                bv = t;
+                bi = i;
             }
         }
-        return rotate(s.q, ldv(s.v, best)) + s.a;
+        for (int m = 1; m < G; m <<= 1)
+        {
+            double ov = __shfl_xor(bv, m, G);
+            uint32_t oi = __shfl_xor(bi, m, G);
+            int on = __shfl_xor(static_cast<int>(nan0), m, G);
+            bool take = oi != kNoIdx && (bi == kNoIdx || ov > bv || (!(bv > ov) && oi < bi));
+            if (take)
+            {
+                bv = ov;
+                bi = oi;
+            }
+            nan0 = nan0 || on != 0;
+        }
+        return nan0 ? 0u : bi;
     }
-    if (s.kind == 1) // obb::support, bounds.h:539-548
+    if (s.kind == 1) // obb::support, bounds.h:539-548: sign of the local direction
     {
         v3 ld = rotate(conj(s.q), d);
-        v3 sg = mk((ld.x >= 0 ? 1.0 : -1.0) * s.b.x, (ld.y >= 0 ? 1.0 : -1.0) * s.b.y,
-                   (ld.z >= 0 ? 1.0 : -1.0) * s.b.z);
-        return s.a + rotate(s.q, sg);
+        return (ld.x >= 0 ? 1u : 0u) | (ld.y >= 0 ? 2u : 0u) | (ld.z >= 0 ? 4u : 0u);
     }
     // aabb::support, bounds.h:188-199
-    return mk(d.x >= 0 ? s.b.x : s.a.x, d.y >= 0 ? s.b.y : s.a.y, d.z >= 0 ? s.b.z : s.a.z);
+    return (d.x >= 0 ? 1u : 0u) | (d.y >= 0 ? 2u : 0u) | (d.z >= 0 ? 4u : 0u);
 }
 
-__device__ __forceinline__ spt msupport(const gshape &A, const gshape &B, v3 d) // collision.h:41-49
+__device__ __forceinline__ v3 support_at(const gshape &s, uint32_t f)
+{
+    if (s.kind == 2) return rotate(s.q, ldv(s.v, f)) + s.a;
+    if (s.kind == 1)
+    {
+        v3 sg = mk(((f & 1u) ? 1.0 : -1.0) * s.b.x, ((f & 2u) ? 1.0 : -1.0) * s.b.y,
+                   ((f & 4u) ? 1.0 : -1.0) * s.b.z);
+        return s.a + rotate(s.q, sg);
+    }
+    return mk((f & 1u) ? s.b.x : s.a.x, (f & 2u) ? s.b.y : s.a.y, (f & 4u) ? s.b.z : s.a.z);
+}
+
+template <int G>
+__device__ __forceinline__ spt msupport(const gshape &A, const gshape &B, v3 d, int gl) // collision.h:41-49
 {
     spt r;
-    r.pa = support(A, d);
-    r.pb = support(B, -d);
-    r.p = r.pa - r.pb;
+    r.ia = support_idx<G>(A, d, gl);
+    r.ib = support_idx<G>(B, -d, gl);
+    r.p = support_at(A, r.ia) - support_at(B, r.ib);
     return r;
 }
 
@@ -71,17 +110,24 @@ struct simplex
     spt s0, s1, s2, s3;
     int n;
 };
+// Written as value selects: a branch per slot lets the optimiser merge the four
+// stores into one store through a selected pointer, which pins the simplex in
+// private memory (scratch) instead of registers.
+__device__ __forceinline__ v3 sel3(bool c, v3 a, v3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+__device__ __forceinline__ void sx_put(spt &dst, bool c, const spt &p)
+{
+    dst.p = sel3(c, p.p, dst.p);
+    dst.ia = c ? p.ia : dst.ia;
+    dst.ib = c ? p.ib : dst.ib;
+}
 __device__ __forceinline__ void sx_push(simplex &x, const spt &p)
 {
-    if (x.n == 0)
-        x.s0 = p;
-    else if (x.n == 1)
-        x.s1 = p;
-    else if (x.n == 2)
-        x.s2 = p;
-    else
-        x.s3 = p;
-    ++x.n;
+    int n = x.n;
+    sx_put(x.s0, n == 0, p);
+    sx_put(x.s1, n == 1, p);
+    sx_put(x.s2, n == 2, p);
+    sx_put(x.s3, n >= 3, p);
+    x.n = n + 1;
 }
 
 __device__ __forceinline__ void handle_line(simplex &x, v3 &dir) // collision.cpp:12-41
@@ -189,19 +235,21 @@ __device__ __forceinline__ bool handle_tetrahedron(simplex &x, v3 &dir) // colli
 }
 
 // boolean GJK, collision.cpp:165-189; returns true with the final simplex on a hit
-// (inlined so the simplex stays in registers)
-__device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x)
+// (inlined so the simplex stays in registers). With G > 1 the G lanes of a group run
+// it together: support searches are split over the lanes, everything else is
+// computed identically in every lane of the group.
+template <int G> __device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x, int gl)
 {
     const double eps = 1e-12;
     x.n = 0;
     v3 dir = mk(1.0, 0.0, 0.0);
-    spt p = msupport(A, B, dir);
+    spt p = msupport<G>(A, B, dir, gl);
     sx_push(x, p);
     if (sqn(p.p) < eps) return true;
     dir = -normalized(p.p);
     for (int it = 0; it < 100; ++it)
     {
-        spt np = msupport(A, B, dir);
+        spt np = msupport<G>(A, B, dir, gl);
         if (dot(np.p, dir) <= 0.0) return false;
         sx_push(x, np);
         if (x.n == 2)
@@ -215,14 +263,15 @@ __device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x
 }
 
 // collision.cpp:191-248
-__device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x)
+template <int G>
+__device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x, int gl)
 {
     int start = x.n;
     if (start <= 1)
     {
         v3 dir = mk(1.0, 0.0, 0.0);
-        spt p2 = msupport(A, B, dir);
-        if (sqn(p2.p - x.s0.p) < 1e-6) p2 = msupport(A, B, -dir);
+        spt p2 = msupport<G>(A, B, dir, gl);
+        if (sqn(p2.p - x.s0.p) < 1e-6) p2 = msupport<G>(A, B, -dir, gl);
         x.s1 = p2;
         x.n = 2;
     }
@@ -232,8 +281,8 @@ __device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, si
         v3 dir = cross(normalized(line), mk(0.0, 1.0, 0.0));
         if (sqn(dir) < 1e-6) dir = cross(normalized(line), mk(0.0, 0.0, 1.0));
         dir = normalized(dir);
-        spt p3 = msupport(A, B, dir);
-        if (sqn(cross(line, p3.p - x.s0.p)) < 1e-6) p3 = msupport(A, B, -dir);
+        spt p3 = msupport<G>(A, B, dir, gl);
+        if (sqn(cross(line, p3.p - x.s0.p)) < 1e-6) p3 = msupport<G>(A, B, -dir, gl);
         x.s2 = p3;
         x.n = 3;
     }
@@ -242,8 +291,8 @@ __device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, si
         v3 ab = x.s1.p - x.s0.p;
         v3 ac = x.s2.p - x.s0.p;
         v3 dir = normalized(cross(ab, ac));
-        spt p4 = msupport(A, B, dir);
-        if (fabs(dot(p4.p - x.s0.p, dir)) < 1e-6) p4 = msupport(A, B, -dir);
+        spt p4 = msupport<G>(A, B, dir, gl);
+        if (fabs(dot(p4.p - x.s0.p, dir)) < 1e-6) p4 = msupport<G>(A, B, -dir, gl);
         x.s3 = p4;
         x.n = 4;
     }
@@ -256,176 +305,167 @@ __device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, si
 
 
 // ---------------- EPA (collision.cpp:251-509) ----------------
-constexpr uint16_t kFNull = 0xFFFF;
-constexpr int kPolyCap = 68; // 4 simplex points + at most one per iteration (64)
-
-// one record per face: a face test reads one 48-byte record
-struct epa_face
+// One thread per solve. The polytope lives in a store of compile-time capacity
+// (FC faces, PC points, index type I) laid out field by field: the face normals and
+// plane distances, vertex and adjacency indices, the obsolete flags, and a min-heap
+// that keeps each slot's plane distance beside its face index so a sift step reads
+// one word instead of chasing the face. Running out of capacity returns 3 and the
+// caller reruns the solve with a larger store; capacity never changes a result.
+template <int FC, int PC, class I> struct epa_store
 {
-    v3 n;        // unit normal (zero for a degenerate triangle)
-    double d;    // distance of the face plane from the origin
-    uint16_t v[3];
-    uint16_t a[3]; // adjacent face across edge (v[i], v[i+1])
-    uint8_t obsolete;
+    static_assert(FC < (1 << (8 * sizeof(I))) && PC < (1 << (8 * sizeof(I))), "index type too small");
+    static constexpr int fcap = FC, pcap = PC;
+    using idx_t = I;
+    static constexpr I null = static_cast<I>(~I(0));
+    double nx[FC], ny[FC], nz[FC], d[FC]; // face normal and plane distance
+    double hd[FC];                        // heap: plane distance of the face in each slot
+    double px[PC], py[PC], pz[PC];        // polytope points
+    uint32_t pia[PC], pib[PC];            // their support features
+    I fv[3][FC], fa[3][FC];               // vertices; face across edge (v[i], v[i+1])
+    I hf[FC];                             // heap: face in each slot
+    I stk[FC], es[FC], ee[FC], eadj[FC];  // silhouette stack and horizon edges
+    uint8_t obs[FC];
 };
-struct epa_edge
-{
-    uint16_t s, e, adj;
-};
+using epa_store_fast = epa_store<192, 68, uint8_t>;    // every hull solve seen so far fits
+using epa_store_big = epa_store<4096, 68, uint16_t>;   // overflow path, global memory
 
-// A store is private to one EPA solve and contiguous, so the working set of a solve
-// is compact in memory.
-template <int FCAP> struct epa_store
-{
-    static constexpr int cap = FCAP;
-    epa_face f[FCAP];
-    spt poly[kPolyCap];
-    uint16_t heap[FCAP];
-    uint16_t stack[FCAP];
-    epa_edge h[FCAP];
-    uint16_t nf[FCAP];
-};
-
-template <class S> struct epa_ctx
+template <class S> struct epa_solver
 {
     S &m;
     int npoly = 0, nfaces = 0, nheap = 0;
+    __device__ explicit epa_solver(S &store) : m(store) {}
 
-    __device__ explicit epa_ctx(S &store) : m(store) {}
+    __device__ __forceinline__ v3 P(int i) const { return mk(m.px[i], m.py[i], m.pz[i]); }
+    __device__ __forceinline__ v3 N(int f) const { return mk(m.nx[f], m.ny[f], m.nz[f]); }
 
     __device__ void init_face(int fi, int i, int j, int k, int opp) // :273-297
     {
-        epa_face &f = m.f[fi];
-        uint16_t v1 = static_cast<uint16_t>(j), v2 = static_cast<uint16_t>(k);
-        v3 pi = m.poly[i].p;
-        v3 ab = m.poly[j].p - pi;
-        v3 ac = m.poly[k].p - pi;
+        int v1 = j, v2 = k;
+        v3 pi = P(i);
+        v3 ab = P(j) - pi;
+        v3 ac = P(k) - pi;
         v3 nrm = cross(ab, ac);
         if (sqn(nrm) < 1e-12)
             nrm = mk(0.0, 0.0, 0.0);
         else
             nrm = normalized(nrm);
-        if (opp >= 0 && dot(nrm, m.poly[opp].p - pi) > 0.0)
+        if (opp >= 0 && dot(nrm, P(opp) - pi) > 0.0)
         {
-            uint16_t t = v1;
+            int t = v1;
             v1 = v2;
             v2 = t;
             nrm = -nrm;
         }
-        f.v[0] = static_cast<uint16_t>(i);
-        f.v[1] = v1;
-        f.v[2] = v2;
-        f.a[0] = kFNull;
-        f.a[1] = kFNull;
-        f.a[2] = kFNull;
-        f.obsolete = 0;
-        f.n = nrm;
-        f.d = dot(nrm, pi);
+        m.fv[0][fi] = static_cast<typename S::idx_t>(i);
+        m.fv[1][fi] = static_cast<typename S::idx_t>(v1);
+        m.fv[2][fi] = static_cast<typename S::idx_t>(v2);
+        m.fa[0][fi] = S::null;
+        m.fa[1][fi] = S::null;
+        m.fa[2][fi] = S::null;
+        m.obs[fi] = 0;
+        m.nx[fi] = nrm.x;
+        m.ny[fi] = nrm.y;
+        m.nz[fi] = nrm.z;
+        m.d[fi] = dot(nrm, pi);
     }
-    __device__ void link(int f1, int f2, int va, int vb) // :305-313
+    __device__ __forceinline__ int vidx(int f, int va) const
+    { return (m.fv[0][f] == va) ? 0 : (m.fv[1][f] == va ? 1 : 2); }
+    __device__ __forceinline__ void link(int f1, int f2, int va, int vb) // :305-313
     {
-        epa_face &a = m.f[f1];
-        epa_face &b = m.f[f2];
-        int e1 = (a.v[0] == va) ? 0 : (a.v[1] == va ? 1 : 2);
-        int e2 = (b.v[0] == vb) ? 0 : (b.v[1] == vb ? 1 : 2);
-        a.a[e1] = static_cast<uint16_t>(f2);
-        b.a[e2] = static_cast<uint16_t>(f1);
+        int e1 = vidx(f1, va);
+        int e2 = vidx(f2, vb);
+        m.fa[e1][f1] = static_cast<typename S::idx_t>(f2);
+        m.fa[e2][f2] = static_cast<typename S::idx_t>(f1);
     }
-    // binary min-heap on face distance; sift order of libstdc++'s push_heap/pop_heap
-    __device__ bool before(int x, int y) const { return m.f[x].d > m.f[y].d; }
-    __device__ void sift_up(int hole, int top, uint16_t val)
+    // min-heap on plane distance, in the sift order of libstdc++'s push_heap/pop_heap
+    __device__ __forceinline__ void sift_up(int hole, int vf, double vd)
     {
-        double dv = m.f[val].d;
         int parent = (hole - 1) / 2;
-        while (hole > top)
+        while (hole > 0)
         {
-            uint16_t pv = m.heap[parent];
-            if (!(m.f[pv].d > dv)) break;
-            m.heap[hole] = pv;
+            double pd = m.hd[parent];
+            if (!(pd > vd)) break;
+            m.hd[hole] = pd;
+            m.hf[hole] = m.hf[parent];
             hole = parent;
             parent = (hole - 1) / 2;
         }
-        m.heap[hole] = val;
+        m.hd[hole] = vd;
+        m.hf[hole] = static_cast<typename S::idx_t>(vf);
     }
-    __device__ void push(int f)
+    __device__ __forceinline__ void push(int f)
     {
-        m.heap[nheap] = static_cast<uint16_t>(f);
         ++nheap;
-        sift_up(nheap - 1, 0, static_cast<uint16_t>(f));
+        sift_up(nheap - 1, f, m.d[f]);
     }
     __device__ int pop()
     {
         while (nheap > 0)
         {
             int len = nheap;
-            uint16_t top = m.heap[0];
+            int top = m.hf[0];
             if (len > 1)
             {
-                uint16_t val = m.heap[len - 1];
-                m.heap[len - 1] = top;
+                int vf = m.hf[len - 1];
+                double vd = m.hd[len - 1];
                 int n = len - 1;
                 int hole = 0, child = 0;
                 while (child < (n - 1) / 2)
                 {
                     child = 2 * (child + 1);
-                    uint16_t r = m.heap[child], l = m.heap[child - 1];
-                    if (before(r, l))
+                    double rd = m.hd[child], ld = m.hd[child - 1];
+                    if (rd > ld)
                     {
                         child--;
-                        r = l;
+                        rd = ld;
                     }
-                    m.heap[hole] = r;
+                    m.hd[hole] = rd;
+                    m.hf[hole] = m.hf[child];
                     hole = child;
                 }
                 if ((n & 1) == 0 && child == (n - 2) / 2)
                 {
                     child = 2 * (child + 1);
-                    m.heap[hole] = m.heap[child - 1];
+                    m.hd[hole] = m.hd[child - 1];
+                    m.hf[hole] = m.hf[child - 1];
                     hole = child - 1;
                 }
-                sift_up(hole, 0, val);
+                sift_up(hole, vf, vd);
             }
             --nheap;
-            if (!m.f[top].obsolete) return top;
+            if (!m.obs[top]) return top;
         }
         return -1;
     }
-    __device__ int alloc_face()
-    {
-        if (nfaces >= S::cap) return -1;
-        return nfaces++;
-    }
-    // :315-353; returns the number of horizon edges, -1 on overflow
+    // :315-353; horizon edges into es/ee/eadj; returns their count, -1 on overflow
     __device__ int silhouette(int f0, v3 p)
     {
         const double tol = 1e-6;
         int sp = 0, nh = 0;
-        m.stack[sp++] = static_cast<uint16_t>(f0);
-        m.f[f0].obsolete = 1;
+        m.stk[sp++] = static_cast<typename S::idx_t>(f0);
+        m.obs[f0] = 1;
         while (sp > 0)
         {
-            int cur = m.stack[--sp];
-            uint16_t cv[3] = {m.f[cur].v[0], m.f[cur].v[1], m.f[cur].v[2]};
-            uint16_t ca[3] = {m.f[cur].a[0], m.f[cur].a[1], m.f[cur].a[2]};
+            int cur = m.stk[--sp];
+            int cv[3] = {m.fv[0][cur], m.fv[1][cur], m.fv[2][cur]};
+            int ca[3] = {m.fa[0][cur], m.fa[1][cur], m.fa[2][cur]};
             for (int i = 0; i < 3; ++i)
             {
                 int nb = ca[i];
-                if (nb == kFNull) continue;
-                epa_face &g = m.f[nb];
-                if (g.obsolete) continue;
-                if (dot(g.n, p) > g.d + tol)
+                if (nb == S::null) continue;
+                if (m.obs[nb]) continue;
+                if (dot(N(nb), p) > m.d[nb] + tol)
                 {
-                    g.obsolete = 1;
-                    m.stack[sp++] = static_cast<uint16_t>(nb); // each face is pushed at most once
+                    m.obs[nb] = 1;
+                    m.stk[sp++] = static_cast<typename S::idx_t>(nb); // each face is pushed at most once
                 }
                 else
                 {
-                    if (nh >= S::cap) return -1;
-                    epa_edge ed;
-                    ed.s = cv[i];
-                    ed.e = cv[(i + 1) % 3];
-                    ed.adj = static_cast<uint16_t>(nb);
-                    m.h[nh++] = ed;
+                    if (nh >= S::fcap) return -1;
+                    m.es[nh] = static_cast<typename S::idx_t>(cv[i]);
+                    m.ee[nh] = static_cast<typename S::idx_t>(cv[(i + 1) % 3]);
+                    m.eadj[nh] = static_cast<typename S::idx_t>(nb);
+                    ++nh;
                 }
             }
         }
@@ -444,37 +484,37 @@ template <class S> struct epa_ctx
             for (int j = i + 1; j < 4; ++j)
                 for (int e1 = 0; e1 < 3; ++e1)
                 {
-                    int u1 = m.f[i].v[e1], v1 = m.f[i].v[(e1 + 1) % 3];
+                    int u1 = m.fv[e1][i], v1 = m.fv[(e1 + 1) % 3][i];
                     for (int e2 = 0; e2 < 3; ++e2)
                     {
-                        int u2 = m.f[j].v[e2], v2 = m.f[j].v[(e2 + 1) % 3];
+                        int u2 = m.fv[e2][j], v2 = m.fv[(e2 + 1) % 3][j];
                         if (u1 == v2 && v1 == u2)
                         {
-                            m.f[i].a[e1] = static_cast<uint16_t>(j);
-                            m.f[j].a[e2] = static_cast<uint16_t>(i);
+                            m.fa[e1][i] = static_cast<typename S::idx_t>(j);
+                            m.fa[e2][j] = static_cast<typename S::idx_t>(i);
                         }
                     }
                 }
     }
     // :424-454; out = normal3, world_a3, world_b3, depth
-    __device__ void barycentric(int fi, double *out) const
+    __device__ void barycentric(int fi, const gshape &A, const gshape &B, double *out) const
     {
-        const epa_face &f = m.f[fi];
-        const spt &p0 = m.poly[f.v[0]];
-        const spt &p1 = m.poly[f.v[1]];
-        const spt &p2 = m.poly[f.v[2]];
-        v3 nrm = f.n;
-        double fd = f.d;
+        int i0 = m.fv[0][fi], i1 = m.fv[1][fi], i2 = m.fv[2][fi];
+        const v3 q0 = P(i0), q1 = P(i1), q2 = P(i2);
+        const v3 p0a = support_at(A, m.pia[i0]), p1a = support_at(A, m.pia[i1]), p2a = support_at(A, m.pia[i2]);
+        const v3 p0b = support_at(B, m.pib[i0]), p1b = support_at(B, m.pib[i1]), p2b = support_at(B, m.pib[i2]);
+        v3 nrm = N(fi);
+        double fd = m.d[fi];
         v3 pm = nrm * fd;
-        v3 v0 = p1.p - p0.p, v1 = p2.p - p0.p, v2 = pm - p0.p;
+        v3 v0 = q1 - q0, v1 = q2 - q0, v2 = pm - q0;
         double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
         double d20 = dot(v2, v0), d21 = dot(v2, v1);
         double denom = d00 * d11 - d01 * d01;
         double v = (d11 * d20 - d01 * d21) / denom;
         double w = (d00 * d21 - d01 * d20) / denom;
         double u = 1.0 - v - w;
-        v3 wa = (p0.pa * u + p1.pa * v) + p2.pa * w;
-        v3 wb = (p0.pb * u + p1.pb * v) + p2.pb * w;
+        v3 wa = (p0a * u + p1a * v) + p2a * w;
+        v3 wb = (p0b * u + p1b * v) + p2b * w;
         out[0] = -nrm.x;
         out[1] = -nrm.y;
         out[2] = -nrm.z;
@@ -486,20 +526,27 @@ template <class S> struct epa_ctx
         out[8] = wb.z;
         out[9] = fd;
     }
+    __device__ __forceinline__ void put_point(int i, const spt &p)
+    {
+        m.px[i] = p.p.x;
+        m.py[i] = p.p.y;
+        m.pz[i] = p.p.z;
+        m.pia[i] = p.ia;
+        m.pib[i] = p.ib;
+    }
 };
 
 // epa_solver::solve (collision.cpp:411-504) from a GJK simplex.
 // returns 0 no contact (degenerate simplex or empty heap), 1 contact written to out,
 // 3 the store's capacity was exceeded (rerun with a larger store)
-template <class S>
-__device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store, double *out)
+template <class S> __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store, double *out)
 {
-    if (x.n < 4 && !pad_simplex(A, B, x)) return 0;
-    epa_ctx<S> e(store);
-    store.poly[0] = x.s0;
-    store.poly[1] = x.s1;
-    store.poly[2] = x.s2;
-    store.poly[3] = x.s3;
+    if (x.n < 4 && !pad_simplex<1>(A, B, x, 0)) return 0;
+    epa_solver<S> e(store);
+    e.put_point(0, x.s0);
+    e.put_point(1, x.s1);
+    e.put_point(2, x.s2);
+    e.put_point(3, x.s3);
     e.npoly = 4;
     e.initial_tetrahedron();
     const double tol = 1e-6;
@@ -507,43 +554,47 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
     {
         int mf = e.pop();
         if (mf < 0) break;
-        v3 nrm = store.f[mf].n;
-        double fd = store.f[mf].d;
-        spt p = msupport(A, B, nrm);
+        v3 nrm = e.N(mf);
+        double fd = store.d[mf];
+        spt p = msupport<1>(A, B, nrm, 0);
         double pd = dot(nrm, p.p);
         if (pd - fd < tol)
         {
-            e.barycentric(mf, out);
+            e.barycentric(mf, A, B, out);
             return 1;
         }
         int nh = e.silhouette(mf, p.p);
         if (nh < 0) return 3;
         if (nh == 0) break;
-        store.poly[e.npoly] = p;
+        if (e.npoly >= S::pcap || e.nfaces + nh > S::fcap) return 3;
         int pi = e.npoly++;
+        e.put_point(pi, p);
+        // new faces on the horizon, linked and pushed in edge order (:481-498)
+        int base = e.nfaces;
         for (int h = 0; h < nh; ++h)
         {
-            int f = e.alloc_face();
-            if (f < 0) return 3;
-            epa_edge ed = store.h[h];
-            e.init_face(f, ed.s, ed.e, pi, -1);
-            e.link(f, ed.adj, ed.s, ed.e);
+            int f = base + h;
+            e.init_face(f, store.es[h], store.ee[h], pi, -1);
+            e.link(f, store.eadj[h], store.es[h], store.ee[h]);
             e.push(f);
-            store.nf[h] = static_cast<uint16_t>(f);
         }
+        e.nfaces += nh;
         for (int i = 0; i < nh; ++i)
+        {
+            int hs_i = store.es[i], he_i = store.ee[i];
             for (int j = i + 1; j < nh; ++j)
             {
-                epa_edge hi = store.h[i], hj = store.h[j];
-                if (hi.e == hj.s)
-                    e.link(store.nf[i], store.nf[j], hi.e, pi);
-                else if (hi.s == hj.e)
-                    e.link(store.nf[j], store.nf[i], hj.e, pi);
+                int hs_j = store.es[j], he_j = store.ee[j];
+                if (he_i == hs_j)
+                    e.link(base + i, base + j, he_i, pi);
+                else if (hs_i == he_j)
+                    e.link(base + j, base + i, he_j, pi);
             }
+        }
     }
     int mf = e.pop();
     if (mf < 0) return 0;
-    e.barycentric(mf, out);
+    e.barycentric(mf, A, B, out);
     return 1;
 }
 

@@ -55,6 +55,7 @@ struct manifolds_dev
     double *data;
 };
 constexpr int kMF = 13; // normal3 local_a3 local_b3 depth ln lt1 lt2
+constexpr int kSimplexWords = 17; // GJK simplex hand-off per hit: count + 4 x (p.xyz, packed features)
 
 // device counters (one 64-bit slot each)
 enum counter_slot
