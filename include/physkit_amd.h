@@ -119,6 +119,16 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
  * starts with its exact world bounds, as dynamic_bvh::add stores them (bvh.h:294-302). */
 pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *bodies);
 
+/* Batched independent worlds. The reference steps one world per physkit::world
+ * (world.h:379-424); a caller with many worlds concatenates their bodies, uploads them
+ * once, and declares bodies [first_body[w], first_body[w+1]) as world w
+ * (first_body has n_worlds + 1 entries, from 0 to the body count). Bodies of
+ * different worlds never pair, and each world's result is the one it would reach
+ * stepped alone: pair keys stay global body ids (subtract first_body[w] for the
+ * world's own keys), and the solver colours each world from its own pair order.
+ * pkg_upload_bodies resets to a single world; this call resets pairs and manifolds. */
+pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first_body);
+
 /* Overwrites kinematic state of existing bodies (a caller moving objects between
  * steps). ids: count entries; arrays as in pkg_body_desc (pos/vel/angvel/quat). */
 pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids,
@@ -149,8 +159,9 @@ pkg_status pkg_get_events(pkg_ctx *ctx, uint64_t *keys, int32_t *kinds, size_t c
 pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out);
 
 /* Colours of the solver manifolds of the last step (manifolds with at least one
- * contact row, in pair-key order). */
-pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, size_t cap, size_t *n);
+ * contact row, in pair-key order), and optionally their pair keys. No counterpart in
+ * the reference (its solver is sequential, constraint.h:1107-1201): a test hook. */
+pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, uint64_t *keys, size_t cap, size_t *n);
 
 /* pkg_step with per-stage device time, measured with HIP events on the ctx stream.
  * Stages: 0 integrate_vel + leaf update, 1 LBVH build, 2 pair-set update, 3 GJK
@@ -183,11 +194,14 @@ typedef struct pkg_scene_info
     uint32_t iterations;
     uint32_t steps;
     uint32_t flags; /* pkg_config.flags the scene is meant to run with */
+    uint32_t n_worlds; /* independent worlds (kind 3), else 1 */
 } pkg_scene_info;
 
 /* kind 0: 1,024 random boxes over a ground box; kind 1: convex-hull clusters
- * (narrowphase stress); kind 2: box pyramids (solver stress). n_override != 0
- * changes the body count (kind 0, 1) or pyramid count (kind 2). */
+ * (narrowphase stress); kind 2: box pyramids (solver stress); kind 3: 4,096
+ * independent worlds of 16-4,096 bodies (spheres, boxes, hulls), for pkg_set_worlds.
+ * n_override != 0 changes the body count (kind 0, 1), pyramid count (kind 2) or
+ * world count (kind 3). */
 pkg_status pkg_scene_create(pkg_scene **out, int kind, uint64_t seed, uint32_t n_override);
 pkg_status pkg_scene_info_get(const pkg_scene *s, pkg_scene_info *info);
 /* Copies the scene into caller buffers sized from pkg_scene_info. */
@@ -195,6 +209,8 @@ pkg_status pkg_scene_copy(const pkg_scene *s, uint32_t *shape_offsets, double *v
                           uint8_t *is_static, uint32_t *shape, double *pos, double *vel,
                           double *angvel, double *quat, double *mass, double *inertia,
                           double *restitution, double *friction);
+/* first_body: n_worlds + 1 entries (the world partition for pkg_set_worlds). */
+pkg_status pkg_scene_copy_worlds(const pkg_scene *s, uint32_t *first_body);
 pkg_status pkg_scene_destroy(pkg_scene *s);
 
 #ifdef __cplusplus

@@ -34,7 +34,7 @@ EXPORTS = [
     "pkg_set_body_state", "pkg_step", "pkg_step_timed", "pkg_download_bodies", "pkg_get_pairs",
     "pkg_get_manifolds", "pkg_get_events", "pkg_get_counters", "pkg_get_colours",
     "pkg_gjk_epa_batch", "pkg_scene_create", "pkg_scene_info_get", "pkg_scene_copy",
-    "pkg_scene_destroy",
+    "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds",
 ]
 
 
@@ -65,7 +65,7 @@ class pkg_counters(C.Structure):
 class pkg_scene_info(C.Structure):
     _fields_ = [("n_shapes", C.c_uint32), ("n_verts", C.c_uint32), ("n_bodies", C.c_uint32),
                 ("n_static", C.c_uint32), ("dt", C.c_double), ("iterations", C.c_uint32),
-                ("steps", C.c_uint32), ("flags", C.c_uint32)]
+                ("steps", C.c_uint32), ("flags", C.c_uint32), ("n_worlds", C.c_uint32)]
 
 
 _lib = None
@@ -94,7 +94,9 @@ def lib():
     L.pkg_get_manifolds.argtypes = [P, I32, D, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_get_events.argtypes = [P, U64, I32, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_get_counters.argtypes = [P, C.POINTER(pkg_counters)]
-    L.pkg_get_colours.argtypes = [P, U32, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.pkg_get_colours.argtypes = [P, U32, U64, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.pkg_set_worlds.argtypes = [P, C.c_uint32, U32]
+    L.pkg_scene_copy_worlds.argtypes = [P, U32]
     L.pkg_gjk_epa_batch.argtypes = [P, C.c_uint32, I32, D, U32, U32, I32, D, U32, U32, D,
                                     C.c_uint32, D, I32]
     L.pkg_scene_create.argtypes = [C.POINTER(P), C.c_int, C.c_uint64, C.c_uint32]
@@ -117,7 +119,7 @@ def _f64(a):
 
 
 # ------------------------------------------------------------------ scenes
-SCENE_BOXES, SCENE_HULLS, SCENE_PYRAMIDS = 0, 1, 2
+SCENE_BOXES, SCENE_HULLS, SCENE_PYRAMIDS, SCENE_BATCHED = 0, 1, 2, 3
 SEED_BASE = 20261202
 
 
@@ -145,8 +147,10 @@ def scene(kind: int, seed: int | None = None, n_override: int = 0) -> dict:
                          _dp(s["pos"]), _dp(s["vel"]), _dp(s["angvel"]), _dp(s["quat"]),
                          _dp(s["mass"]), _dp(s["inertia"]), _dp(s["restitution"]),
                          _dp(s["friction"]))
+        fb = np.zeros(info.n_worlds + 1, np.uint32)
+        L.pkg_scene_copy_worlds(h, fb.ctypes.data_as(U32))
         s.update(dt=info.dt, iterations=info.iterations, steps=info.steps, flags=info.flags,
-                 n_static=info.n_static)
+                 n_static=info.n_static, first_body=fb)
         return s
     finally:
         L.pkg_scene_destroy(h)
@@ -257,14 +261,20 @@ class GpuWorld:
                                               n.value, C.byref(n)), "get_events")
         return k, t
 
-    def colours(self):
+    def colours(self, with_keys=False):
         n = C.c_size_t()
-        self._check(self.L.pkg_get_colours(self.h, None, 0, C.byref(n)), "get_colours")
+        self._check(self.L.pkg_get_colours(self.h, None, None, 0, C.byref(n)), "get_colours")
         out = np.zeros(n.value, np.uint32)
+        keys = np.zeros(n.value, np.uint64)
         if n.value:
-            self._check(self.L.pkg_get_colours(self.h, out.ctypes.data_as(U32), n.value,
-                                               C.byref(n)), "get_colours")
-        return out
+            self._check(self.L.pkg_get_colours(self.h, out.ctypes.data_as(U32), keys.ctypes.data_as(U64),
+                                               n.value, C.byref(n)), "get_colours")
+        return (out, keys) if with_keys else out
+
+    def set_worlds(self, first_body):
+        """Batched independent worlds: bodies [first_body[w], first_body[w+1]) form world w."""
+        fb = np.ascontiguousarray(first_body, dtype=np.uint32)
+        self._check(self.L.pkg_set_worlds(self.h, len(fb) - 1, fb.ctypes.data_as(U32)), "set_worlds")
 
     def counters(self):
         c = pkg_counters()

@@ -113,6 +113,9 @@ struct pkg_ctx
     dbuf<uint8_t> leaf_moved, covered;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
+    // batched worlds
+    uint32_t n_worlds = 1;
+    dbuf<uint32_t> world_of, wstart, pair_base;
     dbuf<uint64_t> comb_keys, comb_keys_alt, akeys;
     dbuf<uint32_t> comb_vals, comb_vals_alt, flag, fpos, bsums, total, srcb, asrc;
 
@@ -244,6 +247,10 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     t.visit = visit.p;
     t.defer = defer.p;
     t.scene = scene.p;
+    t.n_worlds = n_worlds;
+    t.wbits = id_bits(n_worlds);
+    t.world_of = n_worlds > 1 ? world_of.p : nullptr;
+    t.wstart = n_worlds > 1 ? wstart.p : nullptr;
     launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
     record(ev, 2, st);
     if (t.morton != morton.p)
@@ -385,13 +392,21 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         launch_contact_setup(keys[nxt].p, Pn, b, MF(nxt), r, dt, thr, colour.p, counters.p, st);
         if (read_counter(C_ROWS) > 0)
         {
+            const uint32_t *wo = nullptr, *pb = nullptr;
+            if (n_worlds > 1)
+            {
+                pair_base.reserve(n_worlds);
+                launch_pair_base(keys[nxt].p, Pn, world_of.p, pair_base.p, st);
+                wo = world_of.p;
+                pb = pair_base.p;
+            }
             uint32_t c = 0;
             while (true)
             {
                 launch_fill_u64(claim.p, ~0ull, n, st);
-                launch_colour_claim(keys[nxt].p, Pn, b, colour.p, claim.p, st);
+                launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
                 reset_counter(C_REMAIN);
-                launch_colour_assign(keys[nxt].p, Pn, b, colour.p, claim.p, c, counters.p, st);
+                launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, c, counters.p, st);
                 ++c;
                 if (read_counter(C_REMAIN) == 0) break;
                 if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
@@ -605,8 +620,37 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         launch_body_init(ctx->B(), ctx->S(), st);
         ctx->P[0] = ctx->P[1] = 0;
         ctx->cur = 0;
+        ctx->n_worlds = 1;
         ctx->have_bodies = true;
         hip_check(hipStreamSynchronize(st), "body upload");
+    })
+}
+
+pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first_body)
+{
+    if (!ctx || !n_worlds || !first_body || !ctx->have_bodies) return PKG_INVALID_ARG;
+    if (first_body[0] != 0 || first_body[n_worlds] != ctx->n)
+    {
+        ctx->err = "first_body must start at 0 and end at the body count";
+        return PKG_INVALID_ARG;
+    }
+    for (uint32_t w = 0; w < n_worlds; ++w)
+        if (first_body[w + 1] < first_body[w])
+        {
+            ctx->err = "first_body must be non-decreasing";
+            return PKG_INVALID_ARG;
+        }
+    PKG_GUARD(ctx, {
+        hipStream_t st = ctx->st;
+        std::vector<uint32_t> wo(ctx->n);
+        for (uint32_t w = 0; w < n_worlds; ++w)
+            for (uint32_t i = first_body[w]; i < first_body[w + 1]; ++i) wo[i] = w;
+        upload(ctx->world_of, wo.data(), wo.size(), st);
+        upload(ctx->wstart, first_body, static_cast<size_t>(n_worlds) + 1, st);
+        hip_check(hipStreamSynchronize(st), "world upload");
+        ctx->n_worlds = n_worlds;
+        ctx->P[0] = ctx->P[1] = 0; // pairs restart under the new partition
+        ctx->cur = 0;
     })
 }
 
@@ -770,7 +814,7 @@ pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out)
     return PKG_OK;
 }
 
-pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, size_t cap, size_t *n)
+pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, uint64_t *keys, size_t cap, size_t *n)
 {
     if (!ctx || !n) return PKG_INVALID_ARG;
     PKG_GUARD(ctx, {
@@ -779,10 +823,16 @@ pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, size_t cap, size_t *
         if (ctx->n_colours && P)
         {
             std::vector<uint32_t> col = download(ctx->colour, P, ctx->st);
+            std::vector<uint64_t> k;
+            if (keys) k = download(ctx->keys[ctx->cur], P, ctx->st);
             for (uint32_t p = 0; p < P; ++p)
                 if (col[p] != kNone)
                 {
-                    if (colours && cnt < cap) colours[cnt] = col[p];
+                    if (cnt < cap)
+                    {
+                        if (colours) colours[cnt] = col[p];
+                        if (keys) keys[cnt] = k[p];
+                    }
                     ++cnt;
                 }
         }

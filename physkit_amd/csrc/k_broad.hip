@@ -117,7 +117,7 @@ __device__ __forceinline__ uint64_t split3(uint64_t x)
     return x;
 }
 
-__global__ void k_morton(bodies_dev b, const unsigned long long *scene, uint64_t *keys, uint32_t *ids)
+__global__ void k_morton(bodies_dev b, lbvh_dev t, const unsigned long long *scene, uint64_t *keys, uint32_t *ids)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.n) return;
@@ -131,7 +131,10 @@ __global__ void k_morton(bodies_dev b, const unsigned long long *scene, uint64_t
         uint64_t x = static_cast<uint64_t>(u * 2097152.0);
         q[k] = x > 2097151ull ? 2097151ull : x;
     }
-    keys[i] = (split3(q[0]) << 2) | (split3(q[1]) << 1) | split3(q[2]);
+    uint64_t m = (split3(q[0]) << 2) | (split3(q[1]) << 1) | split3(q[2]);
+    if (t.n_worlds > 1) // world id in the top bits: each world's leaves sort together
+        m = (static_cast<uint64_t>(t.world_of[i]) << (63 - t.wbits)) | (m >> t.wbits);
+    keys[i] = m;
     ids[i] = i;
 }
 
@@ -330,6 +333,16 @@ __global__ void __launch_bounds__(kBlock)
             qhi[k] = f_up(p[3 + k]);
         }
     }
+    // leaves of other worlds are never visited: the node's leaf range must meet the
+    // sorted range of the querying leaf's world
+    const bool batched = t.n_worlds > 1;
+    uint32_t wlo = 0, whi = n - 1;
+    if (active && batched)
+    {
+        uint32_t w = t.world_of[a];
+        wlo = t.wstart[w];
+        whi = t.wstart[w + 1] - 1;
+    }
     uint32_t node = active ? 0u : kNone;
     for (uint32_t guard = 0; node != kNone; ++guard)
     {
@@ -345,6 +358,7 @@ __global__ void __launch_bounds__(kBlock)
             bool ov = lo.x <= qhi[0] && hi.x >= qlo[0] && lo.y <= qhi[1] && hi.y >= qlo[1] && lo.z <= qhi[2] &&
                       hi.z >= qlo[2];
             next = __float_as_uint(hi.w);
+            if (ov && batched) ov = t.first[node] <= whi && (t.lastf[node] & 0x7fffffffu) >= wlo;
             if (ov)
             {
                 uint32_t lf = t.lastf[node];
@@ -356,7 +370,7 @@ __global__ void __launch_bounds__(kBlock)
         {
             uint32_t j = node - (n - 1);
             next = t.escape[node];
-            if (j != i)
+            if (j != i && j >= wlo && j <= whi)
             {
                 const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
                 box3 lb;
@@ -466,7 +480,7 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     k_scene_init<<<1, 64, 0, st>>>(t.scene);
     unsigned sb = grid(n) < 512 ? grid(n) : 512;
     k_scene_bounds<<<sb, kBlock, 0, st>>>(b, t.scene);
-    k_morton<<<grid(n), kBlock, 0, st>>>(b, t.scene, t.morton, t.ids);
+    k_morton<<<grid(n), kBlock, 0, st>>>(b, t, t.scene, t.morton, t.ids);
     static const int shifts[8] = {0, 8, 16, 24, 32, 40, 48, 56};
     if (radix_sort_u64(t.morton, t.ids, t.morton_alt, t.ids_alt, n, shifts, 8, sc, st))
     {

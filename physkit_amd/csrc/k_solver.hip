@@ -251,20 +251,34 @@ __global__ void __launch_bounds__(kBlock)
 __device__ __forceinline__ uint64_t prio(uint32_t p)
 { return (static_cast<uint64_t>(mix32(p)) << 32) | p; }
 
+__device__ __forceinline__ uint32_t local_index(uint32_t p, uint32_t ia, const uint32_t *world_of,
+                                                const uint32_t *pair_base)
+{ return pair_base ? p - pair_base[world_of[ia]] : p; }
+
+// first pair of each world in the sorted pair list (worlds without pairs are not written)
+__global__ void k_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    uint32_t w = world_of[static_cast<uint32_t>(keys[p] >> 32)];
+    if (p == 0 || world_of[static_cast<uint32_t>(keys[p - 1] >> 32)] != w) pair_base[w] = p;
+}
+
 __global__ void k_claim(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, const uint32_t *colour,
-                        unsigned long long *claim)
+                        const uint32_t *world_of, const uint32_t *pair_base, unsigned long long *claim)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs || colour[p] != kUncoloured) return;
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-    unsigned long long pr = prio(p);
+    unsigned long long pr = prio(local_index(p, ia, world_of, pair_base));
     if (!b.is_static[ia]) atomicMin(&claim[ia], pr);
     if (!b.is_static[ib]) atomicMin(&claim[ib], pr);
 }
 
 __global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, uint32_t *colour,
-                         const unsigned long long *claim, uint32_t c, unsigned long long *counters)
+                         const uint32_t *world_of, const uint32_t *pair_base, const unsigned long long *claim,
+                         uint32_t c, unsigned long long *counters)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool remain = false;
@@ -272,7 +286,7 @@ __global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, u
     {
         uint64_t key = keys[p];
         uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        unsigned long long pr = prio(p);
+        unsigned long long pr = prio(local_index(p, ia, world_of, pair_base));
         bool ok = (b.is_static[ia] || claim[ia] == pr) && (b.is_static[ib] || claim[ib] == pr);
         if (ok)
             colour[p] = c;
@@ -392,16 +406,25 @@ void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_d
 {
     if (n_pairs) k_setup<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, r, dt, threshold, colour, counters);
 }
-void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                         const uint32_t *colour, unsigned long long *claim, hipStream_t st)
+void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
+                      hipStream_t st)
 {
-    if (n_pairs) k_claim<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, claim);
+    if (n_pairs) k_pair_base<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, world_of, pair_base);
+}
+void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                         unsigned long long *claim, hipStream_t st)
+{
+    if (n_pairs) k_claim<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim);
 }
 void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                          uint32_t *colour, const unsigned long long *claim, uint32_t c,
-                          unsigned long long *counters, hipStream_t st)
+                          uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                          const unsigned long long *claim, uint32_t c, unsigned long long *counters,
+                          hipStream_t st)
 {
-    if (n_pairs) k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, claim, c, counters);
+    if (n_pairs)
+        k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim, c,
+                                                   counters);
 }
 void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
                           uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,

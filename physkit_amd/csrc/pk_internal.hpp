@@ -127,6 +127,13 @@ struct lbvh_dev
     uint32_t *visit;     // n-1 refit arrival counters
     uint32_t *defer;     // 2n: refit arrivals at nodes spanning workgroup chunks
     unsigned long long *scene; // 6 ordered-integer bounds of leaf centroids
+    // batched independent worlds (n_worlds > 1): bodies [wstart[w], wstart[w+1]) form
+    // world w; Morton keys carry the world in their top wbits bits, so world w's leaves
+    // are sorted positions [wstart[w], wstart[w+1]) and the walk prunes other worlds
+    uint32_t n_worlds;
+    int wbits;
+    const uint32_t *world_of; // n: world of each body
+    const uint32_t *wstart;   // n_worlds + 1
 };
 void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
                        unsigned long long *counters, hipStream_t st);
@@ -191,11 +198,18 @@ constexpr int kRowF = 38;
 void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
                           manifolds_dev m, rows_dev r, double dt, double threshold,
                           uint32_t *colour, unsigned long long *counters, hipStream_t st);
+// With batched worlds, a manifold's colouring priority uses its index within its own
+// world's pair list (p - pair_base[world]), so each world colours as if stepped alone;
+// world_of and pair_base are null for a single world.
+void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
+                      hipStream_t st);
 void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                         const uint32_t *colour, unsigned long long *claim, hipStream_t st);
+                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                         unsigned long long *claim, hipStream_t st);
 void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                          uint32_t *colour, const unsigned long long *claim, uint32_t c,
-                          unsigned long long *counters, hipStream_t st);
+                          uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                          const unsigned long long *claim, uint32_t c, unsigned long long *counters,
+                          hipStream_t st);
 void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
                           uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,
                           uint32_t *order, hipStream_t st);

@@ -17,6 +17,7 @@ struct pkg_scene
     std::vector<uint8_t> is_static;
     std::vector<uint32_t> shape;
     std::vector<double> pos, vel, angvel, quat, mass, inertia, restitution, friction;
+    std::vector<uint32_t> first_body{0}; // world w = bodies [first_body[w], first_body[w+1])
     pkg_scene_info info{};
 };
 
@@ -212,6 +213,105 @@ void scene_pyramids(pkg_scene &s, uint64_t seed, uint32_t n_pyr)
     // NaN-witness contacts; this scene runs with the documented guard
     s.info.flags = PKG_FLAG_EPA_GUARD;
 }
+// vertices of mesh::sphere(r, stacks, sectors) (src/mesh.cpp:60-115): north pole, the
+// rings from the top down, south pole
+uint32_t add_sphere_shape(pkg_scene &s, double rad, unsigned stacks, unsigned sectors)
+{
+    s.verts.push_back(0.0);
+    s.verts.push_back(rad);
+    s.verts.push_back(0.0);
+    for (unsigned i = 1; i < stacks; ++i)
+    {
+        double phi = kPi * static_cast<double>(i) / static_cast<double>(stacks);
+        for (unsigned j = 0; j < sectors; ++j)
+        {
+            double th = 2.0 * kPi * static_cast<double>(j) / static_cast<double>(sectors);
+            s.verts.push_back(rad * std::sin(phi) * std::cos(th));
+            s.verts.push_back(rad * std::cos(phi));
+            s.verts.push_back(rad * std::sin(phi) * std::sin(th));
+        }
+    }
+    s.verts.push_back(0.0);
+    s.verts.push_back(-rad);
+    s.verts.push_back(0.0);
+    s.shape_offsets.push_back(static_cast<uint32_t>(s.verts.size() / 3));
+    return static_cast<uint32_t>(s.shape_offsets.size() - 2);
+}
+
+// hull of k points on an ellipsoid (all of them are vertices of the hull)
+uint32_t add_ellipsoid_hull(pkg_scene &s, rng &r, uint32_t k, double a, double b, double c)
+{
+    for (uint32_t j = 0; j < k; ++j)
+    {
+        double z = r.uni(-1.0, 1.0), phi = r.uni(0.0, 2.0 * kPi);
+        double rho = std::sqrt(1.0 - z * z);
+        s.verts.push_back(a * rho * std::cos(phi));
+        s.verts.push_back(b * rho * std::sin(phi));
+        s.verts.push_back(c * z);
+    }
+    s.shape_offsets.push_back(static_cast<uint32_t>(s.verts.size() / 3));
+    return static_cast<uint32_t>(s.shape_offsets.size() - 2);
+}
+
+// Batched uneven worlds: n_worlds independent worlds with body counts log-uniform in
+// [16, 4096]; each is the boxes scene scaled to its size (cube edge 20 m x
+// (N/1024)^(1/3), ground box half extent 2.5 x edge) with a body mix of 40% UV
+// spheres (r U[0.05,0.5], 16 stacks x 32 sectors), 40% boxes (half extents
+// U[0.1,0.5]) and 20% hulls of 8-16 points on ellipsoids (semi-axes U[0.1,0.5]);
+// density 1000, v ~ N(0,1), uniform rotations, e = mu = 0.5.
+void scene_batched(pkg_scene &s, uint64_t seed, uint32_t n_worlds)
+{
+    rng r(seed);
+    double I[9];
+    for (uint32_t w = 0; w < n_worlds; ++w)
+    {
+        uint32_t nb = static_cast<uint32_t>(16.0 * std::pow(256.0, r.u01()));
+        double edge = 20.0 * std::cbrt(static_cast<double>(nb) / 1024.0);
+        double gh = 2.5 * edge;
+        uint32_t g = add_box_shape(s, gh, 0.5, gh);
+        const double gp[3] = {0.0, -0.5, 0.0};
+        diag(I, 1, 1, 1);
+        add_body(s, true, g, gp, kZero3, kIdentityQ, 0.0, I, 0.5, 0.5);
+        for (uint32_t i = 0; i < nb; ++i)
+        {
+            double kind = r.u01();
+            double p[3] = {r.uni(-0.5 * edge, 0.5 * edge), r.uni(1.0, 1.0 + edge), r.uni(-0.5 * edge, 0.5 * edge)};
+            double q[4];
+            r.quat(q);
+            double v[3] = {r.normal(), r.normal(), r.normal()};
+            uint32_t sh;
+            double m;
+            if (kind < 0.4)
+            {
+                double rad = r.uni(0.05, 0.5);
+                sh = add_sphere_shape(s, rad, 16, 32);
+                m = 1000.0 * (4.0 / 3.0) * kPi * rad * rad * rad;
+                diag(I, 0.4 * m * rad * rad, 0.4 * m * rad * rad, 0.4 * m * rad * rad);
+            }
+            else if (kind < 0.8)
+            {
+                double hx = r.uni(0.1, 0.5), hy = r.uni(0.1, 0.5), hz = r.uni(0.1, 0.5);
+                sh = add_box_shape(s, hx, hy, hz);
+                m = 1000.0 * 8.0 * hx * hy * hz;
+                diag(I, m / 3.0 * (hy * hy + hz * hz), m / 3.0 * (hx * hx + hz * hz), m / 3.0 * (hx * hx + hy * hy));
+            }
+            else
+            {
+                uint32_t k = 8 + static_cast<uint32_t>(r.next() % 9);
+                double a = r.uni(0.1, 0.5), b = r.uni(0.1, 0.5), c = r.uni(0.1, 0.5);
+                sh = add_ellipsoid_hull(s, r, k, a, b, c);
+                m = 1000.0 * (4.0 / 3.0) * kPi * a * b * c;
+                diag(I, m / 5.0 * (b * b + c * c), m / 5.0 * (a * a + c * c), m / 5.0 * (a * a + b * b));
+            }
+            add_body(s, false, sh, p, v, q, m, I, 0.5, 0.5);
+        }
+        s.first_body.push_back(static_cast<uint32_t>(s.shape.size()));
+    }
+    s.info.dt = 1.0 / 60.0;
+    s.info.iterations = 10;
+    s.info.steps = 120;
+    s.info.flags = 0;
+}
 } // namespace
 
 extern "C"
@@ -226,8 +326,11 @@ pkg_status pkg_scene_create(pkg_scene **out, int kind, uint64_t seed, uint32_t n
     case 0: scene_boxes(*s, seed, n_override ? n_override : 1024); break;
     case 1: scene_hulls(*s, seed, n_override ? n_override : 262144); break;
     case 2: scene_pyramids(*s, seed, n_override ? n_override : 64); break;
+    case 3: scene_batched(*s, seed, n_override ? n_override : 4096); break;
     default: delete s; return PKG_INVALID_ARG;
     }
+    if (kind != 3) s->first_body.push_back(static_cast<uint32_t>(s->shape.size()));
+    s->info.n_worlds = static_cast<uint32_t>(s->first_body.size() - 1);
     s->info.n_shapes = static_cast<uint32_t>(s->shape_offsets.size() - 1);
     s->info.n_verts = static_cast<uint32_t>(s->verts.size() / 3);
     s->info.n_bodies = static_cast<uint32_t>(s->shape.size());
@@ -267,6 +370,13 @@ pkg_status pkg_scene_copy(const pkg_scene *s, uint32_t *shape_offsets, double *v
     cp(inertia, s->inertia);
     cp(restitution, s->restitution);
     cp(friction, s->friction);
+    return PKG_OK;
+}
+
+pkg_status pkg_scene_copy_worlds(const pkg_scene *s, uint32_t *first_body)
+{
+    if (!s || !first_body) return PKG_INVALID_ARG;
+    std::memcpy(first_body, s->first_body.data(), s->first_body.size() * sizeof(uint32_t));
     return PKG_OK;
 }
 

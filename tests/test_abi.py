@@ -44,3 +44,25 @@ def test_scene_shapes(pk):
     assert nv.min() >= 8 and nv.max() <= 32
     p = pk.scene(2)
     assert len(p["shape"]) == 64 * 120 + 1
+
+
+def test_batched_scene_partition(pk):
+    s = pk.scene(pk.SCENE_BATCHED, 777, 16)
+    fb = s["first_body"].astype(np.int64)
+    sizes = np.diff(fb) - 1  # one static ground box per world
+    assert fb[0] == 0 and fb[-1] == len(s["shape"])
+    assert sizes.min() >= 16 and sizes.max() < 4096
+    shp = s["shape"].astype(np.int64)
+    offs = s["shape_offsets"].astype(np.int64)
+    nv = (offs[1:] - offs[:-1])[shp]
+    for w in range(len(fb) - 1):
+        b0, b1 = fb[w], fb[w + 1]
+        assert s["is_static"][b0] == 1 and not s["is_static"][b0 + 1:b1].any()
+        # shapes of a world are contiguous and private to it
+        assert np.array_equal(np.sort(shp[b0:b1]), np.arange(shp[b0], shp[b0] + (b1 - b0)))
+    dyn = ~s["is_static"].astype(bool)
+    frac_sphere = np.mean(nv[dyn] == 2 + 15 * 32)
+    frac_box = np.mean(nv[dyn] == 8)
+    frac_hull = np.mean((nv[dyn] >= 9) & (nv[dyn] <= 16))
+    assert abs(frac_sphere - 0.4) < 0.03 and abs(frac_hull - 0.2 * 8 / 9) < 0.03
+    assert abs(frac_box - (0.4 + 0.2 / 9)) < 0.03
