@@ -33,6 +33,9 @@ WORKLOADS = {
     "boxes": (0, "1,024 random boxes over a static ground box, dt=1/60, 10 PGS iterations, FP64"),
     "pyramids": (2, "64 pyramids of 15-layer unit-box stacks (7,680 boxes), dt=1/120, "
                     "20 PGS iterations, FP64"),
+    "batched": (3, "4,096 independent worlds of 16-4,096 bodies (log-uniform; 40% UV spheres "
+                   "16x32, 40% boxes, 20% 8-16-point hulls) over static ground boxes, dt=1/60, "
+                   "10 PGS iterations, one batched step per launch sequence, FP64"),
 }
 
 
@@ -87,6 +90,44 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
+def cpu_baseline_batched(s, warmup, budget_s=12.0):
+    """The oracle on the batched workload, its independent worlds stepped by a pool of
+    host threads (the oracle releases the interpreter lock inside a step). Worlds are
+    taken in order, in chunks: each chunk's worlds are loaded and given the bench's
+    warmup steps untimed, then the next step of every world in the chunk (the first
+    step the GPU times) is timed as wall time, until about budget_s seconds of timed
+    stepping."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+    import batched_util
+    from concurrent.futures import ThreadPoolExecutor
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    fb = s["first_body"]
+    n_worlds = len(fb) - 1
+
+    def setup(w):
+        ws, b0, b1 = batched_util.world_slice(s, w)
+        o = oracle_ffi.OracleWorld(iterations=ws["iterations"], solver_mode=1, flags=ws["flags"])
+        o.load_scene(ws)
+        o.step(ws["dt"], warmup)
+        return o, int(ws["n_static"]), int(b1 - b0)
+
+    bodies, timed, w = 0, 0.0, 0
+    with ThreadPoolExecutor(cores) as ex:
+        while w < n_worlds and timed < budget_s:
+            chunk = list(range(w, min(w + 4 * cores, n_worlds)))
+            worlds = list(ex.map(setup, chunk))
+            t0 = time.perf_counter()
+            list(ex.map(lambda x: x[0].step(s["dt"], 1), worlds))
+            timed += time.perf_counter() - t0
+            bodies += sum(nb - ns for _, ns, nb in worlds)
+            w += len(chunk)
+    return {"value": bodies / timed, "unit": "body-steps/s", "cores": cores, "kind": "port",
+            "sample": f"step {warmup + 1} of worlds 0-{w - 1} of {n_worlds} "
+                      f"({bodies} dynamic bodies), {timed:.1f} s of wall time on {cores} threads"}
+
+
 def cpu_baseline(s, budget_s=12.0):
     """The oracle (FP64 CPU restatement, one thread) on the same workload, stepped until
     about budget_s seconds have elapsed (at least one step)."""
@@ -128,6 +169,8 @@ def main():
     s = pk.scene(kind, None, args.n)
     g = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
     g.load_scene(s)
+    if len(s["first_body"]) > 2:
+        g.set_worlds(s["first_body"])
     dt = s["dt"]
     n_dyn = len(s["shape"]) - int(s["n_static"])
     print(f"[bench] workload={args.workload} bodies={len(s['shape'])} steps={args.steps} "
@@ -153,7 +196,7 @@ def main():
     pairs = g.pairs()
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
 
-    step_b = step_bytes(s, c, unique_vertices=(args.workload == "hulls"))
+    step_b = step_bytes(s, c, unique_vertices=(args.workload in ("hulls", "batched")))
 
     # dominant kernel of the narrowphase and its roofline
     dom = max(("gjk", "epa"), key=lambda k: stage[k])
@@ -175,12 +218,16 @@ def main():
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": desc, "bodies": len(s["shape"]), "dynamic_bodies": n_dyn,
                    "pairs_per_step": c["pairs"], "gjk_hits_per_step": c["gjk_hits"],
+                   "last_step_counters": {k: c[k] for k in ("contacts", "rows", "colours", "epa_overflow",
+                                                            "candidates", "moved")},
                    "contact_pairs_per_s": round(pairs_total / wall, 1),
                    "colliding_pairs_per_s": round(hits_total / wall, 1),
                    "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stage.items()},
                    "step_algorithmic_bytes": step_b,
                    "step_hbm_frac": round(step_b / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 5),
-                   "parallelism": "1 GPU, one world"},
+                   "worlds": len(s["first_body"]) - 1,
+                   "parallelism": "1 GPU, " + ("one world" if len(s["first_body"]) == 2
+                                               else f"{len(s['first_body']) - 1} worlds batched")},
         "roofline": roof,
     }
     g.close()
@@ -196,7 +243,8 @@ def main():
         out["copy_check"] = {"bytes": nbytes, "ms": round(ms.value, 4),
                              "GB/s": round(2 * nbytes / ms.value / 1e6, 1)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s)
+        out["cpu_baseline"] = (cpu_baseline_batched(s, args.warmup) if len(s["first_body"]) > 2
+                               else cpu_baseline(s))
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,7 @@
 // pair count, GJK hit count, EPA overflow count, solver rows, colour rounds.
 #include "physkit_amd.h"
 #include "pk_access.hpp"
+#include "uv_sphere.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -92,7 +93,8 @@ struct pkg_ctx
     std::string err;
 
     uint32_t n_shapes = 0, n_verts = 0;
-    dbuf<uint32_t> shape_off;
+    bool has_uv = false; // some shape is a tagged UV sphere
+    dbuf<uint32_t> shape_off, shape_uv;
     dbuf<double> verts, shape_box;
 
     uint32_t n = 0;
@@ -176,6 +178,7 @@ struct pkg_ctx
         s.off = shape_off.p;
         s.verts = verts.p;
         s.local_box = shape_box.p;
+        s.uv = shape_uv.p;
         return s;
     }
     manifolds_dev MF(int k)
@@ -351,11 +354,11 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
     if (Pn)
     {
-        launch_gjk(keys[nxt].p, Pn, b, s, g, counters.p, st);
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, g, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);
-        launch_epa(keys[nxt].p, hits, b, s, g, ovf_list.p, counters.p, st);
+        launch_epa(keys[nxt].p, hits, b, s, has_uv, g, ovf_list.p, counters.p, st);
         uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
         if (novf) // rerun with the 4096-face store; capacity never changes a result
         {
@@ -551,6 +554,12 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
         }
     PKG_GUARD(ctx, {
         uint32_t nv = offsets[n_shapes];
+        std::vector<uint32_t> uv(std::max<uint32_t>(n_shapes, 1), 0u);
+        pk::uv_sphere_detector det;
+        for (uint32_t i = 0; i < n_shapes; ++i)
+            uv[i] = det.tag(verts + 3 * static_cast<size_t>(offsets[i]), offsets[i + 1] - offsets[i]);
+        upload(ctx->shape_uv, uv.data(), uv.size(), ctx->st);
+        ctx->has_uv = std::any_of(uv.begin(), uv.end(), [](uint32_t t) { return t != 0; });
         upload(ctx->shape_off, offsets, n_shapes + 1, ctx->st);
         upload(ctx->verts, verts, static_cast<size_t>(nv) * 3, ctx->st);
         ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
@@ -862,8 +871,20 @@ pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a, co
         hipStream_t st = ctx->st;
         dbuf<int32_t> ka, kb, dh;
         dbuf<double> pa, pb, dv, dout;
-        dbuf<uint32_t> ba, ea, bb, eb;
+        dbuf<uint32_t> ba, ea, bb, eb, ua, ub;
         dbuf<char> pool;
+        {
+            std::vector<uint32_t> tag_a(n, 0u), tag_b(n, 0u);
+            pk::uv_sphere_detector det;
+            for (uint32_t i = 0; i < n; ++i)
+            {
+                if (kind_a[i] == 2) tag_a[i] = det.tag(verts + 3 * static_cast<size_t>(vbeg_a[i]), vend_a[i] - vbeg_a[i]);
+                if (kind_b[i] == 2) tag_b[i] = det.tag(verts + 3 * static_cast<size_t>(vbeg_b[i]), vend_b[i] - vbeg_b[i]);
+            }
+            upload(ua, tag_a.data(), n, st);
+            upload(ub, tag_b.data(), n, st);
+            hip_check(hipStreamSynchronize(st), "tag upload");
+        }
         upload(ka, kind_a, n, st);
         upload(kb, kind_b, n, st);
         upload(pa, prm_a, 10 * static_cast<size_t>(n), st);
@@ -890,6 +911,8 @@ pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a, co
         sp.vbeg_b = bb.p;
         sp.vend_b = eb.p;
         sp.verts = dv.p;
+        sp.uv_a = ua.p;
+        sp.uv_b = ub.p;
         launch_gjk_epa_batch(sp, n, dout.p, dh.p, pool.p, st);
         hip_check(hipGetLastError(), "batch launch");
         std::vector<double> o = download(dout, 10 * static_cast<size_t>(n), st);

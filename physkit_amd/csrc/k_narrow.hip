@@ -18,7 +18,9 @@ constexpr int kBlock = 256;
 constexpr int kEpaBlock = 64;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
-__device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_dev &s, uint32_t id)
+// UV: the shapes may include tagged UV spheres (a kernel built without it carries no
+// code for their fast support)
+template <bool UV> __device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_dev &s, uint32_t id)
 {
     gshape g;
     g.kind = 2;
@@ -29,6 +31,7 @@ __device__ __forceinline__ gshape body_shape(const bodies_dev &b, const shapes_d
     uint32_t o = s.off[sh];
     g.v = s.verts + 3 * static_cast<size_t>(o);
     g.nv = s.off[sh + 1] - o;
+    g.uv = UV ? s.uv[sh] : 0u;
     return g;
 }
 
@@ -74,6 +77,7 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
 // the same shape (coalesced) and reduce the support search; the simplex logic runs
 // identically in every lane of the group, and lane 0 writes the results.
 constexpr int kGjkGroup = 8;
+template <bool UV>
 __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
     k_gjk(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, gjk_out_dev g,
           unsigned long long *counters)
@@ -87,9 +91,9 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
     if (p < n_pairs)
     {
         uint64_t key = keys[p];
-        gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
-        gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-        hit = gjk<kGjkGroup>(A, B, x, gl);
+        gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
+        gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        hit = gjk<kGjkGroup, UV>(A, B, x, gl);
         if (gl == 0) g.status[p] = 0;
     }
     bool lead = hit && gl == 0;
@@ -108,6 +112,7 @@ __device__ __forceinline__ void write_info(double *info, uint32_t cap, uint32_t 
 
 // one thread per hit, 192-face store in private memory; a solve that needs more is
 // listed for the 4096-face kernel below
+template <bool UV>
 __global__ void __launch_bounds__(kEpaBlock)
     k_epa(const uint64_t *keys, uint32_t n_hits, bodies_dev b, shapes_dev s, gjk_out_dev g,
           uint32_t *ovf_list, unsigned long long *counters)
@@ -118,13 +123,13 @@ __global__ void __launch_bounds__(kEpaBlock)
     {
         uint32_t p = g.hit_list[h];
         uint64_t key = keys[p];
-        gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
-        gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
+        gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
         simplex x;
         load_simplex(g.simplex, g.cap, h, x);
         epa_store_fast store;
         double out[10];
-        int r = epa_solve(A, B, x, store, out);
+        int r = epa_solve<UV>(A, B, x, store, out);
         if (r == 1) write_info(g.info, g.pair_cap, p, out);
         g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
         ovf = r == 3;
@@ -133,6 +138,7 @@ __global__ void __launch_bounds__(kEpaBlock)
     if (ovf) ovf_list[slot] = h;
 }
 
+template <bool UV>
 __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b,
                           shapes_dev s, gjk_out_dev g, epa_store_big *pool, unsigned long long *counters)
 {
@@ -141,12 +147,12 @@ __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32
     uint32_t h = ovf_list[t];
     uint32_t p = g.hit_list[h];
     uint64_t key = keys[p];
-    gshape A = body_shape(b, s, static_cast<uint32_t>(key >> 32));
-    gshape B = body_shape(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+    gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
+    gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
     simplex x;
     load_simplex(g.simplex, g.cap, h, x);
     double out[10];
-    int r = epa_solve(A, B, x, pool[t], out);
+    int r = epa_solve<UV>(A, B, x, pool[t], out);
     if (r == 1) write_info(g.info, g.pair_cap, p, out);
     g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
     if (r == 3) atomicAdd(&counters[C_EPA_BIG_FAIL], 1ull);
@@ -332,13 +338,14 @@ __global__ void __launch_bounds__(kBlock)
 
 // ---- standalone batch ----
 __device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint32_t *vb, const uint32_t *ve,
-                             const double *verts, uint32_t i)
+                             const double *verts, const uint32_t *uv, uint32_t i)
 {
     const double *q = prm + 10 * static_cast<size_t>(i);
     gshape g;
     g.kind = kind[i];
     g.v = nullptr;
     g.nv = 0;
+    g.uv = 0;
     g.b = mk(0.0, 0.0, 0.0);
     g.q = mkq(1.0, 0.0, 0.0, 0.0);
     if (g.kind == 0)
@@ -358,6 +365,7 @@ __device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint3
         g.q = mkq(q[3], q[4], q[5], q[6]);
         g.v = verts + 3 * static_cast<size_t>(vb[i]);
         g.nv = ve[i] - vb[i];
+        g.uv = uv[i];
     }
     return g;
 }
@@ -365,8 +373,8 @@ __device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint3
 template <class S>
 __device__ void batch_one(const shape_pair_dev &sp, uint32_t i, double *out, int32_t *hit, S &store)
 {
-    gshape A = pair_shape(sp.kind_a, sp.prm_a, sp.vbeg_a, sp.vend_a, sp.verts, i);
-    gshape B = pair_shape(sp.kind_b, sp.prm_b, sp.vbeg_b, sp.vend_b, sp.verts, i);
+    gshape A = pair_shape(sp.kind_a, sp.prm_a, sp.vbeg_a, sp.vend_a, sp.verts, sp.uv_a, i);
+    gshape B = pair_shape(sp.kind_b, sp.prm_b, sp.vbeg_b, sp.vend_b, sp.verts, sp.uv_b, i);
     simplex x;
     if (!gjk<1>(A, B, x, 0))
     {
@@ -401,16 +409,25 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
-void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs) k_gjk<<<grid(static_cast<size_t>(n_pairs) * kGjkGroup), kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    if (!n_pairs) return;
+    unsigned gr = grid(static_cast<size_t>(n_pairs) * kGjkGroup);
+    if (uv)
+        k_gjk<true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    else
+        k_gjk<false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
 }
 
-void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
+void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
 {
-    if (n_hits) k_epa<<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+    if (!n_hits) return;
+    if (uv)
+        k_epa<true><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+    else
+        k_epa<false><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
 }
 
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
@@ -418,8 +435,8 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                     unsigned long long *counters, hipStream_t st)
 {
     if (n_ovf)
-        k_epa_big<<<grid(n_ovf, 64), 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g,
-                                                  static_cast<epa_store_big *>(pool), counters);
+        k_epa_big<true><<<grid(n_ovf, 64), 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g,
+                                                        static_cast<epa_store_big *>(pool), counters);
 }
 
 void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,

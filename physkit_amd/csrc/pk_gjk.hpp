@@ -29,6 +29,7 @@ struct gshape
     qt q;
     const double *v;
     uint32_t nv;
+    uint32_t uv; // mesh built as a UV sphere: (stacks << 16) | sectors, else 0
 };
 
 __device__ __forceinline__ v3 ldv(const double *v, uint32_t i) { return mk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
@@ -41,11 +42,89 @@ constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
 // greater dot product, i.e. the lowest index among the maxima, except that a NaN dot
 // at vertex 0 keeps vertex 0 (no later value compares greater) and NaN dots elsewhere
 // are never taken.
+// Support of a UV-sphere mesh (tag s.uv, stacks and sectors at most 1024) from the
+// vertices around the analytic optimum, with a certificate. Direction d in the mesh
+// frame has polar angle phi* from +y and azimuth alpha; the candidates are both poles
+// and the sectors nearest alpha (3) on the rings nearest phi* (3), evaluated with the
+// same dot product as the full scan and ranked by (value, lowest index). phi* and
+// alpha are located in single precision from the normalised direction (error ~1e-7
+// rad, far below a tenth of a cell), so every other vertex lies at least 1.3 ring
+// steps (pi/stacks) from phi* or, on a candidate ring, 1.3 sector steps
+// (2 pi/sectors) from alpha, which bounds its dot product by
+//   r |d| cos(1.3 pi/stacks)                                   (other rings)
+//   r sin(phi_i) rho cos(1.3 2pi/sectors) + r cos(phi_i) d.y   (other sectors of ring i)
+// with rho = |(d.x, d.z)| (the true gap is 1.5 steps; the difference absorbs the
+// single-precision cosines). If the best candidate exceeds both bounds by a slack far
+// above every rounding error of the dot products and the vertex coordinates, no other
+// vertex can equal or beat it, so the result is the full scan's; otherwise
+// (degenerate directions) the caller scans every vertex.
+__device__ __forceinline__ bool uv_support(const gshape &s, v3 ld, uint32_t &out)
+{
+    const int S = static_cast<int>(s.uv >> 16), T = static_cast<int>(s.uv & 0xFFFFu);
+    if (!(isfinite(ld.x) && isfinite(ld.y) && isfinite(ld.z))) return false;
+    const double rho = sqrt(ld.x * ld.x + ld.z * ld.z);
+    const double dn = sqrt(rho * rho + ld.y * ld.y);
+    if (!(dn > 0.0)) return false;
+    const float pif = 3.14159265f;
+    const float phi = atan2f(static_cast<float>(rho / dn), static_cast<float>(ld.y / dn));
+    int i0 = static_cast<int>(rintf(phi * (static_cast<float>(S) / pif)));
+    i0 = i0 < 1 ? 1 : (i0 > S - 1 ? S - 1 : i0);
+    int j0 = 0;
+    if (rho > 0.0)
+    {
+        const float al = atan2f(static_cast<float>(ld.z / rho), static_cast<float>(ld.x / rho));
+        j0 = static_cast<int>(rintf(al * (static_cast<float>(T) / (2.0f * pif))));
+        j0 = ((j0 % T) + T) % T;
+    }
+    const int ilo = i0 > 1 ? i0 - 1 : 1, ihi = i0 < S - 1 ? i0 + 1 : S - 1;
+    double bv = dot(ldv(s.v, 0), ld);
+    uint32_t bi = 0;
+    for (int i = ilo; i <= ihi; ++i)
+        for (int dj = -1; dj <= 1; ++dj)
+        {
+            uint32_t k = 1u + static_cast<uint32_t>((i - 1) * T + (j0 + dj + T) % T);
+            double t = dot(ldv(s.v, k), ld);
+            if (t > bv || (t == bv && k < bi))
+            {
+                bv = t;
+                bi = k;
+            }
+        }
+    {
+        uint32_t k = s.nv - 1;
+        double t = dot(ldv(s.v, k), ld);
+        if (t > bv) // the largest index: it wins no tie
+        {
+            bv = t;
+            bi = k;
+        }
+    }
+    const double r = s.v[1];
+    const double slack = 1e-12 * r * (fabs(ld.x) + fabs(ld.y) + fabs(ld.z));
+    double ub = -__builtin_inf();
+    if (ilo > 1 || ihi < S - 1) ub = r * dn * static_cast<double>(cosf(1.3f * pif / static_cast<float>(S)));
+    if (T > 3)
+    {
+        const double cs = static_cast<double>(cosf(1.3f * 2.0f * pif / static_cast<float>(T)));
+        for (int i = ilo; i <= ihi; ++i)
+        {
+            const double *p = s.v + 3 * (1 + static_cast<size_t>(i - 1) * T); // sector 0: (r sin phi, r cos phi, 0)
+            double u = p[0] * rho * cs + p[1] * ld.y;
+            ub = u > ub ? u : ub;
+        }
+    }
+    if (!(bv > ub + slack)) return false;
+    out = bi;
+    return true;
+}
+
 template <int G> __device__ __forceinline__ uint32_t support_idx(const gshape &s, v3 d, int gl)
 {
     if (s.kind == 2) // mesh::instance::support, mesh.cpp:442-448
     {
         v3 ld = rotate(conj(s.q), d);
+        uint32_t k;
+        if (s.uv && uv_support(s, ld, k)) return k;
         double bv = 0.0;
         uint32_t bi = kNoIdx;
         bool nan0 = false;
@@ -95,12 +174,46 @@ __device__ __forceinline__ v3 support_at(const gshape &s, uint32_t f)
     return mk((f & 1u) ? s.b.x : s.a.x, (f & 2u) ? s.b.y : s.a.y, (f & 4u) ? s.b.z : s.a.z);
 }
 
-template <int G>
-__device__ __forceinline__ spt msupport(const gshape &A, const gshape &B, v3 d, int gl) // collision.h:41-49
+__device__ __forceinline__ v3 sel3(bool c, v3 a, v3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+__device__ __forceinline__ gshape sel_shape(bool c, const gshape &a, const gshape &b)
+{
+    gshape r;
+    r.kind = c ? a.kind : b.kind;
+    r.a = sel3(c, a.a, b.a);
+    r.b = sel3(c, a.b, b.b);
+    r.q = mkq(c ? a.q.w : b.q.w, c ? a.q.x : b.q.x, c ? a.q.y : b.q.y, c ? a.q.z : b.q.z);
+    r.v = c ? a.v : b.v;
+    r.nv = c ? a.nv : b.nv;
+    r.uv = c ? a.uv : b.uv;
+    return r;
+}
+
+// s_A(d) - s_B(-d), collision.h:41-49. With UV-sphere shapes possible (UV) the two
+// support searches share one loop body (kept rolled) so the larger search code is
+// emitted once per call site; without, both are inlined directly.
+template <int G, bool UV = true>
+__device__ __forceinline__ spt msupport(const gshape &A, const gshape &B, v3 d, int gl)
 {
     spt r;
-    r.ia = support_idx<G>(A, d, gl);
-    r.ib = support_idx<G>(B, -d, gl);
+    if (!UV)
+    {
+        r.ia = support_idx<G>(A, d, gl);
+        r.ib = support_idx<G>(B, -d, gl);
+        r.p = support_at(A, r.ia) - support_at(B, r.ib);
+        return r;
+    }
+    r.ia = 0;
+    r.ib = 0;
+#pragma nounroll
+    for (int side = 0; side < 2; ++side)
+    {
+        gshape sh = sel_shape(side == 0, A, B);
+        uint32_t f = support_idx<G>(sh, side == 0 ? d : -d, gl);
+        if (side == 0)
+            r.ia = f;
+        else
+            r.ib = f;
+    }
     r.p = support_at(A, r.ia) - support_at(B, r.ib);
     return r;
 }
@@ -113,7 +226,6 @@ struct simplex
 // Written as value selects: a branch per slot lets the optimiser merge the four
 // stores into one store through a selected pointer, which pins the simplex in
 // private memory (scratch) instead of registers.
-__device__ __forceinline__ v3 sel3(bool c, v3 a, v3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
 __device__ __forceinline__ void sx_put(spt &dst, bool c, const spt &p)
 {
     dst.p = sel3(c, p.p, dst.p);
@@ -238,18 +350,32 @@ __device__ __forceinline__ bool handle_tetrahedron(simplex &x, v3 &dir) // colli
 // (inlined so the simplex stays in registers). With G > 1 the G lanes of a group run
 // it together: support searches are split over the lanes, everything else is
 // computed identically in every lane of the group.
-template <int G> __device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x, int gl)
+template <int G, bool UV = true> __device__ __forceinline__ bool gjk(const gshape &A, const gshape &B, simplex &x, int gl)
 {
     const double eps = 1e-12;
     x.n = 0;
     v3 dir = mk(1.0, 0.0, 0.0);
-    spt p = msupport<G>(A, B, dir, gl);
-    sx_push(x, p);
-    if (sqn(p.p) < eps) return true;
-    dir = -normalized(p.p);
-    for (int it = 0; it < 100; ++it)
+    // pass -1 is the initial support (with UV spheres possible, one call site keeps the
+    // larger search code single; otherwise the first search is peeled off the loop)
+    int first = -1;
+    if (!UV)
     {
-        spt np = msupport<G>(A, B, dir, gl);
+        spt p = msupport<G, UV>(A, B, dir, gl);
+        sx_push(x, p);
+        if (sqn(p.p) < eps) return true;
+        dir = -normalized(p.p);
+        first = 0;
+    }
+    for (int it = first; it < 100; ++it)
+    {
+        spt np = msupport<G, UV>(A, B, dir, gl);
+        if (it < 0)
+        {
+            sx_push(x, np);
+            if (sqn(np.p) < eps) return true;
+            dir = -normalized(np.p);
+            continue;
+        }
         if (dot(np.p, dir) <= 0.0) return false;
         sx_push(x, np);
         if (x.n == 2)
@@ -263,38 +389,45 @@ template <int G> __device__ __forceinline__ bool gjk(const gshape &A, const gsha
 }
 
 // collision.cpp:191-248
-template <int G>
+template <int G, bool UV = true>
 __device__ __forceinline__ bool pad_simplex(const gshape &A, const gshape &B, simplex &x, int gl)
 {
-    int start = x.n;
-    if (start <= 1)
-    {
-        v3 dir = mk(1.0, 0.0, 0.0);
-        spt p2 = msupport<G>(A, B, dir, gl);
-        if (sqn(p2.p - x.s0.p) < 1e-6) p2 = msupport<G>(A, B, -dir, gl);
-        x.s1 = p2;
-        x.n = 2;
-    }
-    if (start <= 2)
+    // stage k fills slot k (1: s1 from direction +x, 2: s2 off the line s0-s1, 3: s3
+    // off the plane), each retried once in the opposite direction; written as one
+    // loop so the support search is emitted once
+    for (int k = x.n < 1 ? 1 : x.n; k <= 3; ++k)
     {
         v3 line = x.s1.p - x.s0.p;
-        v3 dir = cross(normalized(line), mk(0.0, 1.0, 0.0));
-        if (sqn(dir) < 1e-6) dir = cross(normalized(line), mk(0.0, 0.0, 1.0));
-        dir = normalized(dir);
-        spt p3 = msupport<G>(A, B, dir, gl);
-        if (sqn(cross(line, p3.p - x.s0.p)) < 1e-6) p3 = msupport<G>(A, B, -dir, gl);
-        x.s2 = p3;
-        x.n = 3;
-    }
-    if (start <= 3)
-    {
-        v3 ab = x.s1.p - x.s0.p;
-        v3 ac = x.s2.p - x.s0.p;
-        v3 dir = normalized(cross(ab, ac));
-        spt p4 = msupport<G>(A, B, dir, gl);
-        if (fabs(dot(p4.p - x.s0.p, dir)) < 1e-6) p4 = msupport<G>(A, B, -dir, gl);
-        x.s3 = p4;
-        x.n = 4;
+        v3 dir;
+        if (k == 1)
+            dir = mk(1.0, 0.0, 0.0);
+        else if (k == 2)
+        {
+            dir = cross(normalized(line), mk(0.0, 1.0, 0.0));
+            if (sqn(dir) < 1e-6) dir = cross(normalized(line), mk(0.0, 0.0, 1.0));
+            dir = normalized(dir);
+        }
+        else
+            dir = normalized(cross(line, x.s2.p - x.s0.p));
+        spt p;
+#pragma nounroll
+        for (int att = 0; att < 2; ++att)
+        {
+            p = msupport<G, UV>(A, B, att == 0 ? dir : -dir, gl);
+            if (att == 1) break;
+            bool again;
+            if (k == 1)
+                again = sqn(p.p - x.s0.p) < 1e-6;
+            else if (k == 2)
+                again = sqn(cross(line, p.p - x.s0.p)) < 1e-6;
+            else
+                again = fabs(dot(p.p - x.s0.p, dir)) < 1e-6;
+            if (!again) break;
+        }
+        sx_put(x.s1, k == 1, p);
+        sx_put(x.s2, k == 2, p);
+        sx_put(x.s3, k == 3, p);
+        x.n = k + 1;
     }
     v3 ad = x.s0.p - x.s3.p;
     v3 bd = x.s1.p - x.s3.p;
@@ -539,9 +672,10 @@ template <class S> struct epa_solver
 // epa_solver::solve (collision.cpp:411-504) from a GJK simplex.
 // returns 0 no contact (degenerate simplex or empty heap), 1 contact written to out,
 // 3 the store's capacity was exceeded (rerun with a larger store)
-template <class S> __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store, double *out)
+template <bool UV = true, class S>
+__device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store, double *out)
 {
-    if (x.n < 4 && !pad_simplex<1>(A, B, x, 0)) return 0;
+    if (x.n < 4 && !pad_simplex<1, UV>(A, B, x, 0)) return 0;
     epa_solver<S> e(store);
     e.put_point(0, x.s0);
     e.put_point(1, x.s1);
@@ -556,7 +690,7 @@ template <class S> __device__ int epa_solve(const gshape &A, const gshape &B, si
         if (mf < 0) break;
         v3 nrm = e.N(mf);
         double fd = store.d[mf];
-        spt p = msupport<1>(A, B, nrm, 0);
+        spt p = msupport<1, UV>(A, B, nrm, 0);
         double pd = dot(nrm, p.p);
         if (pd - fd < tol)
         {

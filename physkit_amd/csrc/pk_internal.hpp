@@ -44,6 +44,7 @@ struct shapes_dev
     const uint32_t *off; // n+1 vertex offsets
     const double *verts; // xyz interleaved (a support scan reads one shape's vertices in order)
     double *local_box;   // 6 planes over shapes
+    const uint32_t *uv;  // n: UV-sphere tag ((stacks << 16) | sectors) or 0
 };
 
 // manifold cache: up to 4 contacts x 13 fields per pair, planar with stride `cap`
@@ -163,9 +164,9 @@ struct gjk_out_dev
     uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
     uint32_t pair_cap;
 };
-void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s,
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
-void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s,
+void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 size_t epa_big_bytes();
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
@@ -183,6 +184,7 @@ struct shape_pair_dev
     const double *prm_a, *prm_b; // 10 doubles per shape
     const uint32_t *vbeg_a, *vend_a, *vbeg_b, *vend_b;
     const double *verts;
+    const uint32_t *uv_a, *uv_b; // UV-sphere tags of mesh shapes (0: none)
 };
 void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,
                           void *big_pool, hipStream_t st);
