@@ -140,7 +140,10 @@ struct pkg_ctx
     // solver
     dbuf<double> rows;
     dbuf<uint8_t> row_valid;
-    dbuf<uint32_t> colour, colour_count, colour_start, cursor, order;
+    dbuf<uint32_t> colour, colour_start, order, order_alt;
+    dbuf<uint64_t> ckeys, ckeys_alt;
+    dbuf<double> rows_s;      // solver rows in solve order
+    dbuf<uint8_t> row_valid_s;
     dbuf<unsigned long long> claim;
     uint32_t n_colours = 0;
 
@@ -415,20 +418,32 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
             }
             n_colours = c;
-            colour_count.reserve(c);
             colour_start.reserve(c + 1);
-            cursor.reserve(c);
             order.reserve(Pn);
-            launch_colour_bucket(colour.p, Pn, c, colour_count.p, colour_start.p, cursor.p, order.p, st);
+            order_alt.reserve(Pn);
+            ckeys.reserve(Pn);
+            ckeys_alt.reserve(Pn);
+            uint32_t *ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
+                                                 colour_start.p, sc, st);
             std::vector<uint32_t> cs(c + 1);
             hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
                       "colour copy");
             hip_check(hipStreamSynchronize(st), "stream sync");
-            for (uint32_t k = 0; k < c; ++k) launch_solve_colour(keys[nxt].p, order.p, cs[k], cs[k + 1], b, r, true, st);
+            // rows of the solver manifolds gathered into solve order
+            const uint32_t n_act = cs[c];
+            const uint32_t scap = std::max<uint32_t>(n_act, 1);
+            rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
+            row_valid_s.reserve(static_cast<size_t>(4) * scap);
+            rows_dev rs;
+            rs.cap4 = 4 * scap;
+            rs.d = rows_s.p;
+            rs.valid = row_valid_s.p;
+            launch_rows_gather(ord, n_act, r, rs, scap, st);
+            for (uint32_t k = 0; k < c; ++k) launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
             for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
                 for (uint32_t k = 0; k < c; ++k)
-                    launch_solve_colour(keys[nxt].p, order.p, cs[k], cs[k + 1], b, r, false, st);
-            launch_impulse_writeback(Pn, MF(nxt), r, st);
+                    launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
+            launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
         }
     }
     record(ev, 7, st);

@@ -296,34 +296,51 @@ __global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, u
     wave_append(&counters[C_REMAIN], remain);
 }
 
-__global__ void k_colour_hist(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint32_t *count)
+// colour buckets by a stable radix sort of (colour, manifold index); manifolds
+// without rows go to bucket n_colours, after every colour
+__global__ void k_colour_keys(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
+                              uint32_t *vals)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     uint32_t c = colour[p];
-    if (c < n_colours) atomicAdd(&count[c], 1u);
+    keys[p] = c < n_colours ? c : n_colours;
+    vals[p] = p;
 }
 
-__global__ void k_colour_scan(const uint32_t *count, uint32_t n_colours, uint32_t *start, uint32_t *cursor)
+__global__ void k_colour_bounds(const uint64_t *sorted, uint32_t n, uint32_t n_colours, uint32_t *start)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t s = 0;
-    for (uint32_t c = 0; c < n_colours; ++c)
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > n_colours) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi)
     {
-        start[c] = s;
-        cursor[c] = s;
-        s += count[c];
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (sorted[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
-    start[n_colours] = s;
+    start[c] = lo;
 }
 
-__global__ void k_colour_scatter(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint32_t *cursor,
-                                 uint32_t *order)
+// the rows of the solver manifolds gathered into solve order: position t of `order`
+// (manifold order[t]) keeps contact c at slot c * cap + t of every row plane, so the
+// lanes solving one colour read each row field with consecutive addresses
+__global__ void k_rows_gather(const uint32_t *order, uint32_t n_act, rows_dev src, rows_dev dst, uint32_t cap)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    uint32_t c = colour[p];
-    if (c < n_colours) order[atomicAdd(&cursor[c], 1u)] = p;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_act) return;
+    uint32_t p = order[t];
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t s = 4 * p + static_cast<uint32_t>(c), d = static_cast<uint32_t>(c) * cap + t;
+        uint8_t v = src.valid[s];
+        dst.valid[d] = v;
+        if (!v) continue;
+        for (int f = 0; f < kRowF; ++f)
+            dst.d[static_cast<size_t>(f) * dst.cap4 + d] = src.d[static_cast<size_t>(f) * src.cap4 + s];
+    }
 }
 
 __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, int f_bias, int f_acc, uint32_t slot)
@@ -338,9 +355,10 @@ __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, 
     return v;
 }
 
+// one colour: positions [begin, end) of the solve order, rows gathered by k_rows_gather
 __global__ void __launch_bounds__(kBlock)
     k_solve(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows_dev r,
-            int warm)
+            uint32_t cap, int warm)
 {
     uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= end) return;
@@ -350,7 +368,7 @@ __global__ void __launch_bounds__(kBlock)
     body_state A = ld_body(b, ia), B = ld_body(b, ib);
     for (int c = 0; c < 4; ++c)
     {
-        uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+        uint32_t slot = static_cast<uint32_t>(c) * cap + t;
         if (!r.valid[slot]) continue;
         rowv nr = ld_row(r, F_NJV, F_NMEFF, F_NBIAS, F_NACC, slot);
         rowv t1 = ld_row(r, F_T1JV, F_T1MEFF, -1, F_T1ACC, slot);
@@ -383,13 +401,14 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-__global__ void k_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r) // :1191-1196
+__global__ void k_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap) // :1191-1196
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_act) return;
+    uint32_t p = order[t];
     for (int c = 0; c < 4; ++c)
     {
-        uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+        uint32_t slot = static_cast<uint32_t>(c) * cap + t;
         if (!r.valid[slot]) continue;
         double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
         size_t s = m.cap;
@@ -426,24 +445,36 @@ void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_d
         k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim, c,
                                                    counters);
 }
-void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                          uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,
-                          uint32_t *order, hipStream_t st)
+uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
+                               uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,
+                               sort_scratch &sc, hipStream_t st)
 {
-    if (!n_pairs || !n_colours) return;
-    launch_fill_u32(colour_count, 0u, n_colours, st);
-    k_colour_hist<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, colour_count);
-    k_colour_scan<<<1, 64, 0, st>>>(colour_count, n_colours, colour_start, cursor);
-    k_colour_scatter<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, cursor, order);
+    if (!n_pairs || !n_colours) return order;
+    k_colour_keys<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, keys, order);
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= n_colours) ++bits; // keys 0..n_colours
+    int shifts[4];
+    int ns = 0;
+    for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
+    bool alt = radix_sort_u64(keys, order, keys_alt, order_alt, n_pairs, shifts, ns, sc, st);
+    const uint64_t *sk = alt ? keys_alt : keys;
+    k_colour_bounds<<<grid(n_colours + 1), kBlock, 0, st>>>(sk, n_pairs, n_colours, colour_start);
+    return alt ? order_alt : order;
+}
+void launch_rows_gather(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst, uint32_t cap,
+                        hipStream_t st)
+{
+    if (n_act) k_rows_gather<<<grid(n_act), kBlock, 0, st>>>(order, n_act, src, dst, cap);
 }
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
-                         const bodies_dev &b, rows_dev r, bool warm, hipStream_t st)
+                         const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st)
 {
-    if (end > begin) k_solve<<<grid(end - begin), kBlock, 0, st>>>(keys, order, begin, end, b, r, warm ? 1 : 0);
+    if (end > begin) k_solve<<<grid(end - begin), kBlock, 0, st>>>(keys, order, begin, end, b, r, cap, warm ? 1 : 0);
 }
-void launch_impulse_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r, hipStream_t st)
+void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,
+                              hipStream_t st)
 {
-    if (n_pairs) k_writeback<<<grid(n_pairs), kBlock, 0, st>>>(n_pairs, m, r);
+    if (n_act) k_writeback<<<grid(n_act), kBlock, 0, st>>>(order, n_act, m, r, cap);
 }
 
 } // namespace pk

@@ -212,11 +212,16 @@ void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_d
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, uint32_t c, unsigned long long *counters,
                           hipStream_t st);
-void launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                          uint32_t *colour_count, uint32_t *colour_start, uint32_t *cursor,
-                          uint32_t *order, hipStream_t st);
+// stable bucket of the solver manifolds by colour: returns the order array (one of
+// the two given) and colour_start[0..n_colours] (manifolds without rows come last)
+uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
+                               uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,
+                               sort_scratch &sc, hipStream_t st);
+void launch_rows_gather(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst, uint32_t cap,
+                        hipStream_t st);
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
-                         const bodies_dev &b, rows_dev r, bool warm, hipStream_t st);
-void launch_impulse_writeback(uint32_t n_pairs, manifolds_dev m, rows_dev r, hipStream_t st);
+                         const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st);
+void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,
+                              hipStream_t st);
 
 } // namespace pk
