@@ -33,6 +33,12 @@ WORKLOADS = {
     "boxes": (0, "1,024 random boxes over a static ground box, dt=1/60, 10 PGS iterations, FP64"),
     "pyramids": (2, "64 pyramids of 15-layer unit-box stacks (7,680 boxes), dt=1/120, "
                     "20 PGS iterations, FP64"),
+    "granular": (4, "granular pile: 2,097,152 bodies (70% UV spheres 16x32 r U[0.01,0.03], 30% boxes "
+                     "half extents U[0.01,0.03]) on a jittered 0.07 m lattice falling into a 64-box "
+                     "hopper, dt=1/240, 10 PGS iterations, FP64"),
+    "granular10": (6, "granular pile at ten times the lengths: 2,097,152 bodies (70% UV spheres 16x32 "
+                       "r U[0.1,0.3], 30% boxes half extents U[0.1,0.3]) on a jittered 0.7 m lattice "
+                       "falling into a 64-box hopper, dt=1/240, 10 PGS iterations, FP64"),
     "batched": (3, "4,096 independent worlds of 16-4,096 bodies (log-uniform; 40% UV spheres "
                    "16x32, 40% boxes, 20% 8-16-point hulls) over static ground boxes, dt=1/60, "
                    "10 PGS iterations, one batched step per launch sequence, FP64"),
@@ -90,6 +96,29 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
+def per_world_dropin(s, warmup, steps, n_sample=64):
+    """The plain FP64 drop-in on the batched workload: each of the first n_sample worlds
+    stepped alone through its own context (one world per physkit::world), warmup steps
+    untimed, then `steps` timed steps; body-steps/s over the summed wall times."""
+    import physkit_amd as pk
+
+    bodies, wall = 0, 0.0
+    n = min(n_sample, len(s["first_body"]) - 1)
+    for w in range(n):
+        ws, b0, b1 = pk.scene_world(s, w)
+        g = pk.GpuWorld(iterations=ws["iterations"], flags=ws["flags"])
+        g.load_scene(ws)
+        g.step(ws["dt"], warmup)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.step(ws["dt"])
+        wall += time.perf_counter() - t0
+        bodies += (b1 - b0 - ws["n_static"]) * steps
+        g.close()
+    return {"value": round(bodies / wall, 1), "unit": "body-steps/s",
+            "sample": f"worlds 0-{n - 1} stepped one at a time, {steps} timed steps each after {warmup} untimed"}
+
+
 def cpu_baseline_batched(s, warmup, budget_s=12.0):
     """The oracle on the batched workload, its independent worlds stepped by a pool of
     host threads (the oracle releases the interpreter lock inside a step). Worlds are
@@ -99,7 +128,7 @@ def cpu_baseline_batched(s, warmup, budget_s=12.0):
     stepping."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi
-    import batched_util
+    import physkit_amd as pk
     from concurrent.futures import ThreadPoolExecutor
 
     cores = max(1, min(16, os.cpu_count() or 1))
@@ -107,7 +136,7 @@ def cpu_baseline_batched(s, warmup, budget_s=12.0):
     n_worlds = len(fb) - 1
 
     def setup(w):
-        ws, b0, b1 = batched_util.world_slice(s, w)
+        ws, b0, b1 = pk.scene_world(s, w)
         o = oracle_ffi.OracleWorld(iterations=ws["iterations"], solver_mode=1, flags=ws["flags"])
         o.load_scene(ws)
         o.step(ws["dt"], warmup)
@@ -231,6 +260,9 @@ def main():
         "roofline": roof,
     }
     g.close()
+    if len(s["first_body"]) > 2:
+        out["config"]["per_world_dropin"] = per_world_dropin(s, args.warmup, args.steps)
+        out["config"]["batched_over_per_world"] = round(out["value"] / out["config"]["per_world_dropin"]["value"], 2)
     if args.copy_check:
         import ctypes
         from physkit_amd import build as pk_build

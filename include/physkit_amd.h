@@ -199,9 +199,11 @@ typedef struct pkg_scene_info
 
 /* kind 0: 1,024 random boxes over a ground box; kind 1: convex-hull clusters
  * (narrowphase stress); kind 2: box pyramids (solver stress); kind 3: 4,096
- * independent worlds of 16-4,096 bodies (spheres, boxes, hulls), for pkg_set_worlds.
- * n_override != 0 changes the body count (kind 0, 1), pyramid count (kind 2) or
- * world count (kind 3). */
+ * independent worlds of 16-4,096 bodies (spheres, boxes, hulls), for pkg_set_worlds;
+ * kind 4: granular pile of 2,097,152 small spheres (16x32 UV meshes) and boxes falling
+ * into a 64-box hopper; kind 5: the same with 8x12 spheres; kind 6: kind 4 with every
+ * length times 10. n_override != 0 changes the body count (kind 0, 1, 4-6), pyramid
+ * count (kind 2) or world count (kind 3). */
 pkg_status pkg_scene_create(pkg_scene **out, int kind, uint64_t seed, uint32_t n_override);
 pkg_status pkg_scene_info_get(const pkg_scene *s, pkg_scene_info *info);
 /* Copies the scene into caller buffers sized from pkg_scene_info. */

@@ -119,7 +119,7 @@ def _f64(a):
 
 
 # ------------------------------------------------------------------ scenes
-SCENE_BOXES, SCENE_HULLS, SCENE_PYRAMIDS, SCENE_BATCHED = 0, 1, 2, 3
+SCENE_BOXES, SCENE_HULLS, SCENE_PYRAMIDS, SCENE_BATCHED, SCENE_GRANULAR, SCENE_GRANULAR_LOW = 0, 1, 2, 3, 4, 5
 SEED_BASE = 20261202
 
 
@@ -154,6 +154,27 @@ def scene(kind: int, seed: int | None = None, n_override: int = 0) -> dict:
         return s
     finally:
         L.pkg_scene_destroy(h)
+
+
+def scene_world(s: dict, w: int):
+    """World w of a batched scene as a scene of its own (bodies [first_body[w],
+    first_body[w+1]) renumbered from 0, with its contiguous shapes), and the body range
+    it came from."""
+    fb = s["first_body"]
+    b0, b1 = int(fb[w]), int(fb[w + 1])
+    shp = s["shape"][b0:b1].astype(np.int64)
+    s0, s1 = int(shp.min()), int(shp.max()) + 1
+    offs = s["shape_offsets"].astype(np.int64)
+    out = {k: s[k][b0:b1].copy() for k in ("is_static", "pos", "vel", "angvel", "quat", "mass", "inertia",
+                                            "restitution", "friction")}
+    out["shape"] = (shp - s0).astype(np.uint32)
+    out["shape_offsets"] = (offs[s0:s1 + 1] - offs[s0]).astype(np.uint32)
+    out["verts"] = s["verts"][offs[s0]:offs[s1]].copy()
+    for k in ("dt", "iterations", "steps", "flags"):
+        out[k] = s[k]
+    out["n_static"] = int(np.sum(out["is_static"] != 0))
+    out["first_body"] = np.array([0, b1 - b0], np.uint32)
+    return out, b0, b1
 
 
 # ------------------------------------------------------------------ bulk world (SoA)

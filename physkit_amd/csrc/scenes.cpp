@@ -312,6 +312,98 @@ void scene_batched(pkg_scene &s, uint64_t seed, uint32_t n_worlds)
     s.info.steps = 120;
     s.info.flags = 0;
 }
+// Static box at p with half extents h, rotated by q (w, x, y, z)
+void add_static_box(pkg_scene &s, double hx, double hy, double hz, const double *p, const double *q)
+{
+    double I[9];
+    diag(I, 1, 1, 1);
+    uint32_t sh = add_box_shape(s, hx, hy, hz);
+    add_body(s, true, sh, p, kZero3, q, 0.0, I, 0.5, 0.5);
+}
+
+// Granular pile: n bodies (70% UV spheres r U[0.01,0.03] with the given stacks and
+// sectors, 30% boxes with half extents U[0.01,0.03]) on a jittered cubic lattice
+// (spacing 0.07 m, jitter U[-0.004,0.004] per axis, uniform rotations, at rest),
+// dropped into a hopper of 64 static boxes: a 2x2 floor of 5x5 m plates at y = 0
+// and four walls sloping from half-width 1.2 m at y = 1 to 7 m at y = 9, each wall
+// 15 plates (5 along the slope x 3 across). Density 1000, e = mu = 0.5. Every length
+// is multiplied by L (L = 10: bodies of 10-30 cm).
+void scene_granular(pkg_scene &s, uint64_t seed, uint32_t n, unsigned stacks, unsigned sectors, double L = 1.0)
+{
+    rng r(seed);
+    double I[9];
+    // floor
+    for (int k = 0; k < 4; ++k)
+    {
+        double p[3] = {L * ((k & 1) ? 2.5 : -2.5), -0.05 * L, L * ((k & 2) ? 2.5 : -2.5)};
+        add_static_box(s, 2.5 * L, 0.05 * L, 2.5 * L, p, kIdentityQ);
+    }
+    // four sloped walls: wall w faces the centre; a plate's local x runs across the
+    // wall, local z down the slope, local y is the wall normal
+    const double y0 = 1.0 * L, y1 = 9.0 * L, h0 = 1.2 * L, h1 = 7.0 * L;
+    const double run = h1 - h0, rise = y1 - y0;
+    const double len = std::sqrt(run * run + rise * rise);
+    const double tilt = std::atan2(rise, run); // wall angle above the horizontal
+    for (int w = 0; w < 4; ++w)
+    {
+        double yaw = 0.5 * kPi * w; // wall w sits on the +x side rotated by yaw about y
+        for (int a = 0; a < 5; ++a)     // along the slope
+            for (int b = 0; b < 3; ++b) // across
+            {
+                double u = (a + 0.5) / 5.0;
+                double rad = h0 + u * run, y = y0 + u * rise;
+                double across = (b - 1) * (2.0 * rad / 3.0);
+                double lp[3] = {rad + 0.05 * L, y, across}; // in the wall frame (before yaw)
+                double c = std::cos(yaw), sn = std::sin(yaw);
+                double p[3] = {c * lp[0] + sn * lp[2], lp[1], -sn * lp[0] + c * lp[2]};
+                // orientation: about z by tilt - pi/2, taking the plate's long axis
+                // (local y) onto the slope (cos tilt, sin tilt, 0); then the yaw
+                double qt[4] = {std::cos(0.5 * (tilt - 0.5 * kPi)), 0.0, 0.0, std::sin(0.5 * (tilt - 0.5 * kPi))};
+                double qy[4] = {std::cos(0.5 * yaw), 0.0, std::sin(0.5 * yaw), 0.0};
+                double q[4] = {qy[0] * qt[0] - qy[1] * qt[1] - qy[2] * qt[2] - qy[3] * qt[3],
+                               qy[0] * qt[1] + qy[1] * qt[0] + qy[2] * qt[3] - qy[3] * qt[2],
+                               qy[0] * qt[2] - qy[1] * qt[3] + qy[2] * qt[0] + qy[3] * qt[1],
+                               qy[0] * qt[3] + qy[1] * qt[2] - qy[2] * qt[1] + qy[3] * qt[0]};
+                add_static_box(s, 0.05 * L, len / 10.0, rad / 3.0 + 0.05 * L, p, q);
+            }
+    }
+    // lattice of bodies above the hopper opening
+    uint32_t side = 1;
+    while (static_cast<uint64_t>(side) * side * side < n) ++side;
+    const double sp = 0.07 * L, jit = 0.004 * L;
+    const double base = -0.5 * sp * (side - 1);
+    uint32_t placed = 0;
+    for (uint32_t iy = 0; iy < side && placed < n; ++iy)
+        for (uint32_t iz = 0; iz < side && placed < n; ++iz)
+            for (uint32_t ix = 0; ix < side && placed < n; ++ix, ++placed)
+            {
+                double p[3] = {base + sp * ix + r.uni(-jit, jit), 10.0 * L + sp * iy + r.uni(-jit, jit),
+                               base + sp * iz + r.uni(-jit, jit)};
+                double q[4];
+                r.quat(q);
+                uint32_t sh;
+                double m;
+                if (r.u01() < 0.7)
+                {
+                    double rad = L * r.uni(0.01, 0.03);
+                    sh = add_sphere_shape(s, rad, stacks, sectors);
+                    m = 1000.0 * (4.0 / 3.0) * kPi * rad * rad * rad;
+                    diag(I, 0.4 * m * rad * rad, 0.4 * m * rad * rad, 0.4 * m * rad * rad);
+                }
+                else
+                {
+                    double hx = L * r.uni(0.01, 0.03), hy = L * r.uni(0.01, 0.03), hz = L * r.uni(0.01, 0.03);
+                    sh = add_box_shape(s, hx, hy, hz);
+                    m = 1000.0 * 8.0 * hx * hy * hz;
+                    diag(I, m / 3.0 * (hy * hy + hz * hz), m / 3.0 * (hx * hx + hz * hz), m / 3.0 * (hx * hx + hy * hy));
+                }
+                add_body(s, false, sh, p, kZero3, q, m, I, 0.5, 0.5);
+            }
+    s.info.dt = 1.0 / 240.0;
+    s.info.iterations = 10;
+    s.info.steps = 240;
+    s.info.flags = 0;
+}
 } // namespace
 
 extern "C"
@@ -327,6 +419,9 @@ pkg_status pkg_scene_create(pkg_scene **out, int kind, uint64_t seed, uint32_t n
     case 1: scene_hulls(*s, seed, n_override ? n_override : 262144); break;
     case 2: scene_pyramids(*s, seed, n_override ? n_override : 64); break;
     case 3: scene_batched(*s, seed, n_override ? n_override : 4096); break;
+    case 4: scene_granular(*s, seed, n_override ? n_override : 2097152, 16, 32); break;
+    case 5: scene_granular(*s, seed, n_override ? n_override : 2097152, 8, 12); break;
+    case 6: scene_granular(*s, seed, n_override ? n_override : 2097152, 16, 32, 10.0); break;
     default: delete s; return PKG_INVALID_ARG;
     }
     if (kind != 3) s->first_body.push_back(static_cast<uint32_t>(s->shape.size()));

@@ -320,3 +320,10 @@ def test_set_worlds_rejects_bad_partition(pk):
             g.set_worlds(bad)
     g.set_worlds(fb)
     g.close()
+
+
+def test_granular_pile_bitexact(pk, oracle):
+    """The granular-pile generator (UV spheres and boxes of 1-3 cm on a lattice over
+    the 64-box hopper) at 343 bodies, 60 steps of 1/240 s: every step bit-exact."""
+    t = _run(pk, oracle, 4, 343, 60)
+    assert t["pairs"] > 0
