@@ -450,8 +450,11 @@ template <int FC, int PC, class I> struct epa_store
     static constexpr int fcap = FC, pcap = PC;
     using idx_t = I;
     static constexpr I null = static_cast<I>(~I(0));
-    double nx[FC], ny[FC], nz[FC], d[FC]; // face normal and plane distance
-    double hd[FC];                        // heap: plane distance of the face in each slot
+    struct geo
+    {
+        double nx, ny, nz, d; // face normal and plane distance: one 32-byte record
+    } g[FC];
+    double hd[FC]; // heap: plane distance of the face in each slot
     double px[PC], py[PC], pz[PC];        // polytope points
     uint32_t pia[PC], pib[PC];            // their support features
     I fv[3][FC], fa[3][FC];               // vertices; face across edge (v[i], v[i+1])
@@ -469,7 +472,7 @@ template <class S> struct epa_solver
     __device__ explicit epa_solver(S &store) : m(store) {}
 
     __device__ __forceinline__ v3 P(int i) const { return mk(m.px[i], m.py[i], m.pz[i]); }
-    __device__ __forceinline__ v3 N(int f) const { return mk(m.nx[f], m.ny[f], m.nz[f]); }
+    __device__ __forceinline__ v3 N(int f) const { return mk(m.g[f].nx, m.g[f].ny, m.g[f].nz); }
 
     __device__ void init_face(int fi, int i, int j, int k, int opp) // :273-297
     {
@@ -496,10 +499,10 @@ template <class S> struct epa_solver
         m.fa[1][fi] = S::null;
         m.fa[2][fi] = S::null;
         m.obs[fi] = 0;
-        m.nx[fi] = nrm.x;
-        m.ny[fi] = nrm.y;
-        m.nz[fi] = nrm.z;
-        m.d[fi] = dot(nrm, pi);
+        m.g[fi].nx = nrm.x;
+        m.g[fi].ny = nrm.y;
+        m.g[fi].nz = nrm.z;
+        m.g[fi].d = dot(nrm, pi);
     }
     __device__ __forceinline__ int vidx(int f, int va) const
     { return (m.fv[0][f] == va) ? 0 : (m.fv[1][f] == va ? 1 : 2); }
@@ -529,7 +532,7 @@ template <class S> struct epa_solver
     __device__ __forceinline__ void push(int f)
     {
         ++nheap;
-        sift_up(nheap - 1, f, m.d[f]);
+        sift_up(nheap - 1, f, m.g[f].d);
     }
     __device__ int pop()
     {
@@ -587,7 +590,7 @@ template <class S> struct epa_solver
                 int nb = ca[i];
                 if (nb == S::null) continue;
                 if (m.obs[nb]) continue;
-                if (dot(N(nb), p) > m.d[nb] + tol)
+                if (dot(N(nb), p) > m.g[nb].d + tol)
                 {
                     m.obs[nb] = 1;
                     m.stk[sp++] = static_cast<typename S::idx_t>(nb); // each face is pushed at most once
@@ -637,7 +640,7 @@ template <class S> struct epa_solver
         const v3 p0a = support_at(A, m.pia[i0]), p1a = support_at(A, m.pia[i1]), p2a = support_at(A, m.pia[i2]);
         const v3 p0b = support_at(B, m.pib[i0]), p1b = support_at(B, m.pib[i1]), p2b = support_at(B, m.pib[i2]);
         v3 nrm = N(fi);
-        double fd = m.d[fi];
+        double fd = m.g[fi].d;
         v3 pm = nrm * fd;
         v3 v0 = q1 - q0, v1 = q2 - q0, v2 = pm - q0;
         double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
@@ -689,7 +692,7 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         int mf = e.pop();
         if (mf < 0) break;
         v3 nrm = e.N(mf);
-        double fd = store.d[mf];
+        double fd = store.g[mf].d;
         spt p = msupport<1, UV>(A, B, nrm, 0);
         double pd = dot(nrm, p.p);
         if (pd - fd < tol)
