@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="batched workloads: skip the per-world drop-in comparison")
     ap.add_argument("--copy-check", type=int, default=1024, metavar="MIB",
                     help="after the timed steps, time a plain device copy of MIB MiB (measured "
                          "copy bandwidth; the calibration kernel for HBM byte counters); 0 skips it")
@@ -260,7 +262,7 @@ def main():
         "roofline": roof,
     }
     g.close()
-    if len(s["first_body"]) > 2:
+    if len(s["first_body"]) > 2 and not args.no_dropin:
         out["config"]["per_world_dropin"] = per_world_dropin(s, args.warmup, args.steps)
         out["config"]["batched_over_per_world"] = round(out["value"] / out["config"]["per_world_dropin"]["value"], 2)
     if args.copy_check:
