@@ -96,6 +96,47 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
+def fp32_drift(s, steps):
+    """FP32 against FP64 solve from the same start. per_step: the largest one-step
+    position difference, max over bodies of |x32 - x64| / max(|x64|, 1 m), with the
+    FP32 world reset to the FP64 bodies after every step; trajectory: the same
+    difference without resets after step 1, 10 and `steps`."""
+    import physkit_amd as pk
+
+    def pair():
+        w64 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
+        w32 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"] | pk.FLAG_FP32_SOLVE)
+        for w in (w64, w32):
+            w.load_scene(s)
+            if len(s["first_body"]) > 2:
+                w.set_worlds(s["first_body"])
+        return w64, w32
+
+    def err(a, b):
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(a), 1.0)))
+
+    out = {"per_step": 0.0, "trajectory": {}}
+    ids = np.arange(len(s["shape"]), dtype=np.uint32)
+    w64, w32 = pair()
+    for k in range(1, steps + 1):
+        w64.step(s["dt"])
+        w32.step(s["dt"])
+        a = w64.bodies()
+        out["per_step"] = max(out["per_step"], err(a["pos"], w32.bodies()["pos"]))
+        w32.set_state(ids, pos=a["pos"], vel=a["vel"], angvel=a["angvel"], quat=a["quat"])
+    w64.close()
+    w32.close()
+    w64, w32 = pair()
+    for k in range(1, steps + 1):
+        w64.step(s["dt"])
+        w32.step(s["dt"])
+        if k in (1, 10, steps):
+            out["trajectory"][f"step_{k}"] = err(w64.bodies()["pos"], w32.bodies()["pos"])
+    w64.close()
+    w32.close()
+    return out
+
+
 def per_world_dropin(s, warmup, steps, n_sample=64):
     """The plain FP64 drop-in on the batched workload: each of the first n_sample worlds
     stepped alone through its own context (one world per physkit::world), warmup steps
@@ -183,6 +224,9 @@ def main():
     ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--fp32-solve", action="store_true",
+                    help="FP32 contact solve with FP64 velocity accumulation; reports the position "
+                         "drift against the FP64 solve")
     ap.add_argument("--no-dropin", action="store_true",
                     help="batched workloads: skip the per-world drop-in comparison")
     ap.add_argument("--copy-check", type=int, default=1024, metavar="MIB",
@@ -198,7 +242,8 @@ def main():
 
     kind, desc = WORKLOADS[args.workload]
     s = pk.scene(kind, None, args.n)
-    g = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
+    flags = s["flags"] | (pk.FLAG_FP32_SOLVE if args.fp32_solve else 0)
+    g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
     g.load_scene(s)
     if len(s["first_body"]) > 2:
         g.set_worlds(s["first_body"])
@@ -246,7 +291,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f64 geometry, f32 solver rows" if args.fp32_solve else "f64", "data": "synthetic",
         "config": {"workload": desc, "bodies": len(s["shape"]), "dynamic_bodies": n_dyn,
                    "pairs_per_step": c["pairs"], "gjk_hits_per_step": c["gjk_hits"],
                    "last_step_counters": {k: c[k] for k in ("contacts", "rows", "colours", "epa_overflow",
@@ -262,6 +307,8 @@ def main():
         "roofline": roof,
     }
     g.close()
+    if args.fp32_solve:
+        out["config"]["fp32_drift"] = fp32_drift(s, args.warmup + args.steps)
     if len(s["first_body"]) > 2 and not args.no_dropin:
         out["config"]["per_world_dropin"] = per_world_dropin(s, args.warmup, args.steps)
         out["config"]["batched_over_per_world"] = round(out["value"] / out["config"]["per_world_dropin"]["value"], 2)

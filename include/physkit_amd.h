@@ -48,6 +48,11 @@ typedef struct pkg_ctx pkg_ctx;
  * return such a result from a face spanned by three collinear support points
  * (collision.cpp:284-285, 442-445), which then propagates NaN through the solver. */
 #define PKG_FLAG_EPA_GUARD 8u
+/* Mixed-precision contact solve (not in the reference, whose solver is FP64): the
+ * rows are built in FP64 and solved in FP32 (J.v, lambda, clamps, friction block),
+ * each body's velocity increment added to its FP64 velocity. Geometry (pairs, GJK,
+ * EPA, manifolds) stays FP64 and bit-exact; results drift from the FP64 solve. */
+#define PKG_FLAG_FP32_SOLVE 16u
 
 /* Mirrors world_desc (include/physkit/core/world.h:23-49): gravity and solver
  * iterations; defaults there are (0,-9.81,0) and 10. */

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libphyskit_amd.so")
 PKG_OK = 0
 STATUS = {0: "PKG_OK", 1: "PKG_INVALID_ARG", 2: "PKG_OOM", 3: "PKG_OVERFLOW_PAIRS",
           4: "PKG_OVERFLOW_EPA", 5: "PKG_DEVICE_ERROR", 6: "PKG_NO_DEVICE"}
-FLAG_ALL_MOVED, FLAG_NO_SOLVER, FLAG_NO_INTEGRATE, FLAG_EPA_GUARD = 1, 2, 4, 8
+FLAG_ALL_MOVED, FLAG_NO_SOLVER, FLAG_NO_INTEGRATE, FLAG_EPA_GUARD, FLAG_FP32_SOLVE = 1, 2, 4, 8, 16
 
 D = C.POINTER(C.c_double)
 U8 = C.POINTER(C.c_uint8)

@@ -438,12 +438,26 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             rs.cap4 = 4 * scap;
             rs.d = rows_s.p;
             rs.valid = row_valid_s.p;
-            launch_rows_gather(ord, n_act, r, rs, scap, st);
-            for (uint32_t k = 0; k < c; ++k) launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
-            for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+            if (cfg.flags & PKG_FLAG_FP32_SOLVE)
+            {
+                launch_rows_gather32(ord, n_act, r, rs, scap, st);
                 for (uint32_t k = 0; k < c; ++k)
-                    launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
-            launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+                    launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
+                for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                    for (uint32_t k = 0; k < c; ++k)
+                        launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
+                launch_impulse_writeback32(ord, n_act, MF(nxt), rs, scap, st);
+            }
+            else
+            {
+                launch_rows_gather(ord, n_act, r, rs, scap, st);
+                for (uint32_t k = 0; k < c; ++k)
+                    launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
+                for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                    for (uint32_t k = 0; k < c; ++k)
+                        launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
+                launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+            }
         }
     }
     record(ev, 7, st);

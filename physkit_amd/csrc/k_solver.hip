@@ -401,6 +401,167 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// ---- FP32 solve with FP64 per-body velocity accumulation ----
+// Rows are built in FP64 (k_setup) and gathered once as FP32; the row arithmetic of
+// the solve (J.v, lambda, clamping, the 2x2 friction block) runs in FP32 and each
+// body's velocity increment is added to its FP64 velocity.
+struct f3
+{
+    float x, y, z;
+};
+__device__ __forceinline__ f3 f3m(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 to_f3(v3 a) { return {static_cast<float>(a.x), static_cast<float>(a.y), static_cast<float>(a.z)}; }
+__device__ __forceinline__ float dotf(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 scl(f3 a, float k) { return {a.x * k, a.y * k, a.z * k}; }
+__device__ __forceinline__ f3 addf(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 widen(f3 a) { return mk(static_cast<double>(a.x), static_cast<double>(a.y), static_cast<double>(a.z)); }
+
+struct rows32_dev
+{
+    uint32_t cap4;
+    float *d;
+    uint8_t *valid;
+};
+__device__ __forceinline__ float &rg(const rows32_dev &r, int f, uint32_t slot)
+{ return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
+__device__ __forceinline__ f3 rg3(const rows32_dev &r, int f, uint32_t slot)
+{ return f3m(rg(r, f, slot), rg(r, f + 1, slot), rg(r, f + 2, slot)); }
+
+__global__ void k_rows_gather32(const uint32_t *order, uint32_t n_act, rows_dev src, rows32_dev dst, uint32_t cap)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_act) return;
+    uint32_t p = order[t];
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t s = 4 * p + static_cast<uint32_t>(c), d = static_cast<uint32_t>(c) * cap + t;
+        uint8_t v = src.valid[s];
+        dst.valid[d] = v;
+        if (!v) continue;
+        for (int f = 0; f < kRowF; ++f)
+            dst.d[static_cast<size_t>(f) * dst.cap4 + d] =
+                static_cast<float>(src.d[static_cast<size_t>(f) * src.cap4 + s]);
+    }
+}
+
+struct body32
+{
+    v3 vel, ang; // FP64 accumulators
+    float im;
+    float I[9];
+    bool stat;
+};
+__device__ __forceinline__ f3 mulI(const float *I, f3 v)
+{ return f3m((I[0] * v.x + I[1] * v.y) + I[2] * v.z, (I[3] * v.x + I[4] * v.y) + I[5] * v.z, (I[6] * v.x + I[7] * v.y) + I[8] * v.z); }
+__device__ __forceinline__ body32 ld_body32(const bodies_dev &b, uint32_t i)
+{
+    body32 s;
+    s.vel = ld3(b.vel, b.n, i);
+    s.ang = ld3(b.ang, b.n, i);
+    s.im = static_cast<float>(b.inv_mass[i]);
+    m3 I = ld9(b.Iw_inv, b.n, i);
+    for (int k = 0; k < 9; ++k) s.I[k] = static_cast<float>(I.m[k]);
+    s.stat = b.is_static[i] != 0;
+    return s;
+}
+__device__ __forceinline__ float jv32(f3 Jv, f3 Jwa, f3 Jwb, const body32 &A, const body32 &B)
+{ return ((dotf(Jv, to_f3(A.vel)) + dotf(Jwa, to_f3(A.ang))) - dotf(Jv, to_f3(B.vel))) + dotf(Jwb, to_f3(B.ang)); }
+// impulse a along a row (or two rows): FP32 increments, FP64 accumulation
+__device__ __forceinline__ void push32(body32 &A, body32 &B, f3 lin, f3 angA, f3 angB)
+{
+    if (!A.stat)
+    {
+        A.vel = A.vel + widen(scl(lin, A.im));
+        A.ang = A.ang + widen(mulI(A.I, angA));
+    }
+    if (!B.stat)
+    {
+        B.vel = B.vel - widen(scl(lin, B.im));
+        B.ang = B.ang + widen(mulI(B.I, angB));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_solve32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows32_dev r,
+              uint32_t cap, int warm)
+{
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= end) return;
+    uint32_t p = order[t];
+    uint64_t key = keys[p];
+    uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    body32 A = ld_body32(b, ia), B = ld_body32(b, ib);
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t slot = static_cast<uint32_t>(c) * cap + t;
+        if (!r.valid[slot]) continue;
+        f3 nJv = rg3(r, F_NJV, slot), nJa = rg3(r, F_NJWA, slot), nJb = rg3(r, F_NJWB, slot);
+        f3 aJv = rg3(r, F_T1JV, slot), aJa = rg3(r, F_T1JWA, slot), aJb = rg3(r, F_T1JWB, slot);
+        f3 bJv = rg3(r, F_T2JV, slot), bJa = rg3(r, F_T2JWA, slot), bJb = rg3(r, F_T2JWB, slot);
+        float nacc = rg(r, F_NACC, slot), acc1 = rg(r, F_T1ACC, slot), acc2 = rg(r, F_T2ACC, slot);
+        if (warm)
+        {
+            if (nacc != 0.0f) push32(A, B, scl(nJv, nacc), scl(nJa, nacc), scl(nJb, nacc));
+            if (acc1 != 0.0f) push32(A, B, scl(aJv, acc1), scl(aJa, acc1), scl(aJb, acc1));
+            if (acc2 != 0.0f) push32(A, B, scl(bJv, acc2), scl(bJa, acc2), scl(bJb, acc2));
+            continue;
+        }
+        // normal row, lambda >= 0 (constraint.h:49-72)
+        float jv = jv32(nJv, nJa, nJb, A, B);
+        float lambda = -(jv + rg(r, F_NBIAS, slot)) * rg(r, F_NMEFF, slot);
+        float nv = nacc + lambda;
+        nv = nv < 0.0f ? 0.0f : nv;
+        float ap = nv - nacc;
+        nacc = nv;
+        push32(A, B, scl(nJv, ap), scl(nJa, ap), scl(nJb, ap));
+        // friction block on the cone (constraint.h:1131-1187)
+        float maxf = rg(r, F_MU, slot) * nacc;
+        float b1 = jv32(aJv, aJa, aJb, A, B), b2 = jv32(bJv, bJa, bJb, A, B);
+        float i11 = rg(r, F_I11, slot), i12 = rg(r, F_I12, slot), i22 = rg(r, F_I22, slot);
+        float n1 = acc1 - (i11 * b1 + i12 * b2), n2 = acc2 - (i12 * b1 + i22 * b2);
+        float mag = sqrtf(n1 * n1 + n2 * n2);
+        if (mag > maxf)
+        {
+            n1 = n1 * (maxf / mag);
+            n2 = n2 * (maxf / mag);
+        }
+        float a1 = n1 - acc1, a2 = n2 - acc2;
+        acc1 = n1;
+        acc2 = n2;
+        push32(A, B, addf(scl(aJv, a1), scl(bJv, a2)), addf(scl(aJa, a1), scl(bJa, a2)), addf(scl(aJb, a1), scl(bJb, a2)));
+        rg(r, F_NACC, slot) = nacc;
+        rg(r, F_T1ACC, slot) = acc1;
+        rg(r, F_T2ACC, slot) = acc2;
+    }
+    if (!A.stat)
+    {
+        st3(b.vel, b.n, ia, A.vel);
+        st3(b.ang, b.n, ia, A.ang);
+    }
+    if (!B.stat)
+    {
+        st3(b.vel, b.n, ib, B.vel);
+        st3(b.ang, b.n, ib, B.ang);
+    }
+}
+
+__global__ void k_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows32_dev r, uint32_t cap)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_act) return;
+    uint32_t p = order[t];
+    for (int c = 0; c < 4; ++c)
+    {
+        uint32_t slot = static_cast<uint32_t>(c) * cap + t;
+        if (!r.valid[slot]) continue;
+        double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+        size_t s = m.cap;
+        d[10 * s] = static_cast<double>(rg(r, F_NACC, slot));
+        d[11 * s] = static_cast<double>(rg(r, F_T1ACC, slot));
+        d[12 * s] = static_cast<double>(rg(r, F_T2ACC, slot));
+    }
+}
+
 __global__ void k_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap) // :1191-1196
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,6 +636,31 @@ void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_d
                               hipStream_t st)
 {
     if (n_act) k_writeback<<<grid(n_act), kBlock, 0, st>>>(order, n_act, m, r, cap);
+}
+
+static rows32_dev as32(const rows_dev &r)
+{
+    rows32_dev o;
+    o.cap4 = r.cap4;
+    o.d = reinterpret_cast<float *>(r.d);
+    o.valid = r.valid;
+    return o;
+}
+void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
+                          uint32_t cap, hipStream_t st)
+{
+    if (n_act) k_rows_gather32<<<grid(n_act), kBlock, 0, st>>>(order, n_act, src, as32(dst32), cap);
+}
+void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
+                           const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st)
+{
+    if (end > begin)
+        k_solve32<<<grid(end - begin), kBlock, 0, st>>>(keys, order, begin, end, b, as32(r32), cap, warm ? 1 : 0);
+}
+void launch_impulse_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r32, uint32_t cap,
+                                hipStream_t st)
+{
+    if (n_act) k_writeback32<<<grid(n_act), kBlock, 0, st>>>(order, n_act, m, as32(r32), cap);
 }
 
 } // namespace pk

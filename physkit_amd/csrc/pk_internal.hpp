@@ -223,5 +223,13 @@ void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t b
                          const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,
                               hipStream_t st);
+// FP32 solve (rows gathered as FP32 planes into a rows_dev whose d holds floats;
+// body velocities accumulate in FP64)
+void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
+                          uint32_t cap, hipStream_t st);
+void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
+                           const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st);
+void launch_impulse_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r32, uint32_t cap,
+                                hipStream_t st);
 
 } // namespace pk

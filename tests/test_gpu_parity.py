@@ -327,3 +327,32 @@ def test_granular_pile_bitexact(pk, oracle):
     the 64-box hopper) at 343 bodies, 60 steps of 1/240 s: every step bit-exact."""
     t = _run(pk, oracle, 4, 343, 60)
     assert t["pairs"] > 0
+
+
+def test_fp32_solve_drift_small(pk):
+    """FP32 solve (FLAG_FP32_SOLVE) against the FP64 solve on the pyramids, one step at a
+    time from a shared state: after each step the FP32 world's bodies are reset to the
+    FP64 world's, so the difference measured is that of one step. It stays within 1e-5
+    relative to max(|x|, 1 m) (trajectories of exactly stacked boxes diverge much
+    faster, since contacts at zero penetration switch on and off)."""
+    s = pk.scene(pk.SCENE_PYRAMIDS, None, 4)
+    w64 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
+    w32 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"] | pk.FLAG_FP32_SOLVE)
+    w64.load_scene(s)
+    w32.load_scene(s)
+    ids = np.arange(len(s["shape"]), dtype=np.uint32)
+    errs = []
+    rows = 0
+    for k in range(30):
+        w64.step(s["dt"])
+        w32.step(s["dt"])
+        a, b = w64.bodies(), w32.bodies()
+        assert np.all(np.isfinite(b["pos"]))
+        errs.append(float(np.max(np.abs(a["pos"] - b["pos"]) / np.maximum(np.abs(a["pos"]), 1.0))))
+        rows += w32.counters()["rows"]
+        w32.set_state(ids, pos=a["pos"], vel=a["vel"], angvel=a["angvel"], quat=a["quat"])
+    print("fp32 solve rel. position error per step:", max(errs))
+    assert rows > 0
+    assert max(errs) < 1e-5
+    w64.close()
+    w32.close()
