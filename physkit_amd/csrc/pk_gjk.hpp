@@ -457,10 +457,18 @@ template <int FC, int PC, class I> struct epa_store
     double hd[FC]; // heap: plane distance of the face in each slot
     double px[PC], py[PC], pz[PC];        // polytope points
     uint32_t pia[PC], pib[PC];            // their support features
-    I fv[3][FC], fa[3][FC];               // vertices; face across edge (v[i], v[i+1])
-    I hf[FC];                             // heap: face in each slot
-    I stk[FC], es[FC], ee[FC], eadj[FC];  // silhouette stack and horizon edges
-    uint8_t obs[FC];
+    // vertices, face across edge (v[i], v[i+1]) and the obsolete flag of a face, packed
+    // in one aligned word, so a face's bookkeeping is one memory access, not seven
+    struct alignas(8 * sizeof(I)) meta
+    {
+        I v[3], a[3], obs, pad;
+    } fm[FC];
+    struct alignas(4 * sizeof(I)) edge
+    {
+        I s, e, adj, pad; // horizon edge (s, e) and the face across it
+    } eg[FC];
+    I hf[FC];  // heap: face in each slot
+    I stk[FC]; // silhouette stack
 };
 using epa_store_fast = epa_store<192, 68, uint8_t>;    // every hull solve seen so far fits
 using epa_store_big = epa_store<4096, 68, uint16_t>;   // overflow path, global memory
@@ -492,26 +500,32 @@ template <class S> struct epa_solver
             v2 = t;
             nrm = -nrm;
         }
-        m.fv[0][fi] = static_cast<typename S::idx_t>(i);
-        m.fv[1][fi] = static_cast<typename S::idx_t>(v1);
-        m.fv[2][fi] = static_cast<typename S::idx_t>(v2);
-        m.fa[0][fi] = S::null;
-        m.fa[1][fi] = S::null;
-        m.fa[2][fi] = S::null;
-        m.obs[fi] = 0;
+        typename S::meta mt;
+        mt.v[0] = static_cast<typename S::idx_t>(i);
+        mt.v[1] = static_cast<typename S::idx_t>(v1);
+        mt.v[2] = static_cast<typename S::idx_t>(v2);
+        mt.a[0] = S::null;
+        mt.a[1] = S::null;
+        mt.a[2] = S::null;
+        mt.obs = 0;
+        mt.pad = 0;
+        m.fm[fi] = mt;
         m.g[fi].nx = nrm.x;
         m.g[fi].ny = nrm.y;
         m.g[fi].nz = nrm.z;
         m.g[fi].d = dot(nrm, pi);
     }
     __device__ __forceinline__ int vidx(int f, int va) const
-    { return (m.fv[0][f] == va) ? 0 : (m.fv[1][f] == va ? 1 : 2); }
+    {
+        typename S::meta mt = m.fm[f];
+        return (mt.v[0] == va) ? 0 : (mt.v[1] == va ? 1 : 2);
+    }
     __device__ __forceinline__ void link(int f1, int f2, int va, int vb) // :305-313
     {
         int e1 = vidx(f1, va);
         int e2 = vidx(f2, vb);
-        m.fa[e1][f1] = static_cast<typename S::idx_t>(f2);
-        m.fa[e2][f2] = static_cast<typename S::idx_t>(f1);
+        m.fm[f1].a[e1] = static_cast<typename S::idx_t>(f2);
+        m.fm[f2].a[e2] = static_cast<typename S::idx_t>(f1);
     }
     // min-heap on plane distance, in the sift order of libstdc++'s push_heap/pop_heap
     __device__ __forceinline__ void sift_up(int hole, int vf, double vd)
@@ -569,7 +583,7 @@ template <class S> struct epa_solver
                 sift_up(hole, vf, vd);
             }
             --nheap;
-            if (!m.obs[top]) return top;
+            if (!m.fm[top].obs) return top;
         }
         return -1;
     }
@@ -579,28 +593,30 @@ template <class S> struct epa_solver
         const double tol = 1e-6;
         int sp = 0, nh = 0;
         m.stk[sp++] = static_cast<typename S::idx_t>(f0);
-        m.obs[f0] = 1;
+        m.fm[f0].obs = 1;
         while (sp > 0)
         {
             int cur = m.stk[--sp];
-            int cv[3] = {m.fv[0][cur], m.fv[1][cur], m.fv[2][cur]};
-            int ca[3] = {m.fa[0][cur], m.fa[1][cur], m.fa[2][cur]};
+            typename S::meta cm = m.fm[cur];
             for (int i = 0; i < 3; ++i)
             {
-                int nb = ca[i];
+                int nb = cm.a[i];
                 if (nb == S::null) continue;
-                if (m.obs[nb]) continue;
+                if (m.fm[nb].obs) continue;
                 if (dot(N(nb), p) > m.g[nb].d + tol)
                 {
-                    m.obs[nb] = 1;
+                    m.fm[nb].obs = 1;
                     m.stk[sp++] = static_cast<typename S::idx_t>(nb); // each face is pushed at most once
                 }
                 else
                 {
                     if (nh >= S::fcap) return -1;
-                    m.es[nh] = static_cast<typename S::idx_t>(cv[i]);
-                    m.ee[nh] = static_cast<typename S::idx_t>(cv[(i + 1) % 3]);
-                    m.eadj[nh] = static_cast<typename S::idx_t>(nb);
+                    typename S::edge ed;
+                    ed.s = cm.v[i];
+                    ed.e = cm.v[(i + 1) % 3];
+                    ed.adj = static_cast<typename S::idx_t>(nb);
+                    ed.pad = 0;
+                    m.eg[nh] = ed;
                     ++nh;
                 }
             }
@@ -620,14 +636,14 @@ template <class S> struct epa_solver
             for (int j = i + 1; j < 4; ++j)
                 for (int e1 = 0; e1 < 3; ++e1)
                 {
-                    int u1 = m.fv[e1][i], v1 = m.fv[(e1 + 1) % 3][i];
+                    int u1 = m.fm[i].v[e1], v1 = m.fm[i].v[(e1 + 1) % 3];
                     for (int e2 = 0; e2 < 3; ++e2)
                     {
-                        int u2 = m.fv[e2][j], v2 = m.fv[(e2 + 1) % 3][j];
+                        int u2 = m.fm[j].v[e2], v2 = m.fm[j].v[(e2 + 1) % 3];
                         if (u1 == v2 && v1 == u2)
                         {
-                            m.fa[e1][i] = static_cast<typename S::idx_t>(j);
-                            m.fa[e2][j] = static_cast<typename S::idx_t>(i);
+                            m.fm[i].a[e1] = static_cast<typename S::idx_t>(j);
+                            m.fm[j].a[e2] = static_cast<typename S::idx_t>(i);
                         }
                     }
                 }
@@ -635,7 +651,8 @@ template <class S> struct epa_solver
     // :424-454; out = normal3, world_a3, world_b3, depth
     __device__ void barycentric(int fi, const gshape &A, const gshape &B, double *out) const
     {
-        int i0 = m.fv[0][fi], i1 = m.fv[1][fi], i2 = m.fv[2][fi];
+        typename S::meta fmt = m.fm[fi];
+        int i0 = fmt.v[0], i1 = fmt.v[1], i2 = fmt.v[2];
         const v3 q0 = P(i0), q1 = P(i1), q2 = P(i2);
         const v3 p0a = support_at(A, m.pia[i0]), p1a = support_at(A, m.pia[i1]), p2a = support_at(A, m.pia[i2]);
         const v3 p0b = support_at(B, m.pib[i0]), p1b = support_at(B, m.pib[i1]), p2b = support_at(B, m.pib[i2]);
@@ -711,17 +728,20 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         for (int h = 0; h < nh; ++h)
         {
             int f = base + h;
-            e.init_face(f, store.es[h], store.ee[h], pi, -1);
-            e.link(f, store.eadj[h], store.es[h], store.ee[h]);
+            typename S::edge ed = store.eg[h];
+            e.init_face(f, ed.s, ed.e, pi, -1);
+            e.link(f, ed.adj, ed.s, ed.e);
             e.push(f);
         }
         e.nfaces += nh;
         for (int i = 0; i < nh; ++i)
         {
-            int hs_i = store.es[i], he_i = store.ee[i];
+            typename S::edge ei = store.eg[i];
+            int hs_i = ei.s, he_i = ei.e;
             for (int j = i + 1; j < nh; ++j)
             {
-                int hs_j = store.es[j], he_j = store.ee[j];
+                typename S::edge ej = store.eg[j];
+                int hs_j = ej.s, he_j = ej.e;
                 if (he_i == hs_j)
                     e.link(base + i, base + j, he_i, pi);
                 else if (hs_i == he_j)
