@@ -141,6 +141,8 @@ struct pkg_ctx
     dbuf<double> rows;
     dbuf<uint8_t> row_valid;
     dbuf<uint32_t> colour, colour_start, order, order_alt;
+    dbuf<uint32_t> par, iv, iv_alt, iflags, ipos, istarts; // contact islands
+    dbuf<uint64_t> ik, ik_alt;
     dbuf<uint64_t> ckeys, ckeys_alt;
     dbuf<double> rows_s;      // solver rows in solve order
     dbuf<uint8_t> row_valid_s;
@@ -418,45 +420,94 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
             }
             n_colours = c;
-            colour_start.reserve(c + 1);
-            order.reserve(Pn);
-            order_alt.reserve(Pn);
-            ckeys.reserve(Pn);
-            ckeys_alt.reserve(Pn);
-            uint32_t *ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
-                                                 colour_start.p, sc, st);
-            std::vector<uint32_t> cs(c + 1);
-            hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
-                      "colour copy");
-            hip_check(hipStreamSynchronize(st), "stream sync");
-            // rows of the solver manifolds gathered into solve order
-            const uint32_t n_act = cs[c];
-            const uint32_t scap = std::max<uint32_t>(n_act, 1);
-            rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
-            row_valid_s.reserve(static_cast<size_t>(4) * scap);
-            rows_dev rs;
-            rs.cap4 = 4 * scap;
-            rs.d = rows_s.p;
-            rs.valid = row_valid_s.p;
-            if (cfg.flags & PKG_FLAG_FP32_SOLVE)
+            const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
+            bool solved = false;
+            if (c <= kIslandMaxColours)
             {
-                launch_rows_gather32(ord, n_act, r, rs, scap, st);
-                for (uint32_t k = 0; k < c; ++k)
-                    launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
-                for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
-                    for (uint32_t k = 0; k < c; ++k)
-                        launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
-                launch_impulse_writeback32(ord, n_act, MF(nxt), rs, scap, st);
+                // one workgroup per contact island when every island is small enough
+                par.reserve(std::max<uint32_t>(n, 1));
+                ik.reserve(Pn);
+                ik_alt.reserve(Pn);
+                iv.reserve(Pn);
+                iv_alt.reserve(Pn);
+                iflags.reserve(Pn);
+                ipos.reserve(Pn);
+                istarts.reserve(Pn);
+                bsums.reserve(scan_block_sums(Pn));
+                total.reserve(2);
+                reset_counter(C_ISL_MAX);
+                island_sort is = launch_island_sort(keys[nxt].p, colour.p, Pn, c, b, par.p, ik.p, iv.p, ik_alt.p,
+                                                    iv_alt.p, counters.p + C_ISL_MAX, sc, st);
+                launch_island_segments(is, Pn, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
+                uint32_t h2[2] = {0, 0};
+                hip_check(hipMemcpyAsync(h2, total.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "island count");
+                hip_check(hipStreamSynchronize(st), "stream sync");
+                const uint32_t n_isl = h2[0], n_act = h2[1];
+                launch_island_maxlen(istarts.p, n_isl, n_act, counters.p + C_ISL_MAX, st);
+                if (read_counter(C_ISL_MAX) <= kIslandMaxManifolds)
+                {
+                    const uint32_t scap = std::max<uint32_t>(n_act, 1);
+                    rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
+                    row_valid_s.reserve(static_cast<size_t>(4) * scap);
+                    rows_dev rs;
+                    rs.cap4 = 4 * scap;
+                    rs.d = rows_s.p;
+                    rs.valid = row_valid_s.p;
+                    if (f32)
+                        launch_rows_gather32(is.ord, n_act, r, rs, scap, st);
+                    else
+                        launch_rows_gather(is.ord, n_act, r, rs, scap, st);
+                    launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
+                                         cfg.solver_iterations, f32, st);
+                    if (f32)
+                        launch_impulse_writeback32(is.ord, n_act, MF(nxt), rs, scap, st);
+                    else
+                        launch_impulse_writeback(is.ord, n_act, MF(nxt), rs, scap, st);
+                    solved = true;
+                }
             }
-            else
+            if (!solved)
             {
-                launch_rows_gather(ord, n_act, r, rs, scap, st);
-                for (uint32_t k = 0; k < c; ++k)
-                    launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
-                for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                colour_start.reserve(c + 1);
+                order.reserve(Pn);
+                order_alt.reserve(Pn);
+                ckeys.reserve(Pn);
+                ckeys_alt.reserve(Pn);
+                uint32_t *ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
+                                                     colour_start.p, sc, st);
+                std::vector<uint32_t> cs(c + 1);
+                hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                          "colour copy");
+                hip_check(hipStreamSynchronize(st), "stream sync");
+                // rows of the solver manifolds gathered into solve order
+                const uint32_t n_act = cs[c];
+                const uint32_t scap = std::max<uint32_t>(n_act, 1);
+                rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
+                row_valid_s.reserve(static_cast<size_t>(4) * scap);
+                rows_dev rs;
+                rs.cap4 = 4 * scap;
+                rs.d = rows_s.p;
+                rs.valid = row_valid_s.p;
+                if (f32)
+                {
+                    launch_rows_gather32(ord, n_act, r, rs, scap, st);
                     for (uint32_t k = 0; k < c; ++k)
-                        launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
-                launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+                        launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
+                    for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                        for (uint32_t k = 0; k < c; ++k)
+                            launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
+                    launch_impulse_writeback32(ord, n_act, MF(nxt), rs, scap, st);
+                }
+                else
+                {
+                    launch_rows_gather(ord, n_act, r, rs, scap, st);
+                    for (uint32_t k = 0; k < c; ++k)
+                        launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
+                    for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
+                        for (uint32_t k = 0; k < c; ++k)
+                            launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
+                    launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+                }
             }
         }
     }

@@ -355,13 +355,10 @@ __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, 
     return v;
 }
 
-// one colour: positions [begin, end) of the solve order, rows gathered by k_rows_gather
-__global__ void __launch_bounds__(kBlock)
-    k_solve(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows_dev r,
-            uint32_t cap, int warm)
+// the manifold at position t of the solve order (rows gathered by k_rows_gather)
+__device__ __forceinline__ void solve_one(const uint64_t *keys, const uint32_t *order, uint32_t t, const bodies_dev &b,
+                                          const rows_dev &r, uint32_t cap, int warm)
 {
-    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= end) return;
     uint32_t p = order[t];
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
@@ -401,6 +398,15 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// one colour: positions [begin, end) of the solve order
+__global__ void __launch_bounds__(kBlock)
+    k_solve(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows_dev r,
+            uint32_t cap, int warm)
+{
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < end) solve_one(keys, order, t, b, r, cap, warm);
+}
+
 // ---- FP32 solve with FP64 per-body velocity accumulation ----
 // Rows are built in FP64 (k_setup) and gathered once as FP32; the row arithmetic of
 // the solve (J.v, lambda, clamping, the 2x2 friction block) runs in FP32 and each
@@ -422,6 +428,15 @@ struct rows32_dev
     float *d;
     uint8_t *valid;
 };
+// FP32 rows live in a rows_dev buffer whose planes hold floats
+__host__ __device__ __forceinline__ rows32_dev as32(const rows_dev &r)
+{
+    rows32_dev o;
+    o.cap4 = r.cap4;
+    o.d = reinterpret_cast<float *>(r.d);
+    o.valid = r.valid;
+    return o;
+}
 __device__ __forceinline__ float &rg(const rows32_dev &r, int f, uint32_t slot)
 { return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
 __device__ __forceinline__ f3 rg3(const rows32_dev &r, int f, uint32_t slot)
@@ -481,12 +496,9 @@ __device__ __forceinline__ void push32(body32 &A, body32 &B, f3 lin, f3 angA, f3
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_solve32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows32_dev r,
-              uint32_t cap, int warm)
+__device__ __forceinline__ void solve_one32(const uint64_t *keys, const uint32_t *order, uint32_t t, const bodies_dev &b,
+                                            const rows32_dev &r, uint32_t cap, int warm)
 {
-    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= end) return;
     uint32_t p = order[t];
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
@@ -545,6 +557,14 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+__global__ void __launch_bounds__(kBlock)
+    k_solve32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end, bodies_dev b, rows32_dev r,
+              uint32_t cap, int warm)
+{
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < end) solve_one32(keys, order, t, b, r, cap, warm);
+}
+
 __global__ void k_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows32_dev r, uint32_t cap)
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,6 +596,182 @@ __global__ void k_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev
         d[10 * s] = rf(r, F_NACC, slot);
         d[11 * s] = rf(r, F_T1ACC, slot);
         d[12 * s] = rf(r, F_T2ACC, slot);
+    }
+}
+
+// ---- contact islands: connected components of the row manifolds over dynamic bodies ----
+// Union-find with hooking of the larger root under the smaller: every component's root
+// is its smallest body id, whatever order the unions run in.
+__device__ __forceinline__ uint32_t par_ld(uint32_t *par, uint32_t x)
+{ return __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ uint32_t uf_find(uint32_t *par, uint32_t x)
+{
+    uint32_t p = par_ld(par, x);
+    while (p != x)
+    {
+        uint32_t gp = par_ld(par, p);
+        if (gp != p) __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); // path halving
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__global__ void k_uf_init(uint32_t *par, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) par[i] = i;
+}
+
+__global__ void k_uf_union(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                           bodies_dev b, uint32_t *par)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs || colour[p] >= n_colours) return;
+    uint64_t key = keys[p];
+    uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    if (b.is_static[a] || b.is_static[c]) return;
+    for (uint32_t guard = 0; guard < 4096u; ++guard)
+    {
+        uint32_t ra = uf_find(par, a), rc = uf_find(par, c);
+        if (ra == rc) return;
+        uint32_t hi = ra > rc ? ra : rc, lo = ra > rc ? rc : ra;
+        uint32_t expect = hi;
+        if (__hip_atomic_compare_exchange_strong(&par[hi], &expect, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+__global__ void k_uf_flatten(uint32_t *par, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) par[i] = uf_find(par, i);
+}
+
+// a union that gave up (never seen; bounded so no thread can spin forever) leaves two
+// bodies of one manifold in different islands: flagged, and the caller falls back
+__global__ void k_uf_verify(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                            bodies_dev b, const uint32_t *par, unsigned long long *fail)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs || colour[p] >= n_colours) return;
+    uint64_t key = keys[p];
+    uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    if (!b.is_static[a] && !b.is_static[c] && par[a] != par[c]) atomicMax(fail, ~0ull);
+}
+
+// sort key of a row manifold: (island root << cbits) | colour; others sort last
+__global__ void k_island_keys(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                              bodies_dev b, const uint32_t *par, int cbits, uint64_t last, uint64_t *out,
+                              uint32_t *vals)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    uint32_t c = colour[p];
+    uint64_t k = last;
+    if (c < n_colours)
+    {
+        uint64_t key = keys[p];
+        uint32_t a = static_cast<uint32_t>(key >> 32);
+        if (b.is_static[a]) a = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        k = (static_cast<uint64_t>(par[a]) << cbits) | c;
+    }
+    out[p] = k;
+    vals[p] = p;
+}
+
+__global__ void k_island_flags(const uint64_t *sk, uint32_t n, uint64_t last, int cbits, uint32_t *flags)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    flags[t] = (sk[t] != last && (t == 0 || (sk[t] >> cbits) != (sk[t - 1] >> cbits))) ? 1u : 0u;
+}
+
+__global__ void k_island_starts(const uint32_t *flags, const uint32_t *pos, uint32_t n, uint32_t *starts)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && flags[t]) starts[pos[t]] = t;
+}
+
+__global__ void k_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t len = 0;
+    if (i < n_isl) len = (i + 1 < n_isl ? starts[i + 1] : n_act) - starts[i];
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        uint32_t o = __shfl_xor(len, off);
+        len = o > len ? o : len;
+    }
+    if (__lane_id() == 0 && len) atomicMax(mx, static_cast<unsigned long long>(len));
+}
+
+// number of row manifolds: position of the first `last` key
+__global__ void k_count_below(const uint64_t *sk, uint32_t n, uint64_t last, uint32_t *out)
+{
+    if (blockIdx.x || threadIdx.x) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi)
+    {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (sk[mid] < last)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    *out = lo;
+}
+
+// One workgroup per chunk of whole islands: chunk k holds the islands whose first
+// position lies in [k C, (k+1) C) of the island-sorted order. Warm start then `iters`
+// passes; each pass steps through the colours with a workgroup barrier between them,
+// every thread solving its positions of the current colour. Manifolds of one colour
+// share no dynamic body, and the islands of a chunk share none at all, so this is the
+// colour-ordered solve restricted to the chunk.
+constexpr int kIslandThreads = 256;
+constexpr uint32_t kChunkManifolds = 256;
+__device__ __forceinline__ uint32_t lower_u32(const uint32_t *a, uint32_t n, uint32_t key)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi)
+    {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+template <bool F32>
+__global__ void __launch_bounds__(kIslandThreads)
+    k_solve_islands(const uint64_t *keys, const uint32_t *order, const uint64_t *sk, const uint32_t *starts,
+                    uint32_t n_isl, uint32_t n_act, uint32_t n_colours, int cbits, bodies_dev b, rows_dev r,
+                    uint32_t cap, uint32_t iters)
+{
+    const uint32_t k = blockIdx.x;
+    const uint32_t lo = lower_u32(starts, n_isl, k * kChunkManifolds);
+    const uint32_t hi = lower_u32(starts, n_isl, (k + 1) * kChunkManifolds);
+    if (lo >= hi) return; // no island starts in this chunk
+    const uint32_t s0 = starts[lo], e0 = hi < n_isl ? starts[hi] : n_act;
+    const uint64_t cmask = (1ull << cbits) - 1;
+    for (uint32_t pass = 0; pass <= iters; ++pass)
+    {
+        const int warm = pass == 0 ? 1 : 0;
+        for (uint32_t c = 0; c < n_colours; ++c)
+        {
+            for (uint32_t t = s0 + threadIdx.x; t < e0; t += kIslandThreads)
+            {
+                if ((sk[t] & cmask) != c) continue;
+                if (F32)
+                    solve_one32(keys, order, t, b, as32(r), cap, warm);
+                else
+                    solve_one(keys, order, t, b, r, cap, warm);
+            }
+            __syncthreads();
+        }
     }
 }
 } // namespace
@@ -638,14 +834,59 @@ void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_d
     if (n_act) k_writeback<<<grid(n_act), kBlock, 0, st>>>(order, n_act, m, r, cap);
 }
 
-static rows32_dev as32(const rows_dev &r)
+island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                               const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv, uint64_t *ik_alt,
+                               uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc, hipStream_t st)
 {
-    rows32_dev o;
-    o.cap4 = r.cap4;
-    o.d = reinterpret_cast<float *>(r.d);
-    o.valid = r.valid;
-    return o;
+    island_sort is{};
+    const uint32_t n = b.n;
+    k_uf_init<<<grid(n), kBlock, 0, st>>>(par, n);
+    k_uf_union<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par);
+    k_uf_flatten<<<grid(n), kBlock, 0, st>>>(par, n);
+    k_uf_verify<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par, fail);
+    int cbits = 1;
+    while (cbits < 31 && (1u << cbits) <= n_colours) ++cbits;
+    int ibits = 1;
+    while (ibits < 32 && (1ull << ibits) < n) ++ibits;
+    const int bits = cbits + ibits;
+    is.cbits = cbits;
+    is.last = (1ull << bits) - 1;
+    k_island_keys<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par, cbits, is.last, ik, iv);
+    int shifts[8];
+    int ns = 0;
+    for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
+    bool alt = radix_sort_u64(ik, iv, ik_alt, iv_alt, n_pairs, shifts, ns, sc, st);
+    is.sk = alt ? ik_alt : ik;
+    is.ord = alt ? iv_alt : iv;
+    return is;
 }
+void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *flags, uint32_t *pos, uint32_t *bsums,
+                            uint32_t *total, uint32_t *starts, uint32_t *n_act, hipStream_t st)
+{
+    k_count_below<<<1, 64, 0, st>>>(is.sk, n_pairs, is.last, n_act);
+    k_island_flags<<<grid(n_pairs), kBlock, 0, st>>>(is.sk, n_pairs, is.last, is.cbits, flags);
+    exclusive_scan_u32(flags, pos, n_pairs, bsums, total, st);
+    k_island_starts<<<grid(n_pairs), kBlock, 0, st>>>(flags, pos, n_pairs, starts);
+}
+void launch_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx,
+                          hipStream_t st)
+{
+    if (n_isl) k_island_maxlen<<<grid(n_isl), kBlock, 0, st>>>(starts, n_isl, n_act, mx);
+}
+void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
+                          uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
+                          uint32_t iters, bool f32, hipStream_t st)
+{
+    if (!n_isl) return;
+    const unsigned nb = static_cast<unsigned>((n_act + kChunkManifolds - 1) / kChunkManifolds);
+    if (f32)
+        k_solve_islands<true><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act, n_colours,
+                                                              is.cbits, b, r, cap, iters);
+    else
+        k_solve_islands<false><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act, n_colours,
+                                                               is.cbits, b, r, cap, iters);
+}
+
 void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
                           uint32_t cap, hipStream_t st)
 {

@@ -75,6 +75,7 @@ enum counter_slot
     C_TREE_FAIL = 11,
     C_DEFER = 12,
     C_KEEP = 13,
+    C_ISL_MAX = 14,
     C_NCOUNTERS = 16
 };
 
@@ -223,6 +224,30 @@ void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t b
                          const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,
                               hipStream_t st);
+// Contact islands (connected components of the row manifolds over dynamic bodies,
+// each rooted at its smallest body id), manifolds sorted by (island, colour): a
+// workgroup solves a chunk of whole islands, every colour and pass, in one launch.
+struct island_sort
+{
+    const uint64_t *sk;  // sorted keys (island << cbits | colour; `last` for no rows)
+    const uint32_t *ord; // manifold index at each sorted position
+    int cbits;
+    uint64_t last;
+};
+constexpr uint32_t kIslandMaxManifolds = 4096; // larger islands use the per-colour launches
+constexpr uint32_t kIslandMaxColours = 64;  // above: the per-colour launches
+// `fail` becomes all-ones if the islands could not be built (the caller then falls back)
+island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
+                               const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv, uint64_t *ik_alt,
+                               uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc, hipStream_t st);
+// island starts (positions where the island changes) and the row-manifold count
+void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *flags, uint32_t *pos, uint32_t *bsums,
+                            uint32_t *total, uint32_t *starts, uint32_t *n_act, hipStream_t st);
+void launch_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx,
+                          hipStream_t st);
+void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
+                          uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
+                          uint32_t iters, bool f32, hipStream_t st);
 // FP32 solve (rows gathered as FP32 planes into a rows_dev whose d holds floats;
 // body velocities accumulate in FP64)
 void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
