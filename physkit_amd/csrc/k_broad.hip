@@ -266,6 +266,52 @@ __device__ __forceinline__ void finish_node(const lbvh_dev &t, uint32_t node)
 __device__ __forceinline__ uint32_t wg_arrive(uint32_t *p)
 { return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
+// refit state of the nodes inside one workgroup's leaf chunk, in LDS: a node whose leaf
+// range lies in [c_lo, c_hi] has its own index in that range (Karras nodes sit at one
+// end of their range), so slot = node - c_lo
+struct refit_lds
+{
+    uint32_t visit[kChunk];
+    float box[kChunk][6];
+    uint8_t moved[kChunk];
+};
+
+__device__ __forceinline__ fbox child_box_local(const lbvh_dev &t, uint32_t c, uint32_t c_lo, const refit_lds &L)
+{
+    uint32_t n = t.n;
+    if (c >= n - 1) return child_box(t, c);
+    fbox f;
+    const float *b = L.box[c - c_lo];
+    for (int k = 0; k < 3; ++k)
+    {
+        f.lo[k] = b[k];
+        f.hi[k] = b[3 + k];
+    }
+    f.moved = L.moved[c - c_lo];
+    return f;
+}
+
+// finish_node for a node inside the chunk: children from LDS (or the leaf boxes), the
+// result to LDS for the parent and to the node arrays for traversal
+__device__ __forceinline__ void finish_local(const lbvh_dev &t, uint32_t node, uint32_t c_lo, refit_lds &L)
+{
+    uint32_t l = static_cast<uint32_t>(t.left[node]), r = static_cast<uint32_t>(t.right[node]);
+    fbox a = child_box_local(t, l, c_lo, L), b = child_box_local(t, r, c_lo, L);
+    float lo[3] = {fminf(a.lo[0], b.lo[0]), fminf(a.lo[1], b.lo[1]), fminf(a.lo[2], b.lo[2])};
+    float hi[3] = {fmaxf(a.hi[0], b.hi[0]), fmaxf(a.hi[1], b.hi[1]), fmaxf(a.hi[2], b.hi[2])};
+    uint32_t mv = a.moved & b.moved;
+    float *lb = L.box[node - c_lo];
+    for (int k = 0; k < 3; ++k)
+    {
+        lb[k] = lo[k];
+        lb[3 + k] = hi[k];
+    }
+    L.moved[node - c_lo] = static_cast<uint8_t>(mv);
+    t.inode_lo[node] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(l));
+    t.inode_hi[node] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(t.escape[node]));
+    t.lastf[node] = t.last[node] | (mv << 31);
+}
+
 __device__ __forceinline__ uint32_t dev_arrive(uint32_t *p)
 { return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT); }
 
@@ -276,18 +322,28 @@ __device__ __forceinline__ uint32_t dev_arrive(uint32_t *p)
 // acquire/release.
 __global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
 {
+    __shared__ refit_lds L;
     uint32_t n = t.n;
     uint32_t c_lo = blockIdx.x * kChunk;
     uint32_t c_hi = min(c_lo + kChunk, n) - 1;
+    for (uint32_t i = threadIdx.x; i < kChunk; i += blockDim.x) L.visit[i] = 0;
+    __syncthreads();
     for (uint32_t i = c_lo + threadIdx.x; i <= c_hi; i += blockDim.x)
     {
         uint32_t node = t.parent[n - 1 + i];
         for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
         {
             bool local = t.first[node] >= c_lo && t.last[node] <= c_hi;
-            uint32_t before = local ? wg_arrive(&t.visit[node]) : dev_arrive(&t.visit[node]);
-            if (before == 0) break;
-            finish_node(t, node);
+            if (local)
+            {
+                if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
+                finish_local(t, node, c_lo, L);
+            }
+            else
+            {
+                if (dev_arrive(&t.visit[node]) == 0) break;
+                finish_node(t, node);
+            }
             node = t.parent[node];
         }
     }
