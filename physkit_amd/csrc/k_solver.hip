@@ -126,11 +126,54 @@ __device__ __forceinline__ rowv make_row(const body_state &A, const body_state &
     return r;
 }
 
+// Body velocities seen by a manifold solve: in device memory by body id (vel_global),
+// or in a workgroup's LDS table by slot for its dynamic bodies (vel_lds; static bodies,
+// never written, are read from device memory).
+constexpr uint16_t kNoSlot = 0xFFFF;
+struct vel_global
+{
+    __device__ __forceinline__ void load(const bodies_dev &b, uint32_t id, uint16_t, v3 &v, v3 &w) const
+    {
+        v = ld3(b.vel, b.n, id);
+        w = ld3(b.ang, b.n, id);
+    }
+    __device__ __forceinline__ void store(const bodies_dev &b, uint32_t id, uint16_t, v3 v, v3 w) const
+    {
+        st3(b.vel, b.n, id, v);
+        st3(b.ang, b.n, id, w);
+    }
+};
+struct vel_lds
+{
+    double (*L)[6];
+    __device__ __forceinline__ void load(const bodies_dev &b, uint32_t id, uint16_t slot, v3 &v, v3 &w) const
+    {
+        if (slot == kNoSlot)
+        {
+            v = ld3(b.vel, b.n, id);
+            w = ld3(b.ang, b.n, id);
+            return;
+        }
+        v = mk(L[slot][0], L[slot][1], L[slot][2]);
+        w = mk(L[slot][3], L[slot][4], L[slot][5]);
+    }
+    __device__ __forceinline__ void store(const bodies_dev &, uint32_t, uint16_t slot, v3 v, v3 w) const
+    {
+        L[slot][0] = v.x;
+        L[slot][1] = v.y;
+        L[slot][2] = v.z;
+        L[slot][3] = w.x;
+        L[slot][4] = w.y;
+        L[slot][5] = w.z;
+    }
+};
+
+// inverse mass, inverse world inertia and the static flag (velocities come separately)
 __device__ __forceinline__ body_state ld_body(const bodies_dev &b, uint32_t i)
 {
     body_state s;
-    s.vel = ld3(b.vel, b.n, i);
-    s.ang = ld3(b.ang, b.n, i);
+    s.vel = mk(0.0, 0.0, 0.0);
+    s.ang = mk(0.0, 0.0, 0.0);
     s.im = b.inv_mass[i];
     s.Iwi = ld9(b.Iw_inv, b.n, i);
     s.stat = b.is_static[i] != 0;
@@ -171,6 +214,8 @@ __global__ void __launch_bounds__(kBlock)
             uint64_t key = keys[p];
             uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
             body_state A = ld_body(b, ia), B = ld_body(b, ib);
+            vel_global().load(b, ia, kNoSlot, A.vel, A.ang);
+            vel_global().load(b, ib, kNoSlot, B.vel, B.ang);
             v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
             qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
             double e = smax(b.restitution[ia], b.restitution[ib]);
@@ -356,13 +401,17 @@ __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, 
 }
 
 // the manifold at position t of the solve order (rows gathered by k_rows_gather)
+template <class V = vel_global>
 __device__ __forceinline__ void solve_one(const uint64_t *keys, const uint32_t *order, uint32_t t, const bodies_dev &b,
-                                          const rows_dev &r, uint32_t cap, int warm)
+                                          const rows_dev &r, uint32_t cap, int warm, const V &vs = V(),
+                                          uint16_t sa = kNoSlot, uint16_t sb = kNoSlot)
 {
     uint32_t p = order[t];
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
     body_state A = ld_body(b, ia), B = ld_body(b, ib);
+    vs.load(b, ia, sa, A.vel, A.ang);
+    vs.load(b, ib, sb, B.vel, B.ang);
     for (int c = 0; c < 4; ++c)
     {
         uint32_t slot = static_cast<uint32_t>(c) * cap + t;
@@ -386,16 +435,8 @@ __device__ __forceinline__ void solve_one(const uint64_t *keys, const uint32_t *
             rf(r, F_T2ACC, slot) = t2.acc;
         }
     }
-    if (!A.stat)
-    {
-        st3(b.vel, b.n, ia, A.vel);
-        st3(b.ang, b.n, ia, A.ang);
-    }
-    if (!B.stat)
-    {
-        st3(b.vel, b.n, ib, B.vel);
-        st3(b.ang, b.n, ib, B.ang);
-    }
+    if (!A.stat) vs.store(b, ia, sa, A.vel, A.ang);
+    if (!B.stat) vs.store(b, ib, sb, B.vel, B.ang);
 }
 
 // one colour: positions [begin, end) of the solve order
@@ -471,8 +512,6 @@ __device__ __forceinline__ f3 mulI(const float *I, f3 v)
 __device__ __forceinline__ body32 ld_body32(const bodies_dev &b, uint32_t i)
 {
     body32 s;
-    s.vel = ld3(b.vel, b.n, i);
-    s.ang = ld3(b.ang, b.n, i);
     s.im = static_cast<float>(b.inv_mass[i]);
     m3 I = ld9(b.Iw_inv, b.n, i);
     for (int k = 0; k < 9; ++k) s.I[k] = static_cast<float>(I.m[k]);
@@ -496,13 +535,17 @@ __device__ __forceinline__ void push32(body32 &A, body32 &B, f3 lin, f3 angA, f3
     }
 }
 
+template <class V = vel_global>
 __device__ __forceinline__ void solve_one32(const uint64_t *keys, const uint32_t *order, uint32_t t, const bodies_dev &b,
-                                            const rows32_dev &r, uint32_t cap, int warm)
+                                            const rows32_dev &r, uint32_t cap, int warm, const V &vs = V(),
+                                            uint16_t sa = kNoSlot, uint16_t sb = kNoSlot)
 {
     uint32_t p = order[t];
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
     body32 A = ld_body32(b, ia), B = ld_body32(b, ib);
+    vs.load(b, ia, sa, A.vel, A.ang);
+    vs.load(b, ib, sb, B.vel, B.ang);
     for (int c = 0; c < 4; ++c)
     {
         uint32_t slot = static_cast<uint32_t>(c) * cap + t;
@@ -545,16 +588,8 @@ __device__ __forceinline__ void solve_one32(const uint64_t *keys, const uint32_t
         rg(r, F_T1ACC, slot) = acc1;
         rg(r, F_T2ACC, slot) = acc2;
     }
-    if (!A.stat)
-    {
-        st3(b.vel, b.n, ia, A.vel);
-        st3(b.ang, b.n, ia, A.ang);
-    }
-    if (!B.stat)
-    {
-        st3(b.vel, b.n, ib, B.vel);
-        st3(b.ang, b.n, ib, B.ang);
-    }
+    if (!A.stat) vs.store(b, ia, sa, A.vel, A.ang);
+    if (!B.stat) vs.store(b, ib, sb, B.vel, B.ang);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -745,18 +780,99 @@ __device__ __forceinline__ uint32_t lower_u32(const uint32_t *a, uint32_t n, uin
     }
     return lo;
 }
+// Chunks of at most kLdsManifolds manifolds keep the velocities of their dynamic bodies
+// in LDS for all passes: body ids are sorted in LDS (bitonic), a body's slot is its
+// first position in the sorted list, and the velocities are written back at the end.
+constexpr uint32_t kLdsManifolds = 512;
+constexpr uint32_t kLdsIds = 2 * kLdsManifolds;
+__device__ __forceinline__ uint16_t slot_of(const uint32_t *ids, uint32_t id)
+{
+    uint32_t lo = 0, hi = kLdsIds;
+    while (lo < hi)
+    {
+        uint32_t mid = (lo + hi) >> 1;
+        if (ids[mid] < id)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return static_cast<uint16_t>(lo);
+}
+
 template <bool F32>
 __global__ void __launch_bounds__(kIslandThreads)
     k_solve_islands(const uint64_t *keys, const uint32_t *order, const uint64_t *sk, const uint32_t *starts,
                     uint32_t n_isl, uint32_t n_act, uint32_t n_colours, int cbits, bodies_dev b, rows_dev r,
                     uint32_t cap, uint32_t iters)
 {
+    __shared__ uint32_t ids[kLdsIds];
+    __shared__ double lv[kLdsIds][6];
+    __shared__ uint16_t sl[2][kLdsManifolds];
     const uint32_t k = blockIdx.x;
     const uint32_t lo = lower_u32(starts, n_isl, k * kChunkManifolds);
     const uint32_t hi = lower_u32(starts, n_isl, (k + 1) * kChunkManifolds);
     if (lo >= hi) return; // no island starts in this chunk
     const uint32_t s0 = starts[lo], e0 = hi < n_isl ? starts[hi] : n_act;
     const uint64_t cmask = (1ull << cbits) - 1;
+    const uint32_t m = e0 - s0;
+    const bool use_lds = m <= kLdsManifolds;
+    if (use_lds)
+    {
+        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        {
+            uint32_t id = 0xFFFFFFFFu;
+            if (i < 2 * m)
+            {
+                uint64_t key = keys[order[s0 + i / 2]];
+                uint32_t x = (i & 1) ? static_cast<uint32_t>(key & 0xFFFFFFFFu) : static_cast<uint32_t>(key >> 32);
+                if (!b.is_static[x]) id = x;
+            }
+            ids[i] = id;
+        }
+        __syncthreads();
+        for (uint32_t size = 2; size <= kLdsIds; size <<= 1) // bitonic sort
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+            {
+                for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+                {
+                    uint32_t j = i ^ stride;
+                    if (j > i)
+                    {
+                        bool up = (i & size) == 0;
+                        uint32_t x = ids[i], y = ids[j];
+                        if ((x > y) == up)
+                        {
+                            ids[i] = y;
+                            ids[j] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        {
+            uint32_t id = ids[i];
+            if (id != 0xFFFFFFFFu && (i == 0 || ids[i - 1] != id))
+            {
+                v3 v = ld3(b.vel, b.n, id), w = ld3(b.ang, b.n, id);
+                lv[i][0] = v.x;
+                lv[i][1] = v.y;
+                lv[i][2] = v.z;
+                lv[i][3] = w.x;
+                lv[i][4] = w.y;
+                lv[i][5] = w.z;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < m; i += kIslandThreads)
+        {
+            uint64_t key = keys[order[s0 + i]];
+            uint32_t x = static_cast<uint32_t>(key >> 32), y = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+            sl[0][i] = b.is_static[x] ? kNoSlot : slot_of(ids, x);
+            sl[1][i] = b.is_static[y] ? kNoSlot : slot_of(ids, y);
+        }
+        __syncthreads();
+    }
+    vel_lds L{lv};
     for (uint32_t pass = 0; pass <= iters; ++pass)
     {
         const int warm = pass == 0 ? 1 : 0;
@@ -765,7 +881,15 @@ __global__ void __launch_bounds__(kIslandThreads)
             for (uint32_t t = s0 + threadIdx.x; t < e0; t += kIslandThreads)
             {
                 if ((sk[t] & cmask) != c) continue;
-                if (F32)
+                if (use_lds)
+                {
+                    uint16_t sa = sl[0][t - s0], sb = sl[1][t - s0];
+                    if (F32)
+                        solve_one32(keys, order, t, b, as32(r), cap, warm, L, sa, sb);
+                    else
+                        solve_one(keys, order, t, b, r, cap, warm, L, sa, sb);
+                }
+                else if (F32)
                     solve_one32(keys, order, t, b, as32(r), cap, warm);
                 else
                     solve_one(keys, order, t, b, r, cap, warm);
@@ -773,6 +897,16 @@ __global__ void __launch_bounds__(kIslandThreads)
             __syncthreads();
         }
     }
+    if (use_lds)
+        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        {
+            uint32_t id = ids[i];
+            if (id != 0xFFFFFFFFu && (i == 0 || ids[i - 1] != id))
+            {
+                st3(b.vel, b.n, id, mk(lv[i][0], lv[i][1], lv[i][2]));
+                st3(b.ang, b.n, id, mk(lv[i][3], lv[i][4], lv[i][5]));
+            }
+        }
 }
 } // namespace
 
