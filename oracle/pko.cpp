@@ -827,7 +827,7 @@ namespace
 struct jrow // constraint.h:29-47
 {
     v3 Jv, Jwa, Jwb;
-    double Meff = 0, bias = 0, acc = 0, lo = -DBL_MAX, hi = DBL_MAX;
+    double Meff = 0, bias = 0, gamma = 0, acc = 0, lo = -DBL_MAX, hi = DBL_MAX;
 };
 struct cpoint // constraint.h:1203-1212
 {
@@ -843,7 +843,7 @@ inline double row_jv(const jrow &r, const body &A, const body &B) // constraint.
 void row_apply(jrow &r, body &A, body &B) // constraint.h:49-72
 {
     double jv = row_jv(r, A, B);
-    double cfp = 0.0 * r.acc;
+    double cfp = r.gamma * r.acc;
     double lambda = -((jv + r.bias) + cfp) * r.Meff;
     double old = r.acc;
     double v = old + lambda;
@@ -960,7 +960,249 @@ void friction_block(cpoint &pt) // constraint.h:1131-1187
         B.angvel = B.angvel + mul(B.Iw_inv, t1.Jwb * a1 + t2.Jwb * a2);
     }
 }
+// effective mass denominators of the joint rows, each in the reference's term order
+inline v3 ka_of(const body &A, const jrow &r) { return mul(A.Iw_inv, r.Jwa); }
+inline v3 kb_of(const body &B, const jrow &r) { return mul(B.Iw_inv, r.Jwb); }
+
+jrow joint_linear_row(v3 n, v3 ra, v3 rb) // J = [n, ra x n, -n, -(rb x n)]
+{
+    jrow r;
+    r.Jv = n;
+    r.Jwa = cross(ra, n);
+    r.Jwb = -cross(rb, n);
+    return r;
+}
+
+jrow joint_angular_row(v3 n) // J = [0, n, 0, -n] (angular_scale 1 m)
+{
+    jrow r;
+    v3 jt = n * 1.0;
+    r.Jv = v3{0.0, 0.0, 0.0};
+    r.Jwa = jt;
+    r.Jwb = -jt;
+    return r;
+}
+
+double angular_meff(const body &A, const body &B, const jrow &r)
+{ return 1.0 / (dot(r.Jwa, ka_of(A, r)) + dot(r.Jwb, kb_of(B, r))); }
+
+// orientation_error of slider / weld (constraint.h:689-697, 812-820), rotated to world
+v3 orientation_error_world(const body &A, const body &B, qt qrel)
+{
+    qt cur = qmul(conj(A.q), B.q);
+    qt oe = qmul(qrel, conj(cur));
+    if (oe.w < 0) oe = qt{-oe.w, -oe.x, -oe.y, -oe.z};
+    return rotate(A.q, v3{oe.x, oe.y, oe.z} * 2.0);
+}
+
+// build_jacobian_impl of every joint type; returns the number of rows written to out
+int build_joint_rows(const joint &j, const body &A, const body &B, double dt, jrow *out)
+{
+    const v3 axes[3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    v3 ra = rotate(A.q, j.la);
+    v3 rb = rotate(B.q, j.lb);
+    switch (j.type)
+    {
+    case j_distance: // constraint.h:186-228
+    case j_spring:   // constraint.h:291-335
+    {
+        v3 pa = A.pos + ra, pb = B.pos + rb;
+        v3 delta = pa - pb;
+        double cur = norm(delta);
+        if (cur < 1e-6) return 0;
+        v3 u = delta / cur;
+        jrow r;
+        r.Jv = u;
+        r.Jwa = cross(ra, u);
+        r.Jwb = -cross(rb, u);
+        v3 ka = ka_of(A, r), kb = kb_of(B, r);
+        double mc = ((A.inv_mass + B.inv_mass) + dot(r.Jwa, ka)) + dot(r.Jwb, kb);
+        if (j.type == j_distance)
+        {
+            r.Meff = 1.0 / mc;
+            r.bias = (0.1 / dt) * (cur - j.distance);
+        }
+        else
+        {
+            double gamma = 1.0 / (dt * (j.damping + dt * j.stiffness));
+            double beta = (dt * j.stiffness) / (j.damping + dt * j.stiffness);
+            r.Meff = 1.0 / (mc + gamma);
+            r.bias = (beta / dt) * (cur - j.distance);
+            r.gamma = gamma;
+        }
+        r.acc = j.acc[0];
+        out[0] = r;
+        return 1;
+    }
+    case j_ball_socket: // constraint.h:386-425
+    case j_hinge:       // constraint.h:509-573
+    case j_weld:        // constraint.h:763-834
+    {
+        const double bf = j.type == j_hinge ? 0.2 : 0.1;
+        v3 delta = (A.pos + ra) - (B.pos + rb);
+        int k = 0;
+        for (const v3 &n : axes)
+        {
+            jrow r = joint_linear_row(n, ra, rb);
+            v3 ka = ka_of(A, r), kb = kb_of(B, r);
+            r.Meff = 1.0 / (((A.inv_mass + B.inv_mass) + dot(r.Jwa, ka)) + dot(r.Jwb, kb));
+            r.bias = (bf / dt) * dot(delta, n);
+            r.acc = j.acc[static_cast<std::size_t>(k)];
+            out[k++] = r;
+        }
+        if (j.type == j_hinge)
+        {
+            v3 ua = rotate(A.q, j.ax_a), ub = rotate(B.q, j.ax_b);
+            v3 aerr = -cross(ua, ub);
+            const v3 dirs[2] = {rotate(A.q, j.f1), rotate(A.q, j.f2)};
+            for (const v3 &n : dirs)
+            {
+                jrow r = joint_angular_row(n);
+                r.Meff = angular_meff(A, B, r);
+                r.bias = (bf / dt) * (dot(aerr, n) * 1.0);
+                r.acc = j.acc[static_cast<std::size_t>(k)];
+                out[k++] = r;
+            }
+        }
+        else if (j.type == j_weld)
+        {
+            v3 aerr = orientation_error_world(A, B, j.qrel);
+            for (const v3 &n : axes)
+            {
+                jrow r = joint_angular_row(n);
+                r.Meff = angular_meff(A, B, r);
+                r.bias = ((bf / dt) * dot(aerr, n)) * 1.0;
+                r.acc = j.acc[static_cast<std::size_t>(k)];
+                out[k++] = r;
+            }
+        }
+        return k;
+    }
+    case j_slider: // constraint.h:630-713
+    {
+        v3 delta = (A.pos + ra) - (B.pos + rb);
+        v3 axis_a = rotate(A.q, j.ax_a);
+        v3 t1 = rotate(A.q, j.f1), t2 = rotate(A.q, j.f2);
+        v3 lever = ra - delta;
+        int k = 0;
+        for (const v3 &n : {t1, t2})
+        {
+            jrow r;
+            r.Jv = n;
+            r.Jwa = cross(lever, n);
+            r.Jwb = -cross(rb, n);
+            v3 ka = ka_of(A, r), kb = kb_of(B, r);
+            double lin = A.inv_mass + B.inv_mass;
+            double ang = dot(r.Jwa, ka) + dot(r.Jwb, kb);
+            double reg = 0.01 * ang;
+            r.gamma = reg;
+            r.Meff = 1.0 / ((lin + ang) + reg);
+            double e = dot(delta, n);
+            e = (e < -0.2) ? -0.2 : ((0.2 < e) ? 0.2 : e); // std::clamp
+            r.bias = (0.1 / dt) * e;
+            r.acc = j.acc[static_cast<std::size_t>(k)];
+            out[k++] = r;
+        }
+        v3 aerr = orientation_error_world(A, B, j.qrel);
+        for (const v3 &n : {axis_a, t1, t2})
+        {
+            jrow r = joint_angular_row(n);
+            r.Meff = angular_meff(A, B, r);
+            r.bias = (0.1 / dt) * (dot(aerr, n) * 1.0);
+            r.acc = j.acc[static_cast<std::size_t>(k)];
+            out[k++] = r;
+        }
+        return k;
+    }
+    default:
+        return 0;
+    }
+}
 } // namespace
+
+int joint_dimension(int type)
+{
+    switch (type)
+    {
+    case j_distance:
+    case j_spring:
+        return 1;
+    case j_ball_socket:
+        return 3;
+    case j_hinge:
+    case j_slider:
+        return 5;
+    case j_weld:
+        return 6;
+    default:
+        return 0;
+    }
+}
+
+bool world::add_joint(const joint_desc &d) // the constructors of constraint.h:180-184, 287-292, 377-382, 502-507, 612-628, 757-762
+{
+    if (d.type < 0 || d.type > j_spring || d.slot >= (1u << 24) || d.a >= bodies.size() || d.b >= bodies.size())
+        return false;
+    if (bodies[d.a].disabled || bodies[d.b].disabled) return false;
+    joint j;
+    j.type = d.type;
+    j.slot = d.slot;
+    j.a = d.a;
+    j.b = d.b;
+    auto it = std::lower_bound(joints.begin(), joints.end(), j.key(),
+                               [](const joint &x, uint32_t k) { return x.key() < k; });
+    if (it != joints.end() && it->key() == j.key()) return false;
+    const body &A = bodies[d.a];
+    const body &B = bodies[d.b];
+    j.distance = d.distance;
+    j.stiffness = d.stiffness;
+    j.damping = d.damping;
+    switch (d.type)
+    {
+    case j_distance:
+    case j_spring:
+        j.la = d.local_a;
+        j.lb = d.local_b;
+        break;
+    case j_hinge:
+        j.la = d.local_a;
+        j.lb = d.local_b;
+        j.ax_a = normalized(d.local_axis_a);
+        j.ax_b = normalized(d.local_axis_b);
+        basis(j.ax_a, j.f1, j.f2);
+        break;
+    case j_ball_socket:
+    case j_weld:
+    case j_slider:
+        // particle::project_to_local (particle.h:107-108)
+        j.la = rotate(conj(A.q), d.anchor - A.pos);
+        j.lb = rotate(conj(B.q), d.anchor - B.pos);
+        if (d.type != j_ball_socket) j.qrel = qmul(conj(A.q), B.q);
+        if (d.type == j_slider)
+        {
+            v3 axn = normalized(d.axis);
+            j.ax_a = rotate(conj(A.q), axn);
+            v3 w1, w2;
+            basis(axn, w1, w2);
+            j.f1 = rotate(conj(A.q), w1);
+            j.f2 = rotate(conj(A.q), w2);
+        }
+        break;
+    default:
+        break;
+    }
+    joints.insert(it, j);
+    return true;
+}
+
+bool world::remove_joint(uint32_t key)
+{
+    auto it = std::lower_bound(joints.begin(), joints.end(), key,
+                               [](const joint &x, uint32_t k) { return x.key() < k; });
+    if (it == joints.end() || it->key() != key) return false;
+    joints.erase(it);
+    return true;
+}
 
 void world::stage_solve(double dt) // constraint.h:1052-1201
 {
@@ -1064,6 +1306,58 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
         for (std::size_t i = 0; i < M; ++i) colours[i] = static_cast<uint32_t>(col[i]);
     }
 
+    // joint rows (constraint.h:1110-1121), all joints in key order; coloured like the
+    // manifolds, over the joints' dynamic bodies, with priority from the joint index
+    const std::size_t J = joints.size();
+    std::vector<jrow> jr(6 * J);
+    for (std::size_t k = 0; k < J; ++k)
+        joints[k].rows = build_joint_rows(joints[k], bodies[joints[k].a], bodies[joints[k].b], dt, &jr[6 * k]);
+    std::vector<std::size_t> jorder;
+    joint_colours.clear();
+    if (solver_mode == 0)
+    {
+        for (std::size_t k = 0; k < J; ++k) jorder.push_back(k);
+    }
+    else if (J)
+    {
+        const uint64_t none = ~0ull;
+        std::vector<uint64_t> claim(bodies.size(), none);
+        std::vector<int> col(J, -1);
+        std::size_t left = J;
+        uint32_t c = 0;
+        while (left > 0)
+        {
+            std::fill(claim.begin(), claim.end(), none);
+            for (std::size_t k = 0; k < J; ++k)
+            {
+                if (col[k] >= 0) continue;
+                const joint &jj = joints[k];
+                uint64_t pr = (static_cast<uint64_t>(colour_mix32(static_cast<uint32_t>(k))) << 32) | k;
+                if (!bodies[jj.a].is_static) claim[jj.a] = std::min(claim[jj.a], pr);
+                if (!bodies[jj.b].is_static) claim[jj.b] = std::min(claim[jj.b], pr);
+            }
+            for (std::size_t k = 0; k < J; ++k)
+            {
+                if (col[k] >= 0) continue;
+                const joint &jj = joints[k];
+                uint64_t pr = (static_cast<uint64_t>(colour_mix32(static_cast<uint32_t>(k))) << 32) | k;
+                if ((bodies[jj.a].is_static || claim[jj.a] == pr) && (bodies[jj.b].is_static || claim[jj.b] == pr))
+                {
+                    col[k] = static_cast<int>(c);
+                    --left;
+                }
+            }
+            ++c;
+        }
+        for (uint32_t cc = 0; cc < c; ++cc)
+            for (std::size_t k = 0; k < J; ++k)
+                if (col[k] == static_cast<int>(cc)) jorder.push_back(k);
+        for (std::size_t k = 0; k < J; ++k) joint_colours.push_back(static_cast<uint32_t>(col[k]));
+    }
+
+    for (std::size_t k : jorder) // warm start, joints first (constraint.h:1124-1125)
+        for (int r = 0; r < joints[k].rows; ++r)
+            row_warm(jr[6 * k + static_cast<std::size_t>(r)], bodies[joints[k].a], bodies[joints[k].b]);
     for (std::size_t oi : order) // warm start
         for (std::size_t r = mstart[oi]; r < mstart[oi + 1]; ++r)
         {
@@ -1073,6 +1367,10 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
             row_warm(p.t2, *p.A, *p.B);
         }
     for (int it = 0; it < iterations; ++it)
+    {
+        for (std::size_t k : jorder) // joint rows first (constraint.h:1134)
+            for (int r = 0; r < joints[k].rows; ++r)
+                row_apply(jr[6 * k + static_cast<std::size_t>(r)], bodies[joints[k].a], bodies[joints[k].b]);
         for (std::size_t oi : order)
             for (std::size_t r = mstart[oi]; r < mstart[oi + 1]; ++r)
             {
@@ -1080,6 +1378,10 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
                 row_apply(p.n, *p.A, *p.B);
                 friction_block(p);
             }
+    }
+    for (std::size_t k = 0; k < J; ++k) // constraint.h:1196-1198
+        for (int r = 0; r < joints[k].rows; ++r)
+            joints[k].acc[static_cast<std::size_t>(r)] = jr[6 * k + static_cast<std::size_t>(r)].acc;
     for (auto &p : pts)
     {
         p.cache->ln = p.n.acc;

@@ -174,6 +174,45 @@ struct event
     int kind; // 1 begin, 2 end
 };
 
+// ---------------- joints (include/physkit/collision/constraint.h:139-834) ----------------
+// Types numbered as constraint_type (constraint.h:14-23). A joint's key is
+// (type << 24) | slot, slot being its index in the per-type arena (arena.h:64-80); the
+// reference builds joint rows arena by arena in type order, slot order within one
+// (constraint.h:1110-1121), which is ascending key order.
+enum joint_type : int
+{
+    j_distance = 0,
+    j_ball_socket = 1,
+    j_hinge = 2,
+    j_slider = 3,
+    j_weld = 4,
+    j_spring = 5,
+};
+struct joint_desc // the desc structs of constraint.h:144-175, 239-285, 357-374, 446-497, 585-609, 737-755
+{
+    int type = 0;
+    uint32_t slot = 0, a = 0, b = 0;
+    v3 local_a, local_b;          // distance, spring, hinge
+    v3 local_axis_a, local_axis_b; // hinge
+    v3 anchor;                    // ball_socket, slider, weld (world frame)
+    v3 axis;                      // slider (world frame)
+    double distance = 0, stiffness = 0, damping = 0;
+};
+struct joint // state the constructors keep (constraint.h:180-184, 287-292, 377-382, 502-507, 612-628, 757-762)
+{
+    int type = 0;
+    uint32_t slot = 0, a = 0, b = 0;
+    v3 la, lb;     // body-frame anchors
+    v3 ax_a, ax_b; // hinge: normalized body-frame axes; slider: ax_a = body-frame axis of a
+    v3 f1, f2;     // hinge: basis of ax_a; slider: body-frame t1, t2 of a
+    qt qrel{1, 0, 0, 0}; // slider, weld: conj(q_a) q_b at construction
+    double distance = 0, stiffness = 0, damping = 0;
+    std::array<double, 6> acc{}; // accumulated impulses, kept across steps for warm starts
+    int rows = 0;                // rows built in the last solve (0: none)
+    uint32_t key() const { return (static_cast<uint32_t>(type) << 24) | slot; }
+};
+int joint_dimension(int type); // jacobian_dimension of each type
+
 struct step_counters
 {
     uint64_t moved = 0, pairs = 0, gjk_hits = 0, contacts = 0, rows = 0, colours = 0;
@@ -207,6 +246,8 @@ struct world
     std::vector<uint64_t> active;     // sorted pair keys
     std::vector<manifold> manifolds;  // parallel to `active`
     std::vector<event> events;        // begin/end events of the last step, manifold order
+    std::vector<joint> joints;        // ascending key
+    std::vector<uint32_t> joint_colours; // colour of each joint in the last solve (colour order)
     std::vector<uint32_t> colours;    // colour of each manifold that produced rows (last step)
     std::vector<uint64_t> colour_keys; // pair key of each of those manifolds
     step_counters counters;
@@ -214,6 +255,10 @@ struct world
     void finalize_world();            // |g| as world.h:383 computes it
     uint32_t add_body(const body &b); // world_base::create_rigid, world.h:202-208
     void step(double dt);             // world::step_impl, src/world.cpp:20-56
+    // world::add_constraint (world.h:387-395): the constructor reads the bodies' current
+    // poses. False if the key is taken or an argument is out of range.
+    bool add_joint(const joint_desc &d);
+    bool remove_joint(uint32_t key); // world::remove_constraint, world.h:397-398
 
     // stages (exposed for stage tests)
     void stage_integrate_vel(double dt);

@@ -191,6 +191,73 @@ size_t pko_get_events(void *h, uint64_t *keys, int32_t *kinds, size_t cap)
 }
 
 // moved, pairs, gjk_hits, contacts, rows, colours
+// a caller moving bodies between steps (pkg_set_body_state): null arrays are left alone;
+// a new orientation refreshes the derived state (particle::orientation, particle.h:40-44)
+void pko_set_bodies(void *h, int count, const uint32_t *ids, const double *pos, const double *vel,
+                    const double *angvel, const double *quat)
+{
+    auto *w = static_cast<world *>(h);
+    for (int t = 0; t < count; ++t)
+    {
+        body &b = w->bodies[ids[t]];
+        if (pos) b.pos = ld3(pos + 3 * t);
+        if (vel) b.vel = ld3(vel + 3 * t);
+        if (angvel) b.angvel = ld3(angvel + 3 * t);
+        if (quat)
+        {
+            b.q = qt{quat[4 * t], quat[4 * t + 1], quat[4 * t + 2], quat[4 * t + 3]};
+            b.update_derived();
+        }
+    }
+}
+
+// prm: local_a 3, local_b 3, local_axis_a 3, local_axis_b 3, anchor 3, axis 3,
+//      distance, stiffness, damping (21 doubles)
+int pko_add_joint(void *h, int type, uint32_t slot, uint32_t a, uint32_t b, const double *prm)
+{
+    joint_desc d;
+    d.type = type;
+    d.slot = slot;
+    d.a = a;
+    d.b = b;
+    d.local_a = ld3(prm);
+    d.local_b = ld3(prm + 3);
+    d.local_axis_a = ld3(prm + 6);
+    d.local_axis_b = ld3(prm + 9);
+    d.anchor = ld3(prm + 12);
+    d.axis = ld3(prm + 15);
+    d.distance = prm[18];
+    d.stiffness = prm[19];
+    d.damping = prm[20];
+    return static_cast<world *>(h)->add_joint(d) ? 1 : 0;
+}
+
+int pko_remove_joint(void *h, uint32_t key) { return static_cast<world *>(h)->remove_joint(key) ? 1 : 0; }
+
+// live joints in key order: key, the 6 accumulated impulses, rows built by the last solve
+size_t pko_get_joints(void *h, uint32_t *keys, double *acc, int32_t *rows, size_t cap)
+{
+    auto *w = static_cast<world *>(h);
+    size_t n = std::min(cap, w->joints.size());
+    for (size_t k = 0; k < n; ++k)
+    {
+        const joint &j = w->joints[k];
+        if (keys) keys[k] = j.key();
+        if (acc)
+            for (int r = 0; r < 6; ++r) acc[6 * k + static_cast<size_t>(r)] = j.acc[static_cast<size_t>(r)];
+        if (rows) rows[k] = j.rows;
+    }
+    return w->joints.size();
+}
+
+size_t pko_get_joint_colours(void *h, uint32_t *out, size_t cap)
+{
+    auto *w = static_cast<world *>(h);
+    size_t n = std::min(cap, w->joint_colours.size());
+    for (size_t k = 0; k < n; ++k) out[k] = w->joint_colours[k];
+    return w->joint_colours.size();
+}
+
 void pko_get_counters(void *h, uint64_t *out)
 {
     auto *w = static_cast<world *>(h);

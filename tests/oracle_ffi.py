@@ -67,6 +67,15 @@ def lib():
         L.pko_sincos.argtypes = [D, D, D, C.c_int]
         L.pko_exp_map.argtypes = [D, C.c_double, D]
         L.pko_inverse3.argtypes = [D, D]
+        L.pko_set_bodies.argtypes = [C.c_void_p, C.c_int, U32, D, D, D, D]
+        L.pko_add_joint.restype = C.c_int
+        L.pko_add_joint.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, D]
+        L.pko_remove_joint.restype = C.c_int
+        L.pko_remove_joint.argtypes = [C.c_void_p, C.c_uint32]
+        L.pko_get_joints.restype = C.c_size_t
+        L.pko_get_joints.argtypes = [C.c_void_p, U32, D, I32, C.c_size_t]
+        L.pko_get_joint_colours.restype = C.c_size_t
+        L.pko_get_joint_colours.argtypes = [C.c_void_p, U32, C.c_size_t]
         _lib = L
     return _lib
 
@@ -167,6 +176,20 @@ def inverse3(m):
     return out.reshape(3, 3)
 
 
+# ---------------- joints ----------------
+JOINT_FIELDS = (("local_a", 3), ("local_b", 3), ("local_axis_a", 3), ("local_axis_b", 3),
+                ("anchor", 3), ("axis", 3), ("distance", 1), ("stiffness", 1), ("damping", 1))
+
+
+def joint_params(j):
+    """The 21 doubles of pko_add_joint from a joint dict (missing fields are zero)."""
+    out = []
+    for name, n in JOINT_FIELDS:
+        v = j.get(name, [0.0] * n if n > 1 else 0.0)
+        out.extend(list(np.ravel(v)) if n > 1 else [float(v)])
+    return f64(out)
+
+
 # ---------------- worlds ----------------
 class OracleWorld:
     """The oracle world. flags: 1 all leaves moved, 2 no solver, 4 no integration.
@@ -244,6 +267,36 @@ class OracleWorld:
         self.L.pko_get_counters(self.h, out.ctypes.data_as(U64))
         return dict(zip(["moved", "pairs", "gjk_hits", "contacts", "rows", "colours"],
                         [int(x) for x in out]))
+
+    def set_bodies(self, ids, pos=None, vel=None, angvel=None, quat=None):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        arrs = [None if a is None else f64(a) for a in (pos, vel, angvel, quat)]
+        self.L.pko_set_bodies(self.h, len(ids), ids.ctypes.data_as(U32),
+                              *[None if a is None else dp(a) for a in arrs])
+
+    def add_joint(self, j):
+        """j: a joint dict (type, slot, a, b and the desc fields; see joint_params)."""
+        return bool(self.L.pko_add_joint(self.h, int(j["type"]), int(j.get("slot", 0)), int(j["a"]),
+                                         int(j["b"]), dp(joint_params(j))))
+
+    def remove_joint(self, key):
+        return bool(self.L.pko_remove_joint(self.h, int(key)))
+
+    def joints(self):
+        n = self.L.pko_get_joints(self.h, None, None, None, 0)
+        keys = np.zeros(n, dtype=np.uint32)
+        acc = np.zeros((n, 6))
+        rows = np.zeros(n, dtype=np.int32)
+        if n:
+            self.L.pko_get_joints(self.h, keys.ctypes.data_as(U32), dp(acc), rows.ctypes.data_as(I32), n)
+        return keys, acc, rows
+
+    def joint_colours(self):
+        n = self.L.pko_get_joint_colours(self.h, None, 0)
+        out = np.zeros(n, dtype=np.uint32)
+        if n:
+            self.L.pko_get_joint_colours(self.h, out.ctypes.data_as(U32), n)
+        return out
 
     def colours(self, with_keys=False):
         n = self.L.pko_get_colours(self.h, None, None, 0)
