@@ -168,6 +168,56 @@ pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out);
  * the reference (its solver is sequential, constraint.h:1107-1201): a test hook. */
 pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, uint64_t *keys, size_t cap, size_t *n);
 
+/* ---- joint constraints (include/physkit/collision/constraint.h:139-834) ----
+ * Types are numbered as constraint_type (constraint.h:14-23). A joint is identified by
+ * its key (type << 24) | slot, slot being the index the caller's per-type arena gave
+ * it (arena.h:64-80; the reference keeps one arena per type, world.h:401,
+ * constraint.h:1018-1021). Each step the joint rows are built after the contact rows
+ * and, as in constraint_solver::solve_constraints (constraint.h:1107-1201), warm
+ * started and iterated before the contacts; joints are solved in FP64 whatever the
+ * flags. The device colours joints like manifolds (no two joints of a colour share a
+ * dynamic body) and sweeps the colours in order. */
+#define PKG_JOINT_DISTANCE 0
+#define PKG_JOINT_BALL_SOCKET 1
+#define PKG_JOINT_HINGE 2
+#define PKG_JOINT_SLIDER 3
+#define PKG_JOINT_WELD 4
+#define PKG_JOINT_SPRING 5
+#define PKG_JOINT_KEY(type, slot) ((((uint32_t)(type)) << 24) | ((uint32_t)(slot)))
+
+/* The fields of the six desc structs (constraint.h:144-175 distance, 239-285 spring,
+ * 357-374 ball_socket, 446-497 hinge, 585-609 slider, 737-755 weld); a type reads only
+ * its own fields. Vectors are xyz. */
+typedef struct pkg_joint_desc
+{
+    int32_t type;            /* PKG_JOINT_* */
+    uint32_t slot;           /* < 2^24 */
+    uint32_t a, b;           /* body ids, a != b */
+    double local_a[3];       /* distance, spring, hinge: anchor in body a's frame */
+    double local_b[3];       /* distance, spring, hinge: anchor in body b's frame */
+    double local_axis_a[3];  /* hinge: axis in a's frame */
+    double local_axis_b[3];  /* hinge: axis in b's frame */
+    double anchor[3];        /* ball_socket, slider, weld: world anchor */
+    double axis[3];          /* slider: world axis */
+    double distance;         /* distance, spring: rest length */
+    double stiffness;        /* spring */
+    double damping;          /* spring */
+} pkg_joint_desc;
+
+/* world::add_constraint (world.h:387-395) for n joints. As the reference's
+ * constructors do, ball_socket, slider and weld fix their frames from the bodies'
+ * current poses (the state after the last step or upload). PKG_INVALID_ARG if a key
+ * is already live or repeated, a type or slot is out of range, a body id is out of
+ * range, names a disabled slot, or a == b; or if the ctx runs batched worlds. */
+pkg_status pkg_add_joints(pkg_ctx *ctx, const pkg_joint_desc *descs, size_t n);
+
+/* world::remove_constraint (world.h:397-398). PKG_INVALID_ARG if a key is not live. */
+pkg_status pkg_remove_joints(pkg_ctx *ctx, const uint32_t *keys, size_t n);
+
+/* Live joints in key order and, optionally, their accumulated impulses (6 per joint,
+ * rows beyond the type's jacobian_dimension are 0). *n = number of live joints. */
+pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t cap, size_t *n);
+
 /* pkg_step with per-stage device time, measured with HIP events on the ctx stream.
  * Stages: 0 integrate_vel + leaf update, 1 LBVH build, 2 pair-set update, 3 GJK
  * kernel, 4 EPA kernel(s), 5 manifold update kernel, 6 contact solver,

@@ -172,6 +172,86 @@ private:
     uint32_t M_idx = 0xFFFFFFFFu, M_gen = 0;
 };
 
+// joint types, numbered as constraint_type (include/physkit/collision/constraint.h:14-23)
+enum class joint_type : int32_t
+{
+    distance = PKG_JOINT_DISTANCE,
+    ball_socket = PKG_JOINT_BALL_SOCKET,
+    hinge = PKG_JOINT_HINGE,
+    slider = PKG_JOINT_SLIDER,
+    weld = PKG_JOINT_WELD,
+    spring = PKG_JOINT_SPRING,
+};
+
+// The fields of the six desc structs (constraint.h:144-175 distance, 239-285 spring,
+// 357-374 ball_socket, 446-497 hinge, 585-609 slider, 737-755 weld); a type reads only
+// its own fields. Built as in the reference: <type>_constraint::desc::make(a, b).with_*.
+class joint_desc
+{
+public:
+    joint_desc(joint_type t, handle a, handle b) : M_type(t), M_a(a), M_b(b) {}
+    joint_desc &with_local_anchor_a(const vec3 &p) { M_local_a = p; return *this; }
+    joint_desc &with_local_anchor_b(const vec3 &p) { M_local_b = p; return *this; }
+    joint_desc &with_local_axis_a(const vec3 &v) { M_axis_a = v; return *this; }
+    joint_desc &with_local_axis_b(const vec3 &v) { M_axis_b = v; return *this; }
+    joint_desc &with_anchor(const vec3 &p) { M_anchor = p; return *this; }
+    joint_desc &with_axis(const vec3 &v) { M_axis = v; return *this; }
+    joint_desc &with_distance(double d) { M_distance = d; return *this; }
+    joint_desc &with_stiffness(double k) { M_stiffness = k; return *this; }
+    joint_desc &with_damping(double c) { M_damping = c; return *this; }
+
+    joint_type type() const { return M_type; }
+    handle a() const { return M_a; }
+    handle b() const { return M_b; }
+    pkg_joint_desc native(uint32_t slot) const
+    {
+        pkg_joint_desc d{};
+        d.type = static_cast<int32_t>(M_type);
+        d.slot = slot;
+        d.a = M_a.index();
+        d.b = M_b.index();
+        auto put = [](double *o, const vec3 &v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; };
+        put(d.local_a, M_local_a);
+        put(d.local_b, M_local_b);
+        put(d.local_axis_a, M_axis_a);
+        put(d.local_axis_b, M_axis_b);
+        put(d.anchor, M_anchor);
+        put(d.axis, M_axis);
+        d.distance = M_distance;
+        d.stiffness = M_stiffness;
+        d.damping = M_damping;
+        return d;
+    }
+
+private:
+    joint_type M_type;
+    handle M_a, M_b;
+    vec3 M_local_a, M_local_b, M_axis_a, M_axis_b, M_anchor, M_axis;
+    double M_distance = 0, M_stiffness = 0, M_damping = 0;
+};
+
+template <joint_type T> struct joint_of
+{
+    struct desc
+    {
+        static joint_desc make(handle a, handle b) { return joint_desc(T, a, b); }
+    };
+};
+using distance_constraint = joint_of<joint_type::distance>;
+using spring_constraint = joint_of<joint_type::spring>;
+using ball_socket_constraint = joint_of<joint_type::ball_socket>;
+using hinge_constraint = joint_of<joint_type::hinge>;
+using slider_constraint = joint_of<joint_type::slider>;
+using weld_constraint = joint_of<joint_type::weld>;
+
+// handle into the joint type's arena (arena<constraint>::handle, arena.h:34-61)
+struct joint_handle
+{
+    joint_type type;
+    uint32_t index, generation;
+    uint32_t key() const { return PKG_JOINT_KEY(static_cast<int32_t>(type), index); }
+};
+
 // snapshot of one body after the last step
 struct object_state
 {
@@ -249,6 +329,72 @@ public:
         return M_state[h.index()];
     }
 
+    // world::add_constraint (world.h:387-395). As the reference's add command, the joint
+    // is built at the next step, before step_impl, from the bodies' poses then.
+    joint_handle add_constraint(const joint_desc &d)
+    {
+        for (handle h : {d.a(), d.b()})
+            if (h.index() >= M_slots.size() || !M_slots[h.index()].live ||
+                M_slots[h.index()].gen != h.generation())
+                throw std::invalid_argument("add_constraint: stale body handle");
+        const size_t t = static_cast<size_t>(d.type());
+        uint32_t idx;
+        if (!M_jfree[t].empty())
+        {
+            idx = M_jfree[t].back();
+            M_jfree[t].pop_back();
+        }
+        else
+        {
+            idx = static_cast<uint32_t>(M_jgen[t].size());
+            M_jgen[t].push_back(0);
+            M_jlive[t].push_back(false);
+        }
+        M_jlive[t][idx] = true;
+        M_jadd.push_back(d.native(idx));
+        return {d.type(), idx, M_jgen[t][idx]};
+    }
+
+    // world::remove_constraint (world.h:397-398); false for a stale handle
+    bool remove_constraint(joint_handle h)
+    {
+        const size_t t = static_cast<size_t>(h.type);
+        if (h.index >= M_jgen[t].size() || !M_jlive[t][h.index] || M_jgen[t][h.index] != h.generation) return false;
+        M_jlive[t][h.index] = false;
+        M_jgen[t][h.index]++;
+        M_jfree[t].push_back(h.index);
+        const uint32_t key = h.key();
+        for (size_t i = 0; i < M_jadd.size(); ++i)
+            if (PKG_JOINT_KEY(M_jadd[i].type, M_jadd[i].slot) == key)
+            {
+                M_jadd.erase(M_jadd.begin() + static_cast<std::ptrdiff_t>(i));
+                return true;
+            }
+        M_jdel.push_back(key);
+        return true;
+    }
+
+    // accumulated impulses of a live joint (6 entries; rows beyond its dimension are 0)
+    std::optional<std::array<double, 6>> joint_impulses(joint_handle h)
+    {
+        const size_t t = static_cast<size_t>(h.type);
+        if (h.index >= M_jgen[t].size() || !M_jlive[t][h.index] || M_jgen[t][h.index] != h.generation)
+            return std::nullopt;
+        size_t n = 0;
+        check(pkg_get_joints(M_ctx, nullptr, nullptr, 0, &n), "pkg_get_joints");
+        std::vector<uint32_t> keys(n);
+        std::vector<double> imp(6 * n);
+        check(pkg_get_joints(M_ctx, keys.data(), imp.data(), n, &n), "pkg_get_joints");
+        for (size_t i = 0; i < n; ++i)
+            if (keys[i] == h.key())
+            {
+                std::array<double, 6> out{};
+                for (size_t r = 0; r < 6; ++r) out[r] = imp[6 * i + r];
+                return out;
+            }
+        return std::nullopt; // added, not yet built (no step since)
+    }
+
     // begin/end callbacks: what world_base::on_collision / on_collision_exit receive
     // (world.h:327-338), in pair-key order after each step
     std::function<void(handle, handle)> on_collision;
@@ -258,6 +404,7 @@ public:
     void step(double dt)
     {
         if (M_dirty) upload();
+        flush_joints();
         check(pkg_step(M_ctx, dt, 1), "pkg_step");
         M_state_valid = false;
         dispatch_events();
@@ -286,6 +433,25 @@ private:
     std::vector<object_state> M_state;
     bool M_dirty = true, M_state_valid = false, M_uploaded = false;
     size_t M_n_dev = 0; // bodies on the device (slot count at the last upload)
+    // one arena per joint type: generation and liveness per slot, LIFO free slots
+    std::array<std::vector<uint32_t>, 6> M_jgen, M_jfree;
+    std::array<std::vector<bool>, 6> M_jlive;
+    std::vector<pkg_joint_desc> M_jadd; // built at the next step
+    std::vector<uint32_t> M_jdel;       // removed at the next step
+
+    void flush_joints()
+    {
+        if (!M_jdel.empty())
+        {
+            check(pkg_remove_joints(M_ctx, M_jdel.data(), M_jdel.size()), "pkg_remove_joints");
+            M_jdel.clear();
+        }
+        if (!M_jadd.empty())
+        {
+            check(pkg_add_joints(M_ctx, M_jadd.data(), M_jadd.size()), "pkg_add_joints");
+            M_jadd.clear();
+        }
+    }
 
     void check(pkg_status st, const char *what) const
     {

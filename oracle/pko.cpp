@@ -1143,7 +1143,7 @@ bool world::add_joint(const joint_desc &d) // the constructors of constraint.h:1
 {
     if (d.type < 0 || d.type > j_spring || d.slot >= (1u << 24) || d.a >= bodies.size() || d.b >= bodies.size())
         return false;
-    if (bodies[d.a].disabled || bodies[d.b].disabled) return false;
+    if (d.a == d.b || bodies[d.a].disabled || bodies[d.b].disabled) return false;
     joint j;
     j.type = d.type;
     j.slot = d.slot;

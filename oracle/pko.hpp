@@ -256,7 +256,8 @@ struct world
     uint32_t add_body(const body &b); // world_base::create_rigid, world.h:202-208
     void step(double dt);             // world::step_impl, src/world.cpp:20-56
     // world::add_constraint (world.h:387-395): the constructor reads the bodies' current
-    // poses. False if the key is taken or an argument is out of range.
+    // poses. False if the key is taken, an argument is out of range, a body is a
+    // disabled slot, or both ends are the same body.
     bool add_joint(const joint_desc &d);
     bool remove_joint(uint32_t key); // world::remove_constraint, world.h:397-398
 

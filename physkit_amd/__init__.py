@@ -34,8 +34,19 @@ EXPORTS = [
     "pkg_set_body_state", "pkg_step", "pkg_step_timed", "pkg_download_bodies", "pkg_get_pairs",
     "pkg_get_manifolds", "pkg_get_events", "pkg_get_counters", "pkg_get_colours",
     "pkg_gjk_epa_batch", "pkg_scene_create", "pkg_scene_info_get", "pkg_scene_copy",
-    "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds",
+    "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds", "pkg_add_joints",
+    "pkg_remove_joints", "pkg_get_joints",
 ]
+
+# joint types, numbered as constraint_type (include/physkit/collision/constraint.h:14-23)
+JOINT_DISTANCE, JOINT_BALL_SOCKET, JOINT_HINGE, JOINT_SLIDER, JOINT_WELD, JOINT_SPRING = range(6)
+JOINT_DIMENSION = {JOINT_DISTANCE: 1, JOINT_BALL_SOCKET: 3, JOINT_HINGE: 5, JOINT_SLIDER: 5,
+                   JOINT_WELD: 6, JOINT_SPRING: 1}
+
+
+def joint_key(jtype: int, slot: int) -> int:
+    """(type << 24) | slot: the solve order of joints (type, then arena slot)."""
+    return (int(jtype) << 24) | int(slot)
 
 
 class PkgError(RuntimeError):
@@ -60,6 +71,25 @@ class pkg_body_state(C.Structure):
 class pkg_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("moved", "pairs", "gjk_hits", "contacts", "rows",
                                           "colours", "epa_overflow", "candidates")]
+
+
+class pkg_joint_desc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("slot", C.c_uint32), ("a", C.c_uint32), ("b", C.c_uint32),
+                ("local_a", C.c_double * 3), ("local_b", C.c_double * 3), ("local_axis_a", C.c_double * 3),
+                ("local_axis_b", C.c_double * 3), ("anchor", C.c_double * 3), ("axis", C.c_double * 3),
+                ("distance", C.c_double), ("stiffness", C.c_double), ("damping", C.c_double)]
+
+
+def _joint_struct(j: dict) -> pkg_joint_desc:
+    d = pkg_joint_desc()
+    d.type, d.slot, d.a, d.b = int(j["type"]), int(j.get("slot", 0)), int(j["a"]), int(j["b"])
+    for f in ("local_a", "local_b", "local_axis_a", "local_axis_b", "anchor", "axis"):
+        v = np.asarray(j.get(f, (0.0, 0.0, 0.0)), dtype=float).reshape(3)
+        getattr(d, f)[:] = [float(x) for x in v]
+    d.distance = float(j.get("distance", 0.0))
+    d.stiffness = float(j.get("stiffness", 0.0))
+    d.damping = float(j.get("damping", 0.0))
+    return d
 
 
 class pkg_scene_info(C.Structure):
@@ -103,6 +133,9 @@ def lib():
     L.pkg_scene_info_get.argtypes = [P, C.POINTER(pkg_scene_info)]
     L.pkg_scene_copy.argtypes = [P, U32, D, U8, U32, D, D, D, D, D, D, D, D]
     L.pkg_scene_destroy.argtypes = [P]
+    L.pkg_add_joints.argtypes = [P, C.POINTER(pkg_joint_desc), C.c_size_t]
+    L.pkg_remove_joints.argtypes = [P, U32, C.c_size_t]
+    L.pkg_get_joints.argtypes = [P, U32, D, C.c_size_t, C.POINTER(C.c_size_t)]
     for name in EXPORTS:
         if name != "pkg_last_error":
             getattr(L, name).restype = C.c_int
@@ -297,6 +330,26 @@ class GpuWorld:
         fb = np.ascontiguousarray(first_body, dtype=np.uint32)
         self._check(self.L.pkg_set_worlds(self.h, len(fb) - 1, fb.ctypes.data_as(U32)), "set_worlds")
 
+    def add_joints(self, joints):
+        """joints: dicts with type, slot, a, b and the desc fields (pkg_joint_desc)."""
+        arr = (pkg_joint_desc * max(len(joints), 1))(*[_joint_struct(j) for j in joints])
+        self._check(self.L.pkg_add_joints(self.h, arr, len(joints)), "add_joints")
+
+    def remove_joints(self, keys):
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        self._check(self.L.pkg_remove_joints(self.h, k.ctypes.data_as(U32), len(k)), "remove_joints")
+
+    def joints(self):
+        """(keys, impulses[n, 6]) of the live joints in key order."""
+        n = C.c_size_t()
+        self._check(self.L.pkg_get_joints(self.h, None, None, 0, C.byref(n)), "get_joints")
+        keys = np.zeros(n.value, np.uint32)
+        imp = np.zeros((n.value, 6))
+        if n.value:
+            self._check(self.L.pkg_get_joints(self.h, keys.ctypes.data_as(U32), _dp(imp), n.value, C.byref(n)),
+                        "get_joints")
+        return keys, imp
+
     def counters(self):
         c = pkg_counters()
         self._check(self.L.pkg_get_counters(self.h, C.byref(c)), "get_counters")
@@ -445,9 +498,69 @@ class object_desc:
     def with_friction(self, f): return self._set("friction", float(f))
 
 
+class _joint_desc:
+    """The desc structs of the six joint types (constraint.h:144-175, 239-285, 357-374,
+    446-497, 585-609, 737-755): make(a, b) then with_* setters."""
+    jtype = -1
+
+    def __init__(self, a, b):
+        self.a, self.b = int(a), int(b)
+        self.fields = {}
+
+    @classmethod
+    def make(cls, a, b):
+        return cls(a, b)
+
+    def _set(self, k, v):
+        self.fields[k] = v
+        return self
+
+    def as_dict(self, slot):
+        return {"type": self.jtype, "slot": slot, "a": self.a, "b": self.b, **self.fields}
+
+
+class distance_desc(_joint_desc):
+    jtype = JOINT_DISTANCE
+    def with_local_anchor_a(self, p): return self._set("local_a", tuple(p))
+    def with_local_anchor_b(self, p): return self._set("local_b", tuple(p))
+    def with_distance(self, d): return self._set("distance", float(d))
+
+
+class spring_desc(distance_desc):
+    jtype = JOINT_SPRING
+    def with_stiffness(self, k): return self._set("stiffness", float(k))
+    def with_damping(self, c): return self._set("damping", float(c))
+
+
+class ball_socket_desc(_joint_desc):
+    jtype = JOINT_BALL_SOCKET
+    def with_anchor(self, p): return self._set("anchor", tuple(p))
+
+
+class hinge_desc(_joint_desc):
+    jtype = JOINT_HINGE
+    def with_local_anchor_a(self, p): return self._set("local_a", tuple(p))
+    def with_local_anchor_b(self, p): return self._set("local_b", tuple(p))
+    def with_local_axis_a(self, v): return self._set("local_axis_a", tuple(v))
+    def with_local_axis_b(self, v): return self._set("local_axis_b", tuple(v))
+
+
+class slider_desc(_joint_desc):
+    jtype = JOINT_SLIDER
+    def with_anchor(self, p): return self._set("anchor", tuple(p))
+    def with_axis(self, v): return self._set("axis", tuple(v))
+
+
+class weld_desc(_joint_desc):
+    jtype = JOINT_WELD
+    def with_anchor(self, p): return self._set("anchor", tuple(p))
+
+
 class world:
     """physkit::world on the GPU: bodies are collected by create_rigid and uploaded
-    on the first step; handles are arena slot indices (body ids)."""
+    on the first step; handles are arena slot indices (body ids). Joints added with
+    add_constraint are built at the next step from the poses at that time, as the
+    reference runs add_constraint commands before step_impl (world.h:234-242)."""
 
     def __init__(self, desc: world_desc | None = None):
         self.desc = desc or world_desc()
@@ -456,6 +569,49 @@ class world:
         self._dirty = True
         self.on_collision = None       # callback(a, b) at collision begin
         self.on_collision_exit = None  # callback(a, b) at collision end
+        # per-type joint arenas (arena.h:64-80): generation per slot, LIFO free slots
+        self._jgen = {t: [] for t in range(6)}
+        self._jfree = {t: [] for t in range(6)}
+        self._jlive = set()
+        self._jadd = []     # (key, dict) waiting for the next step
+        self._jdel = []     # keys waiting for the next step
+
+    def add_constraint(self, d: _joint_desc):
+        """world::add_constraint (world.h:387-395): returns (type, slot, generation)."""
+        t = d.jtype
+        if self._jfree[t]:
+            slot = self._jfree[t].pop()
+        else:
+            slot = len(self._jgen[t])
+            self._jgen[t].append(0)
+        key = joint_key(t, slot)
+        self._jlive.add(key)
+        self._jadd.append((key, d.as_dict(slot)))
+        return (t, slot, self._jgen[t][slot])
+
+    def remove_constraint(self, h) -> bool:
+        """world::remove_constraint (world.h:397-398); False for a stale handle."""
+        t, slot, gen = h
+        key = joint_key(t, slot)
+        if key not in self._jlive or self._jgen[t][slot] != gen:
+            return False
+        self._jlive.discard(key)
+        self._jgen[t][slot] += 1
+        self._jfree[t].append(slot)
+        pending = [x for x in self._jadd if x[0] == key]
+        if pending:
+            self._jadd.remove(pending[0])
+        else:
+            self._jdel.append(key)
+        return True
+
+    def _flush_joints(self):
+        if self._jdel:
+            self._gpu.remove_joints(self._jdel)
+            self._jdel = []
+        if self._jadd:
+            self._gpu.add_joints([j for _, j in self._jadd])
+            self._jadd = []
 
     def create_rigid(self, d: object_desc) -> int:
         if d.mesh is None:
@@ -499,6 +655,7 @@ class world:
     def step(self, dt):
         if self._dirty:
             self._upload()
+        self._flush_joints()
         self._gpu.step(dt)
         if self.on_collision or self.on_collision_exit:
             keys, kinds = self._gpu.events()

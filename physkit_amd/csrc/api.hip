@@ -85,6 +85,35 @@ int id_bits(uint32_t n)
 }
 } // namespace
 
+namespace
+{
+template <class T> void upload(dbuf<T> &d, const T *h, size_t count, hipStream_t st)
+{
+    d.reserve(count);
+    if (count) hip_check(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+}
+// AoS (k components per element) -> planar
+std::vector<double> planar(const double *a, uint32_t n, int k)
+{
+    std::vector<double> o(static_cast<size_t>(n) * k);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < k; ++c) o[static_cast<size_t>(c) * n + i] = a[static_cast<size_t>(i) * k + c];
+    return o;
+}
+void aos(const std::vector<double> &p, uint32_t n, int k, double *out)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        for (int c = 0; c < k; ++c) out[static_cast<size_t>(i) * k + c] = p[static_cast<size_t>(c) * n + i];
+}
+template <class T> std::vector<T> download(const dbuf<T> &d, size_t count, hipStream_t st)
+{
+    std::vector<T> h(count);
+    if (count) hip_check(hipMemcpyAsync(h.data(), d.p, count * sizeof(T), hipMemcpyDeviceToHost, st), "download");
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    return h;
+}
+} // namespace
+
 struct pkg_ctx
 {
     hipStream_t st = nullptr;
@@ -149,8 +178,47 @@ struct pkg_ctx
     dbuf<unsigned long long> claim;
     uint32_t n_colours = 0;
 
+    // joints, dense in key order; an add or remove rebuilds the records into the other set
+    std::vector<uint32_t> jkey_h;         // live keys
+    std::vector<uint32_t> jab_h;          // bodies a, b of each live joint
+    int jset = 0;
+    uint32_t jcap[2] = {0, 0};
+    dbuf<double> jrec[2];
+    dbuf<uint8_t> jtype[2];
+    dbuf<uint32_t> ja[2], jb[2];
+    dbuf<uint64_t> jkeys[2];
+    dbuf<double> jrows;
+    dbuf<uint8_t> jnrows;
+    dbuf<uint32_t> jcol, jorder, jorder_alt, jcs;
+    dbuf<uint64_t> jck, jck_alt;
+    std::vector<uint32_t> jcs_h;          // joint colour starts
+    uint32_t *jord = nullptr;             // joints in colour order
+    bool jcoloured = false;               // colouring valid for the joint set and static flags
+    std::vector<uint8_t> h_static;        // is_static of the uploaded bodies
+
     pkg_counters last{};
     bool have_bodies = false;
+
+    joints_dev JD(int k)
+    {
+        joints_dev j;
+        j.n = k == jset ? static_cast<uint32_t>(jkey_h.size()) : 0;
+        j.cap = jcap[k];
+        j.rec = jrec[k].p;
+        j.type = jtype[k].p;
+        j.a = ja[k].p;
+        j.b = jb[k].p;
+        j.keys = jkeys[k].p;
+        j.rows = jrows.p;
+        j.nrows = jnrows.p;
+        return j;
+    }
+    // joint rows of this step and, when the joint set or the static flags changed, the
+    // joint colouring (same rounds as the manifolds, priority from the joint index)
+    void joint_prepare(const bodies_dev &b, double dt);
+    void joints_rebuild(const std::vector<uint32_t> &keys, const std::vector<uint32_t> &ab,
+                        const std::vector<uint32_t> &src, const std::vector<double> &desc,
+                        const std::vector<uint32_t> &dint);
 
     bodies_dev B()
     {
@@ -215,6 +283,80 @@ void record(hipEvent_t *ev, int k, hipStream_t st)
     if (ev) hip_check(hipEventRecord(ev[k], st), "event record");
 }
 } // namespace
+
+void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
+{
+    const uint32_t J = static_cast<uint32_t>(jkey_h.size());
+    jrows.reserve(static_cast<size_t>(6) * kJR * jcap[jset]);
+    jnrows.reserve(jcap[jset]);
+    joints_dev jd = JD(jset);
+    launch_joint_rows(jd, b, dt, st);
+    if (jcoloured) return;
+    jcol.reserve(J);
+    claim.reserve(std::max<uint32_t>(n, 1));
+    launch_fill_u32(jcol.p, kUncoloured, J, st);
+    uint32_t c = 0;
+    while (true)
+    {
+        launch_fill_u64(claim.p, ~0ull, n, st);
+        launch_colour_claim(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, st);
+        reset_counter(C_REMAIN);
+        launch_colour_assign(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, c, counters.p, st);
+        ++c;
+        if (read_counter(C_REMAIN) == 0) break;
+        if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
+    }
+    jorder.reserve(J);
+    jorder_alt.reserve(J);
+    jck.reserve(J);
+    jck_alt.reserve(J);
+    jcs.reserve(c + 1);
+    jord = launch_colour_bucket(jcol.p, J, c, jck.p, jck_alt.p, jorder.p, jorder_alt.p, jcs.p, sc, st);
+    jcs_h.assign(c + 1, 0u);
+    hip_check(hipMemcpyAsync(jcs_h.data(), jcs.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+              "joint colour copy");
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    jcoloured = true;
+}
+
+void pkg_ctx::joints_rebuild(const std::vector<uint32_t> &keys, const std::vector<uint32_t> &ab,
+                             const std::vector<uint32_t> &src, const std::vector<double> &desc,
+                             const std::vector<uint32_t> &dint)
+{
+    const int o = jset, k = 1 - jset;
+    const uint32_t J = static_cast<uint32_t>(keys.size());
+    const uint32_t cap = std::max<uint32_t>(64u, (J + 63u) & ~63u);
+    if (jcap[k] < cap || !jrec[k].p)
+    {
+        jrec[k].reserve(static_cast<size_t>(kJW) * cap);
+        jtype[k].reserve(cap);
+        ja[k].reserve(cap);
+        jb[k].reserve(cap);
+        jkeys[k].reserve(cap);
+        jcap[k] = cap;
+    }
+    if (J)
+    {
+        dbuf<uint32_t> dsrc, ddint;
+        dbuf<double> ddesc;
+        upload(dsrc, src.data(), src.size(), st);
+        if (!desc.empty())
+        {
+            upload(ddesc, desc.data(), desc.size(), st);
+            upload(ddint, dint.data(), dint.size(), st);
+        }
+        joints_dev oj = JD(o);
+        joints_dev nj = JD(k);
+        nj.n = J;
+        launch_joint_rebuild(oj, nj, dsrc.p, ddesc.p, ddint.p, B(), st);
+        hip_check(hipGetLastError(), "joint rebuild launch");
+        hip_check(hipStreamSynchronize(st), "joint rebuild");
+    }
+    jset = k;
+    jkey_h = keys;
+    jab_h = ab;
+    jcoloured = false;
+}
 
 void pkg_ctx::step_once(double dt, hipEvent_t *ev)
 {
@@ -384,103 +526,74 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     }
     record(ev, 6, st);
 
-    // ---- 4. contacts (constraint.h:1052-1201)
+    // ---- 4. contacts and joints (constraint.h:1052-1201)
     n_colours = 0;
-    if (solve && Pn)
+    const uint32_t n_joints = static_cast<uint32_t>(jkey_h.size());
+    if (solve && (Pn || n_joints))
     {
-        rows.reserve(static_cast<size_t>(4) * ncap * kRowF);
-        row_valid.reserve(static_cast<size_t>(4) * ncap);
-        colour.reserve(ncap);
-        claim.reserve(std::max<uint32_t>(n, 1));
-        rows_dev r;
-        r.cap4 = static_cast<uint32_t>(std::min(rows.cap / kRowF, row_valid.cap));
-        r.d = rows.p;
-        r.valid = row_valid.p;
-        double thr = 2.0 * gnorm * dt;
-        launch_contact_setup(keys[nxt].p, Pn, b, MF(nxt), r, dt, thr, colour.p, counters.p, st);
-        if (read_counter(C_ROWS) > 0)
+        const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
+        uint32_t c = 0; // manifold colours (0: no contact rows)
+        rows_dev r{};
+        if (Pn)
         {
-            const uint32_t *wo = nullptr, *pb = nullptr;
-            if (n_worlds > 1)
+            rows.reserve(static_cast<size_t>(4) * ncap * kRowF);
+            row_valid.reserve(static_cast<size_t>(4) * ncap);
+            colour.reserve(ncap);
+            claim.reserve(std::max<uint32_t>(n, 1));
+            r.cap4 = static_cast<uint32_t>(std::min(rows.cap / kRowF, row_valid.cap));
+            r.d = rows.p;
+            r.valid = row_valid.p;
+            double thr = 2.0 * gnorm * dt;
+            launch_contact_setup(keys[nxt].p, Pn, b, MF(nxt), r, dt, thr, colour.p, counters.p, st);
+            if (read_counter(C_ROWS) > 0)
             {
-                pair_base.reserve(n_worlds);
-                launch_pair_base(keys[nxt].p, Pn, world_of.p, pair_base.p, st);
-                wo = world_of.p;
-                pb = pair_base.p;
-            }
-            uint32_t c = 0;
-            while (true)
-            {
-                launch_fill_u64(claim.p, ~0ull, n, st);
-                launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
-                reset_counter(C_REMAIN);
-                launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, c, counters.p, st);
-                ++c;
-                if (read_counter(C_REMAIN) == 0) break;
-                if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
-            }
-            n_colours = c;
-            const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
-            bool solved = false;
-            if (c <= kIslandMaxColours)
-            {
-                // one workgroup per contact island when every island is small enough
-                par.reserve(std::max<uint32_t>(n, 1));
-                ik.reserve(Pn);
-                ik_alt.reserve(Pn);
-                iv.reserve(Pn);
-                iv_alt.reserve(Pn);
-                iflags.reserve(Pn);
-                ipos.reserve(Pn);
-                istarts.reserve(Pn);
-                bsums.reserve(scan_block_sums(Pn));
-                total.reserve(2);
-                reset_counter(C_ISL_MAX);
-                island_sort is = launch_island_sort(keys[nxt].p, colour.p, Pn, c, b, par.p, ik.p, iv.p, ik_alt.p,
-                                                    iv_alt.p, counters.p + C_ISL_MAX, sc, st);
-                launch_island_segments(is, Pn, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
-                uint32_t h2[2] = {0, 0};
-                hip_check(hipMemcpyAsync(h2, total.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "island count");
-                hip_check(hipStreamSynchronize(st), "stream sync");
-                const uint32_t n_isl = h2[0], n_act = h2[1];
-                launch_island_maxlen(istarts.p, n_isl, n_act, counters.p + C_ISL_MAX, st);
-                if (read_counter(C_ISL_MAX) <= kIslandMaxManifolds)
+                const uint32_t *wo = nullptr, *pb = nullptr;
+                if (n_worlds > 1)
                 {
-                    const uint32_t scap = std::max<uint32_t>(n_act, 1);
-                    rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
-                    row_valid_s.reserve(static_cast<size_t>(4) * scap);
-                    rows_dev rs;
-                    rs.cap4 = 4 * scap;
-                    rs.d = rows_s.p;
-                    rs.valid = row_valid_s.p;
-                    if (f32)
-                        launch_rows_gather32(is.ord, n_act, r, rs, scap, st);
-                    else
-                        launch_rows_gather(is.ord, n_act, r, rs, scap, st);
-                    launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
-                                         cfg.solver_iterations, f32, st);
-                    if (f32)
-                        launch_impulse_writeback32(is.ord, n_act, MF(nxt), rs, scap, st);
-                    else
-                        launch_impulse_writeback(is.ord, n_act, MF(nxt), rs, scap, st);
-                    solved = true;
+                    pair_base.reserve(n_worlds);
+                    launch_pair_base(keys[nxt].p, Pn, world_of.p, pair_base.p, st);
+                    wo = world_of.p;
+                    pb = pair_base.p;
+                }
+                while (true)
+                {
+                    launch_fill_u64(claim.p, ~0ull, n, st);
+                    launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
+                    reset_counter(C_REMAIN);
+                    launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, c, counters.p, st);
+                    ++c;
+                    if (read_counter(C_REMAIN) == 0) break;
+                    if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
             }
-            if (!solved)
+        }
+        n_colours = c;
+        if (n_joints) joint_prepare(b, dt);
+        bool solved = false;
+        if (c && !n_joints && c <= kIslandMaxColours)
+        {
+            // one workgroup per chunk of whole contact islands when every island is small enough
+            par.reserve(std::max<uint32_t>(n, 1));
+            ik.reserve(Pn);
+            ik_alt.reserve(Pn);
+            iv.reserve(Pn);
+            iv_alt.reserve(Pn);
+            iflags.reserve(Pn);
+            ipos.reserve(Pn);
+            istarts.reserve(Pn);
+            bsums.reserve(scan_block_sums(Pn));
+            total.reserve(2);
+            reset_counter(C_ISL_MAX);
+            island_sort is = launch_island_sort(keys[nxt].p, colour.p, Pn, c, b, par.p, ik.p, iv.p, ik_alt.p,
+                                                iv_alt.p, counters.p + C_ISL_MAX, sc, st);
+            launch_island_segments(is, Pn, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
+            uint32_t h2[2] = {0, 0};
+            hip_check(hipMemcpyAsync(h2, total.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "island count");
+            hip_check(hipStreamSynchronize(st), "stream sync");
+            const uint32_t n_isl = h2[0], n_act = h2[1];
+            launch_island_maxlen(istarts.p, n_isl, n_act, counters.p + C_ISL_MAX, st);
+            if (read_counter(C_ISL_MAX) <= kIslandMaxManifolds)
             {
-                colour_start.reserve(c + 1);
-                order.reserve(Pn);
-                order_alt.reserve(Pn);
-                ckeys.reserve(Pn);
-                ckeys_alt.reserve(Pn);
-                uint32_t *ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
-                                                     colour_start.p, sc, st);
-                std::vector<uint32_t> cs(c + 1);
-                hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
-                          "colour copy");
-                hip_check(hipStreamSynchronize(st), "stream sync");
-                // rows of the solver manifolds gathered into solve order
-                const uint32_t n_act = cs[c];
                 const uint32_t scap = std::max<uint32_t>(n_act, 1);
                 rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
                 row_valid_s.reserve(static_cast<size_t>(4) * scap);
@@ -489,26 +602,74 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 rs.d = rows_s.p;
                 rs.valid = row_valid_s.p;
                 if (f32)
-                {
-                    launch_rows_gather32(ord, n_act, r, rs, scap, st);
-                    for (uint32_t k = 0; k < c; ++k)
-                        launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
-                    for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
-                        for (uint32_t k = 0; k < c; ++k)
-                            launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
-                    launch_impulse_writeback32(ord, n_act, MF(nxt), rs, scap, st);
-                }
+                    launch_rows_gather32(is.ord, n_act, r, rs, scap, st);
                 else
-                {
+                    launch_rows_gather(is.ord, n_act, r, rs, scap, st);
+                launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
+                                     cfg.solver_iterations, f32, st);
+                if (f32)
+                    launch_impulse_writeback32(is.ord, n_act, MF(nxt), rs, scap, st);
+                else
+                    launch_impulse_writeback(is.ord, n_act, MF(nxt), rs, scap, st);
+                solved = true;
+            }
+        }
+        if (!solved)
+        {
+            // per-colour launches: each pass sweeps the joint colours, then the manifold
+            // colours (pass 0 warm starts, constraint.h:1123-1130; then the iterations)
+            uint32_t *ord = nullptr;
+            std::vector<uint32_t> cs;
+            uint32_t n_act = 0, scap = 1;
+            rows_dev rs{};
+            if (c)
+            {
+                colour_start.reserve(c + 1);
+                order.reserve(Pn);
+                order_alt.reserve(Pn);
+                ckeys.reserve(Pn);
+                ckeys_alt.reserve(Pn);
+                ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
+                                           colour_start.p, sc, st);
+                cs.assign(c + 1, 0u);
+                hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
+                          "colour copy");
+                hip_check(hipStreamSynchronize(st), "stream sync");
+                // rows of the solver manifolds gathered into solve order
+                n_act = cs[c];
+                scap = std::max<uint32_t>(n_act, 1);
+                rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);
+                row_valid_s.reserve(static_cast<size_t>(4) * scap);
+                rs.cap4 = 4 * scap;
+                rs.d = rows_s.p;
+                rs.valid = row_valid_s.p;
+                if (f32)
+                    launch_rows_gather32(ord, n_act, r, rs, scap, st);
+                else
                     launch_rows_gather(ord, n_act, r, rs, scap, st);
-                    for (uint32_t k = 0; k < c; ++k)
-                        launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, true, st);
-                    for (uint32_t it = 0; it < cfg.solver_iterations; ++it)
-                        for (uint32_t k = 0; k < c; ++k)
-                            launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, false, st);
-                    launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+            }
+            const joints_dev jd = JD(jset);
+            const uint32_t jc = n_joints ? static_cast<uint32_t>(jcs_h.size()) - 1 : 0;
+            for (uint32_t pass = 0; pass <= cfg.solver_iterations; ++pass)
+            {
+                const bool warm = pass == 0;
+                for (uint32_t k = 0; k < jc; ++k) launch_joint_solve(jd, jord, jcs_h[k], jcs_h[k + 1], b, warm, st);
+                for (uint32_t k = 0; k < c; ++k)
+                {
+                    if (f32)
+                        launch_solve_colour32(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, warm, st);
+                    else
+                        launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, warm, st);
                 }
             }
+            if (c)
+            {
+                if (f32)
+                    launch_impulse_writeback32(ord, n_act, MF(nxt), rs, scap, st);
+                else
+                    launch_impulse_writeback(ord, n_act, MF(nxt), rs, scap, st);
+            }
+            if (n_joints) launch_joint_writeback(jd, st);
         }
     }
     record(ev, 7, st);
@@ -550,34 +711,6 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         return PKG_DEVICE_ERROR;                                                                   \
     }
 
-namespace
-{
-template <class T> void upload(dbuf<T> &d, const T *h, size_t count, hipStream_t st)
-{
-    d.reserve(count);
-    if (count) hip_check(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
-}
-// AoS (k components per element) -> planar
-std::vector<double> planar(const double *a, uint32_t n, int k)
-{
-    std::vector<double> o(static_cast<size_t>(n) * k);
-    for (uint32_t i = 0; i < n; ++i)
-        for (int c = 0; c < k; ++c) o[static_cast<size_t>(c) * n + i] = a[static_cast<size_t>(i) * k + c];
-    return o;
-}
-void aos(const std::vector<double> &p, uint32_t n, int k, double *out)
-{
-    for (uint32_t i = 0; i < n; ++i)
-        for (int c = 0; c < k; ++c) out[static_cast<size_t>(i) * k + c] = p[static_cast<size_t>(c) * n + i];
-}
-template <class T> std::vector<T> download(const dbuf<T> &d, size_t count, hipStream_t st)
-{
-    std::vector<T> h(count);
-    if (count) hip_check(hipMemcpyAsync(h.data(), d.p, count * sizeof(T), hipMemcpyDeviceToHost, st), "download");
-    hip_check(hipStreamSynchronize(st), "stream sync");
-    return h;
-}
-} // namespace
 
 extern "C"
 {
@@ -662,7 +795,15 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
             ctx->err = "body references a shape that was not uploaded (object must have a mesh, object.h:121)";
             return PKG_INVALID_ARG;
         }
+    for (uint32_t x : ctx->jab_h)
+        if (x >= n || d->is_static[x] == 2)
+        {
+            ctx->err = "a live joint references a body that is out of range or a disabled slot";
+            return PKG_INVALID_ARG;
+        }
     PKG_GUARD(ctx, {
+        ctx->h_static.assign(d->is_static, d->is_static + n);
+        ctx->jcoloured = false;
         hipStream_t st = ctx->st;
         ctx->n = n;
         std::vector<double> mass(n);
@@ -718,6 +859,11 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
 pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first_body)
 {
     if (!ctx || !n_worlds || !first_body || !ctx->have_bodies) return PKG_INVALID_ARG;
+    if (n_worlds > 1 && !ctx->jkey_h.empty())
+    {
+        ctx->err = "joints are supported in single-world contexts only";
+        return PKG_INVALID_ARG;
+    }
     if (first_body[0] != 0 || first_body[n_worlds] != ctx->n)
     {
         ctx->err = "first_body must start at 0 and end at the body count";
@@ -740,6 +886,131 @@ pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first
         ctx->n_worlds = n_worlds;
         ctx->P[0] = ctx->P[1] = 0; // pairs restart under the new partition
         ctx->cur = 0;
+    })
+}
+
+pkg_status pkg_add_joints(pkg_ctx *ctx, const pkg_joint_desc *descs, size_t n)
+{
+    if (!ctx || (n && !descs)) return PKG_INVALID_ARG;
+    if (!n) return PKG_OK;
+    if (!ctx->have_bodies)
+    {
+        ctx->err = "joints need uploaded bodies";
+        return PKG_INVALID_ARG;
+    }
+    if (ctx->n_worlds > 1)
+    {
+        ctx->err = "joints are supported in single-world contexts only";
+        return PKG_INVALID_ARG;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> add; // (key, desc index)
+    for (size_t i = 0; i < n; ++i)
+    {
+        const pkg_joint_desc &d = descs[i];
+        if (d.type < PKG_JOINT_DISTANCE || d.type > PKG_JOINT_SPRING || d.slot >= (1u << 24))
+        {
+            ctx->err = "joint type or slot out of range";
+            return PKG_INVALID_ARG;
+        }
+        if (d.a >= ctx->n || d.b >= ctx->n || d.a == d.b || ctx->h_static[d.a] == 2 || ctx->h_static[d.b] == 2)
+        {
+            ctx->err = "joint bodies must be two different live bodies";
+            return PKG_INVALID_ARG;
+        }
+        add.emplace_back(PKG_JOINT_KEY(d.type, d.slot), static_cast<uint32_t>(i));
+    }
+    std::sort(add.begin(), add.end());
+    for (size_t i = 0; i < add.size(); ++i)
+        if ((i && add[i].first == add[i - 1].first) ||
+            std::binary_search(ctx->jkey_h.begin(), ctx->jkey_h.end(), add[i].first))
+        {
+            ctx->err = "joint key already live or repeated";
+            return PKG_INVALID_ARG;
+        }
+    PKG_GUARD(ctx, {
+        std::vector<uint32_t> keys, ab, src, dint;
+        std::vector<double> desc;
+        size_t io = 0, ia = 0;
+        const size_t no = ctx->jkey_h.size();
+        while (io < no || ia < add.size())
+        {
+            if (ia == add.size() || (io < no && ctx->jkey_h[io] < add[ia].first))
+            {
+                keys.push_back(ctx->jkey_h[io]);
+                ab.push_back(ctx->jab_h[2 * io]);
+                ab.push_back(ctx->jab_h[2 * io + 1]);
+                src.push_back(static_cast<uint32_t>(io));
+                ++io;
+            }
+            else
+            {
+                const pkg_joint_desc &d = descs[add[ia].second];
+                keys.push_back(add[ia].first);
+                ab.push_back(d.a);
+                ab.push_back(d.b);
+                src.push_back(0x80000000u | static_cast<uint32_t>(dint.size() / 3));
+                dint.push_back(static_cast<uint32_t>(d.type));
+                dint.push_back(d.a);
+                dint.push_back(d.b);
+                for (const double *v : {d.local_a, d.local_b, d.local_axis_a, d.local_axis_b, d.anchor, d.axis})
+                    desc.insert(desc.end(), v, v + 3);
+                desc.push_back(d.distance);
+                desc.push_back(d.stiffness);
+                desc.push_back(d.damping);
+                ++ia;
+            }
+        }
+        ctx->joints_rebuild(keys, ab, src, desc, dint);
+    })
+}
+
+pkg_status pkg_remove_joints(pkg_ctx *ctx, const uint32_t *keys_in, size_t n)
+{
+    if (!ctx || (n && !keys_in)) return PKG_INVALID_ARG;
+    if (!n) return PKG_OK;
+    std::vector<uint32_t> del(keys_in, keys_in + n);
+    std::sort(del.begin(), del.end());
+    for (size_t i = 0; i < del.size(); ++i)
+        if ((i && del[i] == del[i - 1]) || !std::binary_search(ctx->jkey_h.begin(), ctx->jkey_h.end(), del[i]))
+        {
+            ctx->err = "joint key not live (or repeated)";
+            return PKG_INVALID_ARG;
+        }
+    PKG_GUARD(ctx, {
+        std::vector<uint32_t> keys, ab, src;
+        for (size_t i = 0; i < ctx->jkey_h.size(); ++i)
+        {
+            if (std::binary_search(del.begin(), del.end(), ctx->jkey_h[i])) continue;
+            keys.push_back(ctx->jkey_h[i]);
+            ab.push_back(ctx->jab_h[2 * i]);
+            ab.push_back(ctx->jab_h[2 * i + 1]);
+            src.push_back(static_cast<uint32_t>(i));
+        }
+        ctx->joints_rebuild(keys, ab, src, {}, {});
+    })
+}
+
+pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t cap, size_t *n)
+{
+    if (!ctx || !n) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        const size_t J = ctx->jkey_h.size();
+        *n = J;
+        const size_t m = std::min(cap, J);
+        if (keys)
+            for (size_t i = 0; i < m; ++i) keys[i] = ctx->jkey_h[i];
+        if (impulses && m)
+        {
+            const int k = ctx->jset;
+            const size_t jc = ctx->jcap[k];
+            std::vector<double> acc(6 * jc);
+            hip_check(hipMemcpyAsync(acc.data(), ctx->jrec[k].p + 25 * jc, 6 * jc * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->st),
+                      "joint impulses");
+            hip_check(hipStreamSynchronize(ctx->st), "stream sync");
+            for (size_t i = 0; i < m; ++i)
+                for (size_t r = 0; r < 6; ++r) impulses[6 * i + r] = acc[r * jc + i];
+        }
     })
 }
 

@@ -15,7 +15,6 @@ namespace pk
 namespace
 {
 constexpr int kBlock = 256;
-constexpr uint32_t kUncoloured = 0xFFFFFFFEu;
 unsigned grid(size_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
 struct rowv

@@ -198,6 +198,7 @@ struct rows_dev
     uint8_t *valid;  // cap4
 };
 constexpr int kRowF = 38;
+constexpr uint32_t kUncoloured = 0xFFFFFFFEu; // colour of an item still to be coloured
 void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
                           manifolds_dev m, rows_dev r, double dt, double threshold,
                           uint32_t *colour, unsigned long long *counters, hipStream_t st);
@@ -256,5 +257,32 @@ void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t
                            const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r32, uint32_t cap,
                                 hipStream_t st);
+
+// ---------------- k_joint.hip ----------------
+// Joint records: kJW planes of `cap` doubles (word w of joint j at rec[w * cap + j]),
+// joints dense in key order. Words: la 0-2, lb 3-5, ax_a 6-8, ax_b 9-11, f1 12-14,
+// f2 15-17, qrel 18-21 (w x y z), distance 22, stiffness 23, damping 24, acc 25-30.
+constexpr int kJW = 31;
+constexpr int kJR = 13; // row words: Jv 0-2, Jwa 3-5, Jwb 6-8, Meff 9, bias 10, gamma 11, acc 12
+struct joints_dev
+{
+    uint32_t n, cap;
+    double *rec;
+    uint8_t *type;
+    uint32_t *a, *b;
+    uint64_t *keys;  // (a << 32) | b, the colouring input
+    double *rows;    // 6 * kJR planes of cap: word w of row r of joint j at rows[(r * kJR + w) * cap + j]
+    uint8_t *nrows;  // rows built this step
+};
+// Records after an add or remove: position i of the new list copies joint src[i] of
+// the old one, or, when src[i] has bit 31 set, constructs joint (src[i] & 0x7fffffff)
+// of `desc` (21 doubles each: local_a, local_b, local_axis_a, local_axis_b, anchor,
+// axis, distance, stiffness, damping) with `dint` (type, a, b) from the current poses.
+void launch_joint_rebuild(const joints_dev &old_j, const joints_dev &new_j, const uint32_t *src,
+                          const double *desc, const uint32_t *dint, const bodies_dev &b, hipStream_t st);
+void launch_joint_rows(const joints_dev &j, const bodies_dev &b, double dt, hipStream_t st);
+void launch_joint_solve(const joints_dev &j, const uint32_t *order, uint32_t begin, uint32_t end,
+                        const bodies_dev &b, bool warm, hipStream_t st);
+void launch_joint_writeback(const joints_dev &j, hipStream_t st);
 
 } // namespace pk

@@ -59,7 +59,36 @@ int main()
     REQUIRE(std::fabs(w.get_rigid(d)->pos().y - 1.9) < 1e-12);
     REQUIRE(std::fabs(w.get_rigid(c)->pos().y - (19.9 - 0.2)) < 1e-12);
 
+    // joints: one arena per type, LIFO slot reuse with generations; built at the next step
+    joint_handle j0 = w.add_constraint(ball_socket_constraint::desc::make(b, c).with_anchor({0, 5, 0}));
+    joint_handle j1 = w.add_constraint(ball_socket_constraint::desc::make(c, d).with_anchor({0, 5, 0}));
+    joint_handle j2 = w.add_constraint(distance_constraint::desc::make(b, d).with_distance(2.0));
+    REQUIRE(j0.index == 0 && j1.index == 1 && j2.index == 0);
+    REQUIRE(j0.key() == PKG_JOINT_KEY(PKG_JOINT_BALL_SOCKET, 0) && j2.key() == 0u);
+    REQUIRE(!w.joint_impulses(j0)); // not built before a step
+    w.step(0.1);
+    REQUIRE(w.joint_impulses(j0) && w.joint_impulses(j2));
+    REQUIRE(w.remove_constraint(j0));
+    REQUIRE(!w.remove_constraint(j0));
+    REQUIRE(!w.joint_impulses(j0));
+    joint_handle j3 = w.add_constraint(ball_socket_constraint::desc::make(b, d).with_anchor({1, 1, 1}));
+    REQUIRE(j3.index == 0 && j3.generation == 1); // the freed slot, next generation
+    joint_handle j4 = w.add_constraint(weld_constraint::desc::make(b, c).with_anchor({1, 1, 1}));
+    REQUIRE(w.remove_constraint(j4)); // removed before it was built: never reaches the device
+    w.step(0.1);
+    REQUIRE(w.joint_impulses(j3) && w.joint_impulses(j1) && !w.joint_impulses(j4));
+
     bool threw = false;
+    try
+    {
+        w.add_constraint(distance_constraint::desc::make(a, b)); // a was removed
+    }
+    catch (const std::invalid_argument &)
+    {
+        threw = true;
+    }
+    REQUIRE(threw);
+    threw = false;
     try
     {
         w.create_rigid(object_desc::dynam());

@@ -4,6 +4,7 @@
 // Linked only into tests/cpp/facade_logic; never part of the library.
 #include "physkit_amd.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,7 @@ struct pkg_ctx
     std::vector<uint8_t> st;
     std::vector<double> pos, vel, ang, quat;
     int uploads = 0;
+    std::vector<uint32_t> joints; // live joint keys, ascending
 };
 
 extern "C"
@@ -96,6 +98,39 @@ pkg_status pkg_get_manifolds(pkg_ctx *, int32_t *, double *, size_t, size_t *n)
 pkg_status pkg_get_events(pkg_ctx *, uint64_t *, int32_t *, size_t, size_t *n)
 {
     *n = 0;
+    return PKG_OK;
+}
+pkg_status pkg_add_joints(pkg_ctx *ctx, const pkg_joint_desc *d, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+    {
+        uint32_t k = PKG_JOINT_KEY(d[i].type, d[i].slot);
+        if (d[i].a >= ctx->n || d[i].b >= ctx->n || d[i].a == d[i].b ||
+            std::binary_search(ctx->joints.begin(), ctx->joints.end(), k))
+            return PKG_INVALID_ARG;
+        ctx->joints.insert(std::lower_bound(ctx->joints.begin(), ctx->joints.end(), k), k);
+    }
+    return PKG_OK;
+}
+pkg_status pkg_remove_joints(pkg_ctx *ctx, const uint32_t *keys, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+    {
+        auto it = std::lower_bound(ctx->joints.begin(), ctx->joints.end(), keys[i]);
+        if (it == ctx->joints.end() || *it != keys[i]) return PKG_INVALID_ARG;
+        ctx->joints.erase(it);
+    }
+    return PKG_OK;
+}
+pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t cap, size_t *n)
+{
+    *n = ctx->joints.size();
+    for (size_t i = 0; i < std::min(cap, ctx->joints.size()); ++i)
+    {
+        if (keys) keys[i] = ctx->joints[i];
+        if (impulses)
+            for (size_t r = 0; r < 6; ++r) impulses[6 * i + r] = 0.0;
+    }
     return PKG_OK;
 }
 pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out)

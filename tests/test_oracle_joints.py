@@ -455,6 +455,7 @@ def test_joint_keys_order_and_bookkeeping(oracle):
     assert not w.add_joint({"type": DISTANCE, "slot": 1, "a": 0, "b": 1})  # key taken
     assert not w.add_joint({"type": 6, "slot": 0, "a": 0, "b": 1})  # contact is not a joint type
     assert not w.add_joint({"type": BALL, "slot": 0, "a": 0, "b": 3})  # no body 3
+    assert not w.add_joint({"type": BALL, "slot": 0, "a": 1, "b": 1})  # one body at both ends
     keys, _, _ = w.joints()
     assert list(keys) == [1, 7, (WELD << 24) | 2]
     assert w.remove_joint(7) and not w.remove_joint(7)
