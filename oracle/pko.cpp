@@ -1195,6 +1195,54 @@ bool world::add_joint(const joint_desc &d) // the constructors of constraint.h:1
     return true;
 }
 
+std::optional<double> ray_box_distance(v3 o, v3 inv, const aabb &box, double max_dist) // bvh.h:58-91
+{
+    auto slab = [](double lo, double hi, double &tl, double &th)
+    {
+        if (std::isnan(lo)) lo = -INFINITY;
+        if (std::isnan(hi)) hi = INFINITY;
+        tl = std::fmin(lo, hi);
+        th = std::fmax(lo, hi);
+    };
+    double tminx, tmaxx, tminy, tmaxy, tminz, tmaxz;
+    slab((box.min.x - o.x) * inv.x, (box.max.x - o.x) * inv.x, tminx, tmaxx);
+    slab((box.min.y - o.y) * inv.y, (box.max.y - o.y) * inv.y, tminy, tmaxy);
+    slab((box.min.z - o.z) * inv.z, (box.max.z - o.z) * inv.z, tminz, tmaxz);
+    // std::max / std::min over {x, y, z}: the leftmost of equal values
+    double tmin = tminx;
+    if (tmin < tminy) tmin = tminy;
+    if (tmin < tminz) tmin = tminz;
+    double tmax = tmaxx;
+    if (tmaxy < tmax) tmax = tmaxy;
+    if (tmaxz < tmax) tmax = tmaxz;
+    if (tmax >= 0.0 && tmin <= tmax && tmin <= max_dist) return (0.0 < tmin) ? tmin : 0.0; // std::max(0.0, tmin)
+    return std::nullopt;
+}
+
+std::vector<std::pair<uint32_t, double>> world::raycast(v3 origin, v3 dir, double max_dist) const
+{
+    v3 d = normalized(dir); // ray::ray, bvh.h:46-49
+    v3 inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    std::vector<std::pair<uint32_t, double>> out;
+    for (std::size_t i = 0; i < bodies.size(); ++i)
+    {
+        if (bodies[i].disabled) continue;
+        if (auto t = ray_box_distance(origin, inv, bodies[i].fat, max_dist))
+            out.emplace_back(static_cast<uint32_t>(i), *t);
+    }
+    std::sort(out.begin(), out.end(), [](const auto &x, const auto &y)
+              { return x.second != y.second ? x.second < y.second : x.first < y.first; });
+    return out;
+}
+
+std::vector<uint32_t> world::query_aabb(const aabb &box) const
+{
+    std::vector<uint32_t> out;
+    for (std::size_t i = 0; i < bodies.size(); ++i)
+        if (!bodies[i].disabled && bodies[i].fat.intersects(box)) out.push_back(static_cast<uint32_t>(i));
+    return out;
+}
+
 bool world::remove_joint(uint32_t key)
 {
     auto it = std::lower_bound(joints.begin(), joints.end(), key,

@@ -261,6 +261,16 @@ struct world
     bool add_joint(const joint_desc &d);
     bool remove_joint(uint32_t key); // world::remove_constraint, world.h:397-398
 
+    // World queries over the broadphase leaves (every live body's tree bounds: the fat
+    // box of a dynamic body, the exact box a static body was added with). Brute force:
+    // the trees only prune, so their result sets are these (bvh.h:313-344, 406-450).
+    // raycast: (body, entry distance) of every leaf the ray meets within max_dist
+    // (world.h:259-319), sorted by (distance, body) -- the reference's order follows its
+    // trees' shapes and is not reproduced. query_aabb: bodies whose leaf box intersects
+    // the box (bounds.h:87-92), ascending.
+    std::vector<std::pair<uint32_t, double>> raycast(v3 origin, v3 dir, double max_dist) const;
+    std::vector<uint32_t> query_aabb(const aabb &box) const;
+
     // stages (exposed for stage tests)
     void stage_integrate_vel(double dt);
     void stage_pairs();
@@ -268,6 +278,10 @@ struct world
     void stage_solve(double dt);
     void stage_integrate_pos(double dt);
 };
+
+// ray::intersect_distance (bvh.h:58-91): slab test of a ray with normalized direction,
+// inverse direction 1/d (IEEE infinities for zero components); nullopt when missed
+std::optional<double> ray_box_distance(v3 origin, v3 inv_dir, const aabb &box, double max_dist);
 
 // exponential map of integrate.h:20-31
 qt exp_map(v3 w, double dt);

@@ -211,6 +211,39 @@ void pko_set_bodies(void *h, int count, const uint32_t *ids, const double *pos, 
     }
 }
 
+// world queries: hits of one ray (sorted by distance, then body) / one box (ascending);
+// returns the hit count, writes up to cap entries
+size_t pko_raycast(void *h, const double *origin, const double *dir, double max_dist, uint32_t *ids,
+                   double *dists, size_t cap)
+{
+    auto hits = static_cast<world *>(h)->raycast(ld3(origin), ld3(dir), max_dist);
+    for (size_t k = 0; k < std::min(cap, hits.size()); ++k)
+    {
+        if (ids) ids[k] = hits[k].first;
+        if (dists) dists[k] = hits[k].second;
+    }
+    return hits.size();
+}
+
+size_t pko_query_aabb(void *h, const double *box, uint32_t *ids, size_t cap)
+{
+    auto hits = static_cast<world *>(h)->query_aabb(aabb{ld3(box), ld3(box + 3)});
+    for (size_t k = 0; k < std::min(cap, hits.size()); ++k)
+        if (ids) ids[k] = hits[k];
+    return hits.size();
+}
+
+// ray::intersect_distance on one box; returns 0 when missed
+int pko_ray_box(const double *origin, const double *dir, const double *box, double max_dist, double *t)
+{
+    v3 d = normalized(ld3(dir));
+    auto r = ray_box_distance(ld3(origin), v3{1.0 / d.x, 1.0 / d.y, 1.0 / d.z}, aabb{ld3(box), ld3(box + 3)},
+                              max_dist);
+    if (!r) return 0;
+    *t = *r;
+    return 1;
+}
+
 // prm: local_a 3, local_b 3, local_axis_a 3, local_axis_b 3, anchor 3, axis 3,
 //      distance, stiffness, damping (21 doubles)
 int pko_add_joint(void *h, int type, uint32_t slot, uint32_t a, uint32_t b, const double *prm)

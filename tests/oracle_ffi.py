@@ -68,6 +68,12 @@ def lib():
         L.pko_exp_map.argtypes = [D, C.c_double, D]
         L.pko_inverse3.argtypes = [D, D]
         L.pko_set_bodies.argtypes = [C.c_void_p, C.c_int, U32, D, D, D, D]
+        L.pko_raycast.restype = C.c_size_t
+        L.pko_raycast.argtypes = [C.c_void_p, D, D, C.c_double, U32, D, C.c_size_t]
+        L.pko_query_aabb.restype = C.c_size_t
+        L.pko_query_aabb.argtypes = [C.c_void_p, D, U32, C.c_size_t]
+        L.pko_ray_box.restype = C.c_int
+        L.pko_ray_box.argtypes = [D, D, D, C.c_double, D]
         L.pko_add_joint.restype = C.c_int
         L.pko_add_joint.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, D]
         L.pko_remove_joint.restype = C.c_int
@@ -168,6 +174,13 @@ def exp_map(w, dt):
     q = np.zeros(4)
     lib().pko_exp_map(dp(f64(w)), dt, dp(q))
     return q
+
+
+def ray_box(origin, direction, mn, mx, max_dist=float("inf")):
+    """ray::intersect_distance (bvh.h:58-91): entry distance or None."""
+    t = np.zeros(1)
+    r = lib().pko_ray_box(dp(f64(origin)), dp(f64(direction)), dp(f64(list(mn) + list(mx))), float(max_dist), dp(t))
+    return float(t[0]) if r else None
 
 
 def inverse3(m):
@@ -273,6 +286,24 @@ class OracleWorld:
         arrs = [None if a is None else f64(a) for a in (pos, vel, angvel, quat)]
         self.L.pko_set_bodies(self.h, len(ids), ids.ctypes.data_as(U32),
                               *[None if a is None else dp(a) for a in arrs])
+
+    def raycast(self, origin, direction, max_dist=float("inf")):
+        """[(body, entry distance)] sorted by (distance, body)."""
+        o, d = f64(origin), f64(direction)
+        n = self.L.pko_raycast(self.h, dp(o), dp(d), float(max_dist), None, None, 0)
+        ids = np.zeros(n, np.uint32)
+        dist = np.zeros(n)
+        if n:
+            self.L.pko_raycast(self.h, dp(o), dp(d), float(max_dist), ids.ctypes.data_as(U32), dp(dist), n)
+        return ids, dist
+
+    def query_aabb(self, mn, mx):
+        b = f64(list(mn) + list(mx))
+        n = self.L.pko_query_aabb(self.h, dp(b), None, 0)
+        ids = np.zeros(n, np.uint32)
+        if n:
+            self.L.pko_query_aabb(self.h, dp(b), ids.ctypes.data_as(U32), n)
+        return ids
 
     def add_joint(self, j):
         """j: a joint dict (type, slot, a, b and the desc fields; see joint_params)."""
