@@ -218,6 +218,45 @@ pkg_status pkg_remove_joints(pkg_ctx *ctx, const uint32_t *keys, size_t n);
  * rows beyond the type's jacobian_dimension are 0). *n = number of live joints. */
 pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t cap, size_t *n);
 
+/* ---- world queries over the broadphase tree ----
+ * They see every live body's tree bounds: a dynamic body's fat box, the exact box a
+ * static body was added with (bvh.h:294-302; bvh.cpp:475-514), as they stand after the
+ * last step (between steps the tree is the last step's; after an upload it is built on
+ * first use). ray_world / box_world (optional, batched contexts): query i searches only
+ * world ray_world[i]; null searches every body. Arrays are host pointers unless the
+ * name ends in _device.
+ *
+ * Rays: origins and dirs xyz (dirs are normalized, as ray::ray does, bvh.h:46-49),
+ * max_dist per ray; a body is hit when ray::intersect_distance(bounds, max_dist)
+ * (bvh.h:58-91) returns a distance: the entry distance, 0 for an origin inside. */
+
+/* world_base::raycast (world.h:259-319) for n_rays rays: the hits of ray i are
+ * ids/dists[offsets[i] .. offsets[i+1]) sorted by (distance, body). The reference yields
+ * them in its trees' walk order, merged from the static and dynamic trees; that order
+ * depends on the trees' shapes and is not reproduced. offsets has n_rays + 1 entries;
+ * *total = all hits. If cap < *total only offsets and *total are written. */
+pkg_status pkg_raycast(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                       const double *max_dist, const uint32_t *ray_world, uint32_t *offsets, uint32_t *ids,
+                       double *dists, size_t cap, size_t *total);
+
+/* The first hit of each ray in (distance, body) order: the first element of the sorted
+ * hit list (dynamic_bvh::raycast with a callback that returns the hit distance,
+ * bvh.h:355-404, prunes the same way). ids[i] = 0xFFFFFFFF and dists[i] = +inf when
+ * ray i hits nothing. */
+pkg_status pkg_raycast_closest(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                               const double *max_dist, const uint32_t *ray_world, uint32_t *ids, double *dists);
+/* The same with every array in device memory (the ctx's device); runs on the ctx's
+ * stream and returns when the results are written. */
+pkg_status pkg_raycast_closest_device(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                                      const double *max_dist, const uint32_t *ray_world, uint32_t *ids,
+                                      double *dists);
+
+/* dynamic_bvh::query_aabb (bvh.h:313-344) over both trees for n_boxes boxes (min xyz,
+ * max xyz): bodies whose bounds intersect box i (aabb::intersects, bounds.h:87-92) are
+ * ids[offsets[i] .. offsets[i+1]), ascending. Same cap protocol as pkg_raycast. */
+pkg_status pkg_query_aabb(pkg_ctx *ctx, uint32_t n_boxes, const double *boxes, const uint32_t *box_world,
+                          uint32_t *offsets, uint32_t *ids, size_t cap, size_t *total);
+
 /* pkg_step with per-stage device time, measured with HIP events on the ctx stream.
  * Stages: 0 integrate_vel + leaf update, 1 LBVH build, 2 pair-set update, 3 GJK
  * kernel, 4 EPA kernel(s), 5 manifold update kernel, 6 contact solver,

@@ -19,10 +19,12 @@
 #include <cmath>
 #include <cstdint>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace physkit_amd
@@ -393,6 +395,41 @@ public:
                 return out;
             }
         return std::nullopt; // added, not yet built (no step since)
+    }
+
+    // world_base::raycast (world.h:259-319): every body whose tree bounds the ray meets
+    // within max_dist, with the entry distance, sorted by (distance, body)
+    std::vector<std::pair<handle, double>> raycast(const vec3 &origin, const vec3 &dir,
+                                                   double max_dist = std::numeric_limits<double>::max())
+    {
+        if (M_dirty) upload();
+        const double o[3] = {origin.x, origin.y, origin.z}, d[3] = {dir.x, dir.y, dir.z};
+        uint32_t off[2] = {0, 0};
+        size_t total = 0;
+        check(pkg_raycast(M_ctx, 1, o, d, &max_dist, nullptr, off, nullptr, nullptr, 0, &total), "pkg_raycast");
+        std::vector<uint32_t> ids(total);
+        std::vector<double> dist(total);
+        if (total)
+            check(pkg_raycast(M_ctx, 1, o, d, &max_dist, nullptr, off, ids.data(), dist.data(), total, &total),
+                  "pkg_raycast");
+        std::vector<std::pair<handle, double>> out;
+        for (size_t i = 0; i < total; ++i) out.emplace_back(handle{ids[i], M_slots[ids[i]].gen}, dist[i]);
+        return out;
+    }
+
+    // bodies whose tree bounds intersect the box (dynamic_bvh::query_aabb, bvh.h:313-344)
+    std::vector<handle> query_aabb(const vec3 &mn, const vec3 &mx)
+    {
+        if (M_dirty) upload();
+        const double box[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+        uint32_t off[2] = {0, 0};
+        size_t total = 0;
+        check(pkg_query_aabb(M_ctx, 1, box, nullptr, off, nullptr, 0, &total), "pkg_query_aabb");
+        std::vector<uint32_t> ids(total);
+        if (total) check(pkg_query_aabb(M_ctx, 1, box, nullptr, off, ids.data(), total, &total), "pkg_query_aabb");
+        std::vector<handle> out;
+        for (uint32_t i : ids) out.emplace_back(i, M_slots[i].gen);
+        return out;
     }
 
     // begin/end callbacks: what world_base::on_collision / on_collision_exit receive

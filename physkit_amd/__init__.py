@@ -35,7 +35,8 @@ EXPORTS = [
     "pkg_get_manifolds", "pkg_get_events", "pkg_get_counters", "pkg_get_colours",
     "pkg_gjk_epa_batch", "pkg_scene_create", "pkg_scene_info_get", "pkg_scene_copy",
     "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds", "pkg_add_joints",
-    "pkg_remove_joints", "pkg_get_joints",
+    "pkg_remove_joints", "pkg_get_joints", "pkg_raycast", "pkg_raycast_closest", "pkg_raycast_closest_device",
+    "pkg_query_aabb",
 ]
 
 # joint types, numbered as constraint_type (include/physkit/collision/constraint.h:14-23)
@@ -133,6 +134,11 @@ def lib():
     L.pkg_scene_info_get.argtypes = [P, C.POINTER(pkg_scene_info)]
     L.pkg_scene_copy.argtypes = [P, U32, D, U8, U32, D, D, D, D, D, D, D, D]
     L.pkg_scene_destroy.argtypes = [P]
+    L.pkg_raycast.argtypes = [P, C.c_uint32, D, D, D, U32, U32, U32, D, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.pkg_raycast_closest.argtypes = [P, C.c_uint32, D, D, D, U32, U32, D]
+    L.pkg_raycast_closest_device.argtypes = [P, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
+    L.pkg_query_aabb.argtypes = [P, C.c_uint32, D, U32, U32, U32, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_add_joints.argtypes = [P, C.POINTER(pkg_joint_desc), C.c_size_t]
     L.pkg_remove_joints.argtypes = [P, U32, C.c_size_t]
     L.pkg_get_joints.argtypes = [P, U32, D, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -329,6 +335,62 @@ class GpuWorld:
         """Batched independent worlds: bodies [first_body[w], first_body[w+1]) form world w."""
         fb = np.ascontiguousarray(first_body, dtype=np.uint32)
         self._check(self.L.pkg_set_worlds(self.h, len(fb) - 1, fb.ctypes.data_as(U32)), "set_worlds")
+
+    @staticmethod
+    def _rays(origins, dirs, max_dist):
+        o = _f64(np.asarray(origins, dtype=float).reshape(-1, 3))
+        d = _f64(np.asarray(dirs, dtype=float).reshape(-1, 3))
+        n = len(o)
+        m = _f64(np.broadcast_to(np.asarray(max_dist, dtype=float), (n,)))
+        return n, o, d, m
+
+    @staticmethod
+    def _worlds(world, n):
+        if world is None:
+            return None, None
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(world, dtype=np.uint32), (n,)))
+        return w, w.ctypes.data_as(U32)
+
+    def raycast(self, origins, dirs, max_dist=np.inf, world=None):
+        """world_base::raycast for a batch of rays: (offsets[n+1], ids, dists), the hits
+        of ray i at [offsets[i], offsets[i+1]) sorted by (distance, body)."""
+        n, o, d, m = self._rays(origins, dirs, max_dist)
+        w, wp = self._worlds(world, n)
+        off = np.zeros(n + 1, np.uint32)
+        tot = C.c_size_t()
+        self._check(self.L.pkg_raycast(self.h, n, _dp(o), _dp(d), _dp(m), wp, off.ctypes.data_as(U32), None, None,
+                                       0, C.byref(tot)), "raycast")
+        ids = np.zeros(tot.value, np.uint32)
+        dist = np.zeros(tot.value)
+        if tot.value:
+            self._check(self.L.pkg_raycast(self.h, n, _dp(o), _dp(d), _dp(m), wp, off.ctypes.data_as(U32),
+                                           ids.ctypes.data_as(U32), _dp(dist), tot.value, C.byref(tot)), "raycast")
+        return off, ids, dist
+
+    def raycast_closest(self, origins, dirs, max_dist=np.inf, world=None):
+        """First hit per ray in (distance, body) order: (ids, dists); 0xFFFFFFFF / inf on a miss."""
+        n, o, d, m = self._rays(origins, dirs, max_dist)
+        w, wp = self._worlds(world, n)
+        ids = np.zeros(n, np.uint32)
+        dist = np.zeros(n)
+        self._check(self.L.pkg_raycast_closest(self.h, n, _dp(o), _dp(d), _dp(m), wp, ids.ctypes.data_as(U32),
+                                               _dp(dist)), "raycast_closest")
+        return ids, dist
+
+    def query_aabb(self, boxes, world=None):
+        """query_aabb for a batch of boxes (min xyz, max xyz): (offsets[n+1], ids)."""
+        b = _f64(np.asarray(boxes, dtype=float).reshape(-1, 6))
+        n = len(b)
+        w, wp = self._worlds(world, n)
+        off = np.zeros(n + 1, np.uint32)
+        tot = C.c_size_t()
+        self._check(self.L.pkg_query_aabb(self.h, n, _dp(b), wp, off.ctypes.data_as(U32), None, 0, C.byref(tot)),
+                    "query_aabb")
+        ids = np.zeros(tot.value, np.uint32)
+        if tot.value:
+            self._check(self.L.pkg_query_aabb(self.h, n, _dp(b), wp, off.ctypes.data_as(U32),
+                                              ids.ctypes.data_as(U32), tot.value, C.byref(tot)), "query_aabb")
+        return off, ids
 
     def add_joints(self, joints):
         """joints: dicts with type, slot, a, b and the desc fields (pkg_joint_desc)."""
@@ -670,6 +732,20 @@ class world:
         if self._dirty:
             self._upload()
         return self._gpu.bodies()
+
+    def raycast(self, origin, direction, max_dist=np.inf):
+        """world_base::raycast (world.h:259-319): [(handle, distance)] sorted by distance."""
+        if self._dirty:
+            self._upload()
+        off, ids, dist = self._gpu.raycast([origin], [direction], max_dist)
+        return list(zip(ids.tolist(), dist.tolist()))
+
+    def query_aabb(self, mn, mx):
+        """Handles whose tree bounds intersect the box, ascending."""
+        if self._dirty:
+            self._upload()
+        off, ids = self._gpu.query_aabb([list(mn) + list(mx)])
+        return ids.tolist()
 
     @property
     def gpu(self) -> GpuWorld:

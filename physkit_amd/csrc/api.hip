@@ -196,8 +196,32 @@ struct pkg_ctx
     bool jcoloured = false;               // colouring valid for the joint set and static flags
     std::vector<uint8_t> h_static;        // is_static of the uploaded bodies
 
+    // world queries
+    bool tree_ok = false; // the LBVH holds the current tree bounds (built by the last step or query)
+    dbuf<uint32_t> q_cnt, q_off, q_hid, q_hq, q_perm, q_perm_alt, q_bsums, q_total, q_u32;
+    dbuf<double> q_hd, q_org, q_dir, q_maxd, q_box, q_out_d;
+    dbuf<uint64_t> q_keys, q_keys_alt;
+    dbuf<uint32_t> q_out_id, q_world;
+
     pkg_counters last{};
     bool have_bodies = false;
+
+    lbvh_dev tree();
+    lbvh_dev build_tree(const bodies_dev &b, bool all_moved);
+    // the LBVH of the current tree bounds: the last step's, or built now
+    lbvh_dev query_tree()
+    {
+        if (!tree_ok)
+        {
+            bodies_dev b = B();
+            reset_counter(C_TREE_FAIL);
+            return build_tree(b, false);
+        }
+        return tree();
+    }
+    // offsets of per-query counts (exclusive scan) and the total
+    uint32_t scan_counts(uint32_t nq);
+    query_scratch qscratch(uint32_t m);
 
     joints_dev JD(int k)
     {
@@ -284,6 +308,49 @@ void record(hipEvent_t *ev, int k, hipStream_t st)
 }
 } // namespace
 
+lbvh_dev pkg_ctx::tree()
+{
+    lbvh_dev t;
+    t.n = n;
+    t.morton = morton.p;
+    t.morton_alt = morton_alt.p;
+    t.ids = ids.p;
+    t.ids_alt = ids_alt.p;
+    t.left = left.p;
+    t.right = right.p;
+    t.first = range_first.p;
+    t.last = range_last.p;
+    t.lastf = lastf.p;
+    t.inode_lo = inode_lo.p;
+    t.inode_hi = inode_hi.p;
+    t.parent = parent.p;
+    t.escape = escape.p;
+    t.leaf_box = leaf_box.p;
+    t.leaf_body = leaf_body.p;
+    t.leaf_moved = leaf_moved.p;
+    t.visit = visit.p;
+    t.defer = defer.p;
+    t.scene = scene.p;
+    t.n_worlds = n_worlds;
+    t.wbits = id_bits(n_worlds);
+    t.world_of = n_worlds > 1 ? world_of.p : nullptr;
+    t.wstart = n_worlds > 1 ? wstart.p : nullptr;
+    return t;
+}
+
+lbvh_dev pkg_ctx::build_tree(const bodies_dev &b, bool all_moved)
+{
+    lbvh_dev t = tree();
+    launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
+    if (t.morton != morton.p)
+    {
+        std::swap(morton.p, morton_alt.p);
+        std::swap(ids.p, ids_alt.p);
+    }
+    tree_ok = true;
+    return t;
+}
+
 void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
 {
     const uint32_t J = static_cast<uint32_t>(jkey_h.size());
@@ -358,6 +425,40 @@ void pkg_ctx::joints_rebuild(const std::vector<uint32_t> &keys, const std::vecto
     jcoloured = false;
 }
 
+uint32_t pkg_ctx::scan_counts(uint32_t nq)
+{
+    q_off.reserve(static_cast<size_t>(nq) + 1);
+    q_bsums.reserve(scan_block_sums(nq));
+    q_total.reserve(1);
+    exclusive_scan_u32(q_cnt.p, q_off.p, nq, q_bsums.p, q_total.p, st);
+    uint32_t total = 0;
+    hip_check(hipMemcpyAsync(q_off.p + nq, q_total.p, sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "offset copy");
+    hip_check(hipMemcpyAsync(&total, q_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "hit total");
+    hip_check(hipStreamSynchronize(st), "stream sync");
+    return total;
+}
+
+query_scratch pkg_ctx::qscratch(uint32_t m)
+{
+    const size_t c = std::max<uint32_t>(m, 1);
+    q_hid.reserve(c);
+    q_hq.reserve(c);
+    q_perm.reserve(c);
+    q_perm_alt.reserve(c);
+    q_hd.reserve(c);
+    q_keys.reserve(c);
+    q_keys_alt.reserve(c);
+    query_scratch qs;
+    qs.hid = q_hid.p;
+    qs.hq = q_hq.p;
+    qs.perm = q_perm.p;
+    qs.perm_alt = q_perm_alt.p;
+    qs.hd = q_hd.p;
+    qs.keys = q_keys.p;
+    qs.keys_alt = q_keys_alt.p;
+    return qs;
+}
+
 void pkg_ctx::step_once(double dt, hipEvent_t *ev)
 {
     const bool integrate = (cfg.flags & PKG_FLAG_NO_INTEGRATE) == 0;
@@ -376,38 +477,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     record(ev, 1, st);
 
     // ---- 2. broadphase (collision_phases.h:377-436)
-    lbvh_dev t;
-    t.n = n;
-    t.morton = morton.p;
-    t.morton_alt = morton_alt.p;
-    t.ids = ids.p;
-    t.ids_alt = ids_alt.p;
-    t.left = left.p;
-    t.right = right.p;
-    t.first = range_first.p;
-    t.last = range_last.p;
-    t.lastf = lastf.p;
-    t.inode_lo = inode_lo.p;
-    t.inode_hi = inode_hi.p;
-    t.parent = parent.p;
-    t.escape = escape.p;
-    t.leaf_box = leaf_box.p;
-    t.leaf_body = leaf_body.p;
-    t.leaf_moved = leaf_moved.p;
-    t.visit = visit.p;
-    t.defer = defer.p;
-    t.scene = scene.p;
-    t.n_worlds = n_worlds;
-    t.wbits = id_bits(n_worlds);
-    t.world_of = n_worlds > 1 ? world_of.p : nullptr;
-    t.wstart = n_worlds > 1 ? wstart.p : nullptr;
-    launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
+    lbvh_dev t = build_tree(b, all_moved);
     record(ev, 2, st);
-    if (t.morton != morton.p)
-    {
-        std::swap(morton.p, morton_alt.p);
-        std::swap(ids.p, ids_alt.p);
-    }
     const uint32_t Pprev = P[cur];
     const int nxt = 1 - cur;
     // candidates: pairs with at least one moved leaf, each once
@@ -852,6 +923,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->cur = 0;
         ctx->n_worlds = 1;
         ctx->have_bodies = true;
+        ctx->tree_ok = false;
         hip_check(hipStreamSynchronize(st), "body upload");
     })
 }
@@ -884,6 +956,7 @@ pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first
         upload(ctx->wstart, first_body, static_cast<size_t>(n_worlds) + 1, st);
         hip_check(hipStreamSynchronize(st), "world upload");
         ctx->n_worlds = n_worlds;
+        ctx->tree_ok = false;
         ctx->P[0] = ctx->P[1] = 0; // pairs restart under the new partition
         ctx->cur = 0;
     })
@@ -1011,6 +1084,164 @@ pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t
             for (size_t i = 0; i < m; ++i)
                 for (size_t r = 0; r < 6; ++r) impulses[6 * i + r] = acc[r * jc + i];
         }
+    })
+}
+
+namespace
+{
+pkg_status check_query_worlds(pkg_ctx *ctx, uint32_t nq, const uint32_t *w)
+{
+    if (w)
+        for (uint32_t i = 0; i < nq; ++i)
+            if (w[i] >= ctx->n_worlds)
+            {
+                ctx->err = "query world out of range";
+                return PKG_INVALID_ARG;
+            }
+    return PKG_OK;
+}
+
+// counts of a query batch summed in 64 bits when 32-bit offsets could overflow
+void check_hit_total(pkg_ctx *ctx, uint32_t nq)
+{
+    if (static_cast<uint64_t>(nq) * ctx->n <= 0xFFFFFFFFull) return;
+    std::vector<uint32_t> c = download(ctx->q_cnt, nq, ctx->st);
+    uint64_t sum = 0;
+    for (uint32_t x : c) sum += x;
+    if (sum > 0xFFFFFFFFull) throw pk_error(PKG_OVERFLOW_PAIRS, "query hit count exceeds 32-bit indexing");
+}
+
+void check_tree_walk(pkg_ctx *ctx)
+{
+    if (ctx->read_counter(C_TREE_FAIL)) throw pk_error(PKG_DEVICE_ERROR, "query tree walk exceeded its node count");
+}
+} // namespace
+
+pkg_status pkg_raycast(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                       const double *max_dist, const uint32_t *ray_world, uint32_t *offsets, uint32_t *ids,
+                       double *dists, size_t cap, size_t *total)
+{
+    if (!ctx || !total || !offsets || (n_rays && (!origins || !dirs || !max_dist))) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    if (pkg_status e = check_query_worlds(ctx, n_rays, ray_world)) return e;
+    PKG_GUARD(ctx, {
+        hipStream_t st = ctx->st;
+        *total = 0;
+        offsets[0] = 0;
+        if (!n_rays) return PKG_OK;
+        upload(ctx->q_org, origins, 3 * static_cast<size_t>(n_rays), st);
+        upload(ctx->q_dir, dirs, 3 * static_cast<size_t>(n_rays), st);
+        upload(ctx->q_maxd, max_dist, n_rays, st);
+        if (ray_world) upload(ctx->q_world, ray_world, n_rays, st);
+        const uint32_t *rw = ray_world ? ctx->q_world.p : nullptr;
+        lbvh_dev t = ctx->query_tree();
+        bodies_dev b = ctx->B();
+        unsigned long long *fail = ctx->counters.p + C_TREE_FAIL;
+        ctx->reset_counter(C_TREE_FAIL);
+        ctx->q_cnt.reserve(n_rays);
+        launch_ray_count(t, b, n_rays, ctx->q_org.p, ctx->q_dir.p, ctx->q_maxd.p, rw, ctx->q_cnt.p, fail, st);
+        check_hit_total(ctx, n_rays);
+        const uint32_t m = ctx->scan_counts(n_rays);
+        check_tree_walk(ctx);
+        hip_check(hipMemcpyAsync(offsets, ctx->q_off.p, (static_cast<size_t>(n_rays) + 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, st),
+                  "offsets");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        *total = m;
+        if (!m || cap < m) return PKG_OK;
+        if (!ids || !dists) throw pk_error(PKG_INVALID_ARG, "ids and dists are needed when cap >= total");
+        query_scratch qs = ctx->qscratch(m);
+        launch_ray_fill(t, b, n_rays, ctx->q_org.p, ctx->q_dir.p, ctx->q_maxd.p, rw, ctx->q_off.p, qs.hid, qs.hd,
+                        qs.hq, fail, st);
+        ctx->q_out_id.reserve(m);
+        ctx->q_out_d.reserve(m);
+        launch_hits_sort(qs, m, ctx->n, n_rays, true, ctx->q_out_id.p, ctx->q_out_d.p, ctx->sc, st);
+        hip_check(hipMemcpyAsync(ids, ctx->q_out_id.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "ids");
+        hip_check(hipMemcpyAsync(dists, ctx->q_out_d.p, m * sizeof(double), hipMemcpyDeviceToHost, st), "dists");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        check_tree_walk(ctx);
+    })
+}
+
+pkg_status pkg_raycast_closest_device(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                                      const double *max_dist, const uint32_t *ray_world, uint32_t *ids,
+                                      double *dists)
+{
+    if (!ctx || (n_rays && (!origins || !dirs || !max_dist || !ids || !dists))) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    PKG_GUARD(ctx, {
+        if (!n_rays) return PKG_OK;
+        lbvh_dev t = ctx->query_tree();
+        ctx->reset_counter(C_TREE_FAIL);
+        launch_ray_closest(t, ctx->B(), n_rays, origins, dirs, max_dist, ray_world, ids, dists,
+                           ctx->counters.p + C_TREE_FAIL, ctx->st);
+        hip_check(hipGetLastError(), "raycast launch");
+        check_tree_walk(ctx);
+    })
+}
+
+pkg_status pkg_raycast_closest(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
+                               const double *max_dist, const uint32_t *ray_world, uint32_t *ids, double *dists)
+{
+    if (!ctx || (n_rays && (!origins || !dirs || !max_dist || !ids || !dists))) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    if (pkg_status e = check_query_worlds(ctx, n_rays, ray_world)) return e;
+    PKG_GUARD(ctx, {
+        if (!n_rays) return PKG_OK;
+        hipStream_t st = ctx->st;
+        upload(ctx->q_org, origins, 3 * static_cast<size_t>(n_rays), st);
+        upload(ctx->q_dir, dirs, 3 * static_cast<size_t>(n_rays), st);
+        upload(ctx->q_maxd, max_dist, n_rays, st);
+        if (ray_world) upload(ctx->q_world, ray_world, n_rays, st);
+        ctx->q_out_id.reserve(n_rays);
+        ctx->q_out_d.reserve(n_rays);
+        pkg_status e = pkg_raycast_closest_device(ctx, n_rays, ctx->q_org.p, ctx->q_dir.p, ctx->q_maxd.p,
+                                                  ray_world ? ctx->q_world.p : nullptr, ctx->q_out_id.p,
+                                                  ctx->q_out_d.p);
+        if (e != PKG_OK) return e;
+        hip_check(hipMemcpyAsync(ids, ctx->q_out_id.p, n_rays * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "ids");
+        hip_check(hipMemcpyAsync(dists, ctx->q_out_d.p, n_rays * sizeof(double), hipMemcpyDeviceToHost, st), "dists");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+    })
+}
+
+pkg_status pkg_query_aabb(pkg_ctx *ctx, uint32_t n_boxes, const double *boxes, const uint32_t *box_world,
+                          uint32_t *offsets, uint32_t *ids, size_t cap, size_t *total)
+{
+    if (!ctx || !total || !offsets || (n_boxes && !boxes)) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    if (pkg_status e = check_query_worlds(ctx, n_boxes, box_world)) return e;
+    PKG_GUARD(ctx, {
+        hipStream_t st = ctx->st;
+        *total = 0;
+        offsets[0] = 0;
+        if (!n_boxes) return PKG_OK;
+        upload(ctx->q_box, boxes, 6 * static_cast<size_t>(n_boxes), st);
+        if (box_world) upload(ctx->q_world, box_world, n_boxes, st);
+        const uint32_t *qw = box_world ? ctx->q_world.p : nullptr;
+        lbvh_dev t = ctx->query_tree();
+        bodies_dev b = ctx->B();
+        unsigned long long *fail = ctx->counters.p + C_TREE_FAIL;
+        ctx->reset_counter(C_TREE_FAIL);
+        ctx->q_cnt.reserve(n_boxes);
+        launch_box_count(t, b, n_boxes, ctx->q_box.p, qw, ctx->q_cnt.p, fail, st);
+        check_hit_total(ctx, n_boxes);
+        const uint32_t m = ctx->scan_counts(n_boxes);
+        check_tree_walk(ctx);
+        hip_check(hipMemcpyAsync(offsets, ctx->q_off.p, (static_cast<size_t>(n_boxes) + 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, st),
+                  "offsets");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        *total = m;
+        if (!m || cap < m) return PKG_OK;
+        if (!ids) throw pk_error(PKG_INVALID_ARG, "ids are needed when cap >= total");
+        query_scratch qs = ctx->qscratch(m);
+        launch_box_fill(t, b, n_boxes, ctx->q_box.p, qw, ctx->q_off.p, qs.hid, qs.hq, fail, st);
+        ctx->q_out_id.reserve(m);
+        launch_hits_sort(qs, m, ctx->n, n_boxes, false, ctx->q_out_id.p, nullptr, ctx->sc, st);
+        hip_check(hipMemcpyAsync(ids, ctx->q_out_id.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "ids");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+        check_tree_walk(ctx);
     })
 }
 

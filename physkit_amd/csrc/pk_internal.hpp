@@ -258,6 +258,31 @@ void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t
 void launch_impulse_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r32, uint32_t cap,
                                 hipStream_t st);
 
+// ---------------- k_query.hip ----------------
+// hits of a batch of rays / boxes: body, distance (rays), query index; sort buffers
+struct query_scratch
+{
+    uint32_t *hid, *hq, *perm, *perm_alt;
+    double *hd;
+    uint64_t *keys, *keys_alt;
+};
+void launch_ray_closest(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
+                        const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
+                        unsigned long long *fail, hipStream_t st);
+void launch_ray_count(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
+                      const double *maxd, const uint32_t *rw, uint32_t *cnt, unsigned long long *fail, hipStream_t st);
+void launch_ray_fill(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
+                     const double *maxd, const uint32_t *rw, const uint32_t *off, uint32_t *hid, double *hd,
+                     uint32_t *hq, unsigned long long *fail, hipStream_t st);
+void launch_box_count(const lbvh_dev &t, const bodies_dev &b, uint32_t nq, const double *boxes, const uint32_t *qw,
+                      uint32_t *cnt, unsigned long long *fail, hipStream_t st);
+void launch_box_fill(const lbvh_dev &t, const bodies_dev &b, uint32_t nq, const double *boxes, const uint32_t *qw,
+                     const uint32_t *off, uint32_t *hid, uint32_t *hq, unsigned long long *fail, hipStream_t st);
+// sorts the m hits by (query, distance, body) (by_distance) or (query, body) and writes
+// bodies (and distances) in that order
+void launch_hits_sort(query_scratch &qs, uint32_t m, uint32_t n_bodies, uint32_t n_queries, bool by_distance,
+                      uint32_t *out_id, double *out_d, sort_scratch &sc, hipStream_t st);
+
 // ---------------- k_joint.hip ----------------
 // Joint records: kJW planes of `cap` doubles (word w of joint j at rec[w * cap + j]),
 // joints dense in key order. Words: la 0-2, lb 3-5, ax_a 6-8, ax_b 9-11, f1 12-14,

@@ -133,6 +133,20 @@ pkg_status pkg_get_joints(pkg_ctx *ctx, uint32_t *keys, double *impulses, size_t
     }
     return PKG_OK;
 }
+pkg_status pkg_raycast(pkg_ctx *, uint32_t n_rays, const double *, const double *, const double *,
+                       const uint32_t *, uint32_t *offsets, uint32_t *, double *, size_t, size_t *total)
+{
+    for (uint32_t i = 0; i <= n_rays; ++i) offsets[i] = 0;
+    *total = 0;
+    return PKG_OK;
+}
+pkg_status pkg_query_aabb(pkg_ctx *, uint32_t n_boxes, const double *, const uint32_t *, uint32_t *offsets,
+                          uint32_t *, size_t, size_t *total)
+{
+    for (uint32_t i = 0; i <= n_boxes; ++i) offsets[i] = 0;
+    *total = 0;
+    return PKG_OK;
+}
 pkg_status pkg_get_counters(pkg_ctx *ctx, pkg_counters *out)
 {
     *out = pkg_counters{};
