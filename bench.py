@@ -42,7 +42,11 @@ WORKLOADS = {
     "batched": (3, "4,096 independent worlds of 16-4,096 bodies (log-uniform; 40% UV spheres "
                    "16x32, 40% boxes, 20% 8-16-point hulls) over static ground boxes, dt=1/60, "
                    "10 PGS iterations, one batched step per launch sequence, FP64"),
+    "rays": (1, "closest-hit raycasts: 4,194,304 rays (origins uniform in the scene box grown by "
+                "10 m; half aimed at a random body, half uniform directions; unbounded) into the "
+                "262,144-hull cluster scene's broadphase tree, FP64 slab tests, rays resident in HBM"),
 }
+RAYS = 1 << 22
 
 
 def algorithmic_bytes(stage, s, pairs, hits_frac):
@@ -216,6 +220,138 @@ def cpu_baseline(s, budget_s=12.0):
                       f"{t:.1f} s on one host core"}
 
 
+class _Hip:
+    """hipMalloc / hipMemcpy / hipFree through ctypes (device buffers without torch)."""
+
+    def __init__(self):
+        import ctypes
+        self.C = ctypes
+        path = "/opt/rocm/lib/libamdhip64.so"
+        with open("/proc/self/maps") as f:  # the HIP runtime the library already loaded
+            for line in f:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        self.L = ctypes.CDLL(path)
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        if self.L.hipMalloc(self.C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p
+
+    def h2d(self, dst, a):
+        if self.L.hipMemcpy(dst, a.ctypes.data, a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def d2h(self, a, src):
+        if self.L.hipMemcpy(a.ctypes.data, src, a.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def free(self, p):
+        self.L.hipFree(p)
+
+
+def ray_batch(s, n, seed=20261203):
+    """n rays: origins uniform in the bodies' box grown by 10 m; half aimed at a random
+    body (plus N(0, 0.1 m) per axis), half uniform directions; no distance limit."""
+    rng = np.random.default_rng(seed)
+    lo, hi = s["pos"].min(axis=0) - 10.0, s["pos"].max(axis=0) + 10.0
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    k = n // 2
+    d[:k] = s["pos"][rng.integers(len(s["pos"]), size=k)] + rng.normal(0.0, 0.1, (k, 3)) - o[:k]
+    return o, d, np.full(n, np.inf)
+
+
+def ray_algorithmic_bytes(n_rays, n_bodies):
+    """Compulsory HBM bytes of one closest-hit launch: per ray the origin, direction and
+    limit read (56 B) and the body id and distance written (12 B); the tree read once:
+    n-1 internal nodes of two float4 (32 B) and, per leaf, its FP64 box (48 B), body id,
+    escape index (4 B each) and the body's static flag (1 B)."""
+    return 68.0 * n_rays + 32.0 * (n_bodies - 1) + 57.0 * n_bodies
+
+
+def bench_rays(args, pk, s):
+    import ctypes
+    hip = _Hip()
+    g = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
+    g.load_scene(s)
+    n = args.n_rays
+    o, d, m = ray_batch(s, n)
+    bufs = [hip.alloc(a.nbytes) for a in (o, d, m)]
+    for b, a in zip(bufs, (o, d, m)):
+        hip.h2d(b, np.ascontiguousarray(a))
+    ids_d, dist_d = hip.alloc(4 * n), hip.alloc(8 * n)
+    kms = ctypes.c_double(0.0)
+
+    def call():
+        g._check(g.L.pkg_raycast_closest_device(g.h, n, bufs[0], bufs[1], bufs[2], None, ids_d, dist_d,
+                                                ctypes.byref(kms)), "raycast_closest_device")
+        return kms.value
+
+    print(f"[bench] workload=rays bodies={len(s['shape'])} rays={n} steps={args.steps} "
+          f"warmup={args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):  # the first call builds the tree (no step has run)
+        call()
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms += call()
+    wall = time.perf_counter() - t0
+    ids = np.zeros(n, np.uint32)
+    dist = np.zeros(n)
+    hip.d2h(ids, ids_d)
+    hip.d2h(dist, dist_d)
+    t_launch = kernel_ms / 1e3 / args.steps
+    alg = ray_algorithmic_bytes(n, len(s["shape"]))
+    achieved = alg / t_launch / 1e9
+    out = {
+        "metric": "rays/s (closest hit)", "value": round(n * args.steps / wall, 1), "unit": "rays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": WORKLOADS["rays"][1], "bodies": len(s["shape"]), "rays_per_step": n,
+                   "hit_fraction": round(float(np.mean(ids != 0xFFFFFFFF)), 4),
+                   "kernel_rays_per_s": round(n / t_launch, 1), "parallelism": "1 GPU, one world"},
+        "roofline": {"bound": "hbm", "kernel": "k_ray_closest", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "algorithmic_bytes_per_launch": alg,
+                     "avg_launch_ms": round(t_launch * 1e3, 4)},
+    }
+    g.close()
+    for b in bufs + [ids_d, dist_d]:
+        hip.free(b)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_rays(s, o, d, m, ids, dist)
+    return out
+
+
+def cpu_baseline_rays(s, o, d, m, gpu_ids, gpu_dist, budget_s=12.0):
+    """The oracle's tree-walking closest-hit raycast (one host thread) over the same
+    scene, on leading chunks of the same rays until about budget_s seconds of walking;
+    its answers are checked against the GPU's for those rays."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi
+    w = oracle_ffi.OracleWorld(iterations=s["iterations"], solver_mode=1, flags=s["flags"])
+    w.load_scene(s)
+    done, sec, chunk = 0, 0.0, 131072
+    while sec < budget_s and done < len(o):
+        e = min(done + chunk, len(o))
+        ids, dist, t = w.raycast_closest_bvh(o[done:e], d[done:e], m[done:e])
+        if not (np.array_equal(ids, gpu_ids[done:e]) and np.array_equal(dist, gpu_dist[done:e])):
+            raise RuntimeError("CPU and GPU closest hits differ")
+        sec += t
+        done = e
+    return {"value": done / sec, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"rays 0-{done - 1} of the same batch through a CPU tree walk (median-split BVH "
+                      f"over the same leaves, build untimed), {sec:.1f} s on one host core; "
+                      f"answers equal the GPU's"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +365,7 @@ def main():
                          "drift against the FP64 solve")
     ap.add_argument("--no-dropin", action="store_true",
                     help="batched workloads: skip the per-world drop-in comparison")
+    ap.add_argument("--n-rays", type=int, default=RAYS, help="rays workload: rays per launch")
     ap.add_argument("--copy-check", type=int, default=1024, metavar="MIB",
                     help="after the timed steps, time a plain device copy of MIB MiB (measured "
                          "copy bandwidth; the calibration kernel for HBM byte counters); 0 skips it")
@@ -242,6 +379,9 @@ def main():
 
     kind, desc = WORKLOADS[args.workload]
     s = pk.scene(kind, None, args.n)
+    if args.workload == "rays":
+        print(json.dumps(bench_rays(args, pk, s)), flush=True)
+        return
     flags = s["flags"] | (pk.FLAG_FP32_SOLVE if args.fp32_solve else 0)
     g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
     g.load_scene(s)

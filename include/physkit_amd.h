@@ -246,10 +246,11 @@ pkg_status pkg_raycast(pkg_ctx *ctx, uint32_t n_rays, const double *origins, con
 pkg_status pkg_raycast_closest(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
                                const double *max_dist, const uint32_t *ray_world, uint32_t *ids, double *dists);
 /* The same with every array in device memory (the ctx's device); runs on the ctx's
- * stream and returns when the results are written. */
+ * stream and returns when the results are written. kernel_ms (optional, host): device
+ * time of the query kernel, from HIP events around it on the ctx stream. */
 pkg_status pkg_raycast_closest_device(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
                                       const double *max_dist, const uint32_t *ray_world, uint32_t *ids,
-                                      double *dists);
+                                      double *dists, double *kernel_ms);
 
 /* dynamic_bvh::query_aabb (bvh.h:313-344) over both trees for n_boxes boxes (min xyz,
  * max xyz): bodies whose bounds intersect box i (aabb::intersects, bounds.h:87-92) are

@@ -137,7 +137,7 @@ def lib():
     L.pkg_raycast.argtypes = [P, C.c_uint32, D, D, D, U32, U32, U32, D, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_raycast_closest.argtypes = [P, C.c_uint32, D, D, D, U32, U32, D]
     L.pkg_raycast_closest_device.argtypes = [P, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                             C.c_void_p, C.c_void_p]
+                                             C.c_void_p, C.c_void_p, D]
     L.pkg_query_aabb.argtypes = [P, C.c_uint32, D, U32, U32, U32, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_add_joints.argtypes = [P, C.POINTER(pkg_joint_desc), C.c_size_t]
     L.pkg_remove_joints.argtypes = [P, U32, C.c_size_t]

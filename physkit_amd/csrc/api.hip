@@ -202,6 +202,7 @@ struct pkg_ctx
     dbuf<double> q_hd, q_org, q_dir, q_maxd, q_box, q_out_d;
     dbuf<uint64_t> q_keys, q_keys_alt;
     dbuf<uint32_t> q_out_id, q_world;
+    hipEvent_t q_ev[2] = {nullptr, nullptr}; // query kernel timing
 
     pkg_counters last{};
     bool have_bodies = false;
@@ -819,6 +820,8 @@ pkg_status pkg_destroy(pkg_ctx *ctx)
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     // the dbuf members free their device memory in their destructors
     if (ctx->sc.hist) (void)hipFree(ctx->sc.hist);
+    for (hipEvent_t e : ctx->q_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
@@ -1165,18 +1168,29 @@ pkg_status pkg_raycast(pkg_ctx *ctx, uint32_t n_rays, const double *origins, con
 
 pkg_status pkg_raycast_closest_device(pkg_ctx *ctx, uint32_t n_rays, const double *origins, const double *dirs,
                                       const double *max_dist, const uint32_t *ray_world, uint32_t *ids,
-                                      double *dists)
+                                      double *dists, double *kernel_ms)
 {
     if (!ctx || (n_rays && (!origins || !dirs || !max_dist || !ids || !dists))) return PKG_INVALID_ARG;
     if (!ctx->have_bodies) return PKG_INVALID_ARG;
     PKG_GUARD(ctx, {
+        if (kernel_ms) *kernel_ms = 0.0;
         if (!n_rays) return PKG_OK;
         lbvh_dev t = ctx->query_tree();
         ctx->reset_counter(C_TREE_FAIL);
+        if (kernel_ms && !ctx->q_ev[0])
+            for (auto &e : ctx->q_ev) hip_check(hipEventCreate(&e), "event create");
+        if (kernel_ms) hip_check(hipEventRecord(ctx->q_ev[0], ctx->st), "event record");
         launch_ray_closest(t, ctx->B(), n_rays, origins, dirs, max_dist, ray_world, ids, dists,
                            ctx->counters.p + C_TREE_FAIL, ctx->st);
         hip_check(hipGetLastError(), "raycast launch");
-        check_tree_walk(ctx);
+        if (kernel_ms) hip_check(hipEventRecord(ctx->q_ev[1], ctx->st), "event record");
+        check_tree_walk(ctx); // synchronizes the stream
+        if (kernel_ms)
+        {
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, ctx->q_ev[0], ctx->q_ev[1]), "event time");
+            *kernel_ms = ms;
+        }
     })
 }
 
@@ -1197,7 +1211,7 @@ pkg_status pkg_raycast_closest(pkg_ctx *ctx, uint32_t n_rays, const double *orig
         ctx->q_out_d.reserve(n_rays);
         pkg_status e = pkg_raycast_closest_device(ctx, n_rays, ctx->q_org.p, ctx->q_dir.p, ctx->q_maxd.p,
                                                   ray_world ? ctx->q_world.p : nullptr, ctx->q_out_id.p,
-                                                  ctx->q_out_d.p);
+                                                  ctx->q_out_d.p, nullptr);
         if (e != PKG_OK) return e;
         hip_check(hipMemcpyAsync(ids, ctx->q_out_id.p, n_rays * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "ids");
         hip_check(hipMemcpyAsync(dists, ctx->q_out_d.p, n_rays * sizeof(double), hipMemcpyDeviceToHost, st), "dists");

@@ -72,6 +72,8 @@ def lib():
         L.pko_raycast.argtypes = [C.c_void_p, D, D, C.c_double, U32, D, C.c_size_t]
         L.pko_query_aabb.restype = C.c_size_t
         L.pko_query_aabb.argtypes = [C.c_void_p, D, U32, C.c_size_t]
+        L.pko_raycast_closest_bvh.restype = C.c_double
+        L.pko_raycast_closest_bvh.argtypes = [C.c_void_p, C.c_size_t, D, D, D, U32, D]
         L.pko_ray_box.restype = C.c_int
         L.pko_ray_box.argtypes = [D, D, D, C.c_double, D]
         L.pko_add_joint.restype = C.c_int
@@ -296,6 +298,16 @@ class OracleWorld:
         if n:
             self.L.pko_raycast(self.h, dp(o), dp(d), float(max_dist), ids.ctypes.data_as(U32), dp(dist), n)
         return ids, dist
+
+    def raycast_closest_bvh(self, origins, dirs, max_dist):
+        """Closest hit per ray through a CPU tree walk: (ids, dists, seconds walking)."""
+        o = f64(np.asarray(origins, float).reshape(-1, 3))
+        d = f64(np.asarray(dirs, float).reshape(-1, 3))
+        m = f64(np.broadcast_to(np.asarray(max_dist, float), (len(o),)))
+        ids = np.zeros(len(o), np.uint32)
+        dist = np.zeros(len(o))
+        sec = self.L.pko_raycast_closest_bvh(self.h, len(o), dp(o), dp(d), dp(m), ids.ctypes.data_as(U32), dp(dist))
+        return ids, dist, sec
 
     def query_aabb(self, mn, mx):
         b = f64(list(mn) + list(mx))

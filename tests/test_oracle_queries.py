@@ -131,3 +131,22 @@ def test_queries_see_fat_bounds_and_skip_disabled_slots(oracle):
     assert list(w.raycast((5, 10, 0), (0, -1, 0))[0]) == []  # the disabled slot
     assert list(w.query_aabb((4, -1, -1), (6, 1, 1))) == []
     assert list(w.query_aabb((-1, -6, -1), (1, 1, 1))) == [0, 2]
+
+
+def test_cpu_tree_closest_hit_equals_brute_force(oracle):
+    """The CPU tree walk (the CPU baseline of the raycast bench) returns the first
+    element of the brute-force hit list for every ray."""
+    centers = RNG.uniform(-30, 30, (3000, 3))
+    w, _ = _world(oracle, centers, half=0.7)
+    o = RNG.uniform(-40, 40, (1500, 3))
+    d = centers[RNG.integers(3000, size=1500)] - o
+    d[::7] = RNG.normal(size=(len(d[::7]), 3))
+    d[::11, 1] = 0.0
+    m = np.where(RNG.random(1500) < 0.5, np.inf, RNG.uniform(1, 60, 1500))
+    ids, dist, sec = w.raycast_closest_bvh(o, d, m)
+    for r in range(1500):
+        bi, bd = w.raycast(o[r], d[r], m[r])
+        if len(bi):
+            assert ids[r] == bi[0] and dist[r] == bd[0], r
+        else:
+            assert ids[r] == 0xFFFFFFFF and dist[r] == np.inf, r
