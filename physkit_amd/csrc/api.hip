@@ -198,6 +198,8 @@ struct pkg_ctx
 
     // world queries
     bool tree_ok = false; // the LBVH holds the current tree bounds (built by the last step or query)
+    bool rn_ok = false;   // q_rn holds the walk topology of that tree
+    dbuf<uint4> q_rn;
     dbuf<uint32_t> q_cnt, q_off, q_hid, q_hq, q_perm, q_perm_alt, q_bsums, q_total, q_u32;
     dbuf<double> q_hd, q_org, q_dir, q_maxd, q_box, q_out_d;
     dbuf<uint64_t> q_keys, q_keys_alt;
@@ -349,6 +351,7 @@ lbvh_dev pkg_ctx::build_tree(const bodies_dev &b, bool all_moved)
         std::swap(ids.p, ids_alt.p);
     }
     tree_ok = true;
+    rn_ok = false;
     return t;
 }
 
@@ -1179,8 +1182,14 @@ pkg_status pkg_raycast_closest_device(pkg_ctx *ctx, uint32_t n_rays, const doubl
         ctx->reset_counter(C_TREE_FAIL);
         if (kernel_ms && !ctx->q_ev[0])
             for (auto &e : ctx->q_ev) hip_check(hipEventCreate(&e), "event create");
+        if (!ctx->rn_ok)
+        {
+            ctx->q_rn.reserve(ctx->n);
+            launch_ray_nodes(t, ctx->q_rn.p, ctx->st);
+            ctx->rn_ok = true;
+        }
         if (kernel_ms) hip_check(hipEventRecord(ctx->q_ev[0], ctx->st), "event record");
-        launch_ray_closest(t, ctx->B(), n_rays, origins, dirs, max_dist, ray_world, ids, dists,
+        launch_ray_closest(t, ctx->q_rn.p, ctx->B(), n_rays, origins, dirs, max_dist, ray_world, ids, dists,
                            ctx->counters.p + C_TREE_FAIL, ctx->st);
         hip_check(hipGetLastError(), "raycast launch");
         if (kernel_ms) hip_check(hipEventRecord(ctx->q_ev[1], ctx->st), "event record");

@@ -137,26 +137,126 @@ __device__ __forceinline__ void walk_ray(const lbvh_dev &t, const bodies_dev &b,
     }
 }
 
+// Walk topology for ordered ray walks: per internal node (left, right, parent, split
+// axis). The split axis is the axis of the highest Morton bit that differs between the
+// last leaf of the left child and the first of the right child (bit 3L+2 is x, 3L+1 y,
+// 3L z); a ray enters the child on its own side of that split first.
+__global__ void k_ray_nodes(lbvh_dev t, uint4 *rn)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = t.n;
+    if (i + 1 >= n) return;
+    uint32_t lc = static_cast<uint32_t>(t.left[i]), rc = static_cast<uint32_t>(t.right[i]);
+    uint32_t g = lc >= n - 1 ? lc - (n - 1) : t.last[lc];
+    uint64_t x = t.morton[g] ^ t.morton[g + 1];
+    uint32_t axis = 0;
+    if (x)
+    {
+        int bit = 63 - __clzll(static_cast<long long>(x));
+        int wb = t.n_worlds > 1 ? t.wbits : 0;
+        if (bit < 63 - wb) axis = static_cast<uint32_t>(2 - (bit + wb) % 3);
+    }
+    rn[i] = make_uint4(lc, rc, t.parent[i], axis);
+}
+
+// Closest hit with an ordered stackless walk (Hapala, Davidovic, Wald, Havran,
+// Slusallek, "Efficient Stack-less BVH Traversal for Ray Tracing", SCCG 2011): the
+// near child first, moving through parent links and a three-state machine, so the
+// first hits found are close and prune the rest of the walk with their distance.
 __global__ void __launch_bounds__(kBlock)
-    k_ray_closest(lbvh_dev t, bodies_dev b, uint32_t nr, const double *org, const double *dir, const double *maxd,
-                  const uint32_t *rw, uint32_t *out_id, double *out_d, unsigned long long *fail)
+    k_ray_closest(lbvh_dev t, const uint4 *rn, bodies_dev b, uint32_t nr, const double *org, const double *dir,
+                  const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
+                  unsigned long long *fail)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     const qray q = load_ray(t, r, org, dir, maxd, rw);
+    const uint32_t n = t.n;
     double best = q.maxd;
     uint32_t best_id = kNone;
-    walk_ray(
-        t, b, q,
-        [&](uint32_t body, double d)
+    const bool neg[3] = {q.inv.x < 0.0, q.inv.y < 0.0, q.inv.z < 0.0};
+    const bool ranged = q.wlo != 0 || q.whi != n - 1;
+    // box test of a node (leaves: exact FP64, accepting the hit)
+    auto visit = [&](uint32_t node) -> bool
+    {
+        double td;
+        if (node + 1 < n)
         {
-            if (d < best || (d == best && body < best_id))
+            float4 lo = t.inode_lo[node], hi = t.inode_hi[node];
+            bool hit = ray_slab(q.o, q.inv, mk(lo.x, lo.y, lo.z), mk(hi.x, hi.y, hi.z), best, td);
+            if (hit && ranged) hit = node_in_range(t, node, q.wlo, q.whi);
+            return hit;
+        }
+        const uint32_t j = node - (n - 1);
+        if (j >= q.wlo && j <= q.whi)
+        {
+            const uint32_t body = t.leaf_body[j];
+            const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
+            if (b.is_static[body] != 2 && ray_slab(q.o, q.inv, mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]), best, td) &&
+                (td < best || (td == best && body < best_id)))
             {
-                best = d;
+                best = td;
                 best_id = body;
             }
-        },
-        [&]() { return best; }, fail);
+        }
+        return false; // a leaf has nothing below it
+    };
+    if (n == 1)
+        visit(0);
+    else if (n > 1 && q.whi >= q.wlo && q.whi != 0xFFFFFFFFu && visit(0))
+    {
+        enum
+        {
+            FROM_PARENT,
+            FROM_SIBLING,
+            FROM_CHILD
+        } state = FROM_PARENT;
+        uint4 pr = rn[0]; // the node whose children are being walked
+        uint32_t cur = neg[pr.w] ? pr.y : pr.x;
+        uint32_t par = 0;
+        for (uint32_t guard = 0;; ++guard)
+        {
+            if (guard >= 6 * n)
+            {
+                atomicAdd(fail, 1ull); // malformed tree: reported, never loops
+                break;
+            }
+            if (state == FROM_CHILD)
+            {
+                if (cur == 0) break; // back at the root: done
+                par = cur + 1 < n ? rn[cur].z : t.parent[cur];
+                pr = rn[par];
+                uint32_t near = neg[pr.w] ? pr.y : pr.x, far = neg[pr.w] ? pr.x : pr.y;
+                if (cur == near)
+                {
+                    cur = far;
+                    state = FROM_SIBLING;
+                }
+                else
+                    cur = par;
+                continue;
+            }
+            const bool inside = visit(cur);
+            if (!inside)
+            {
+                if (state == FROM_PARENT)
+                {
+                    cur = neg[pr.w] ? pr.x : pr.y; // the far sibling
+                    state = FROM_SIBLING;
+                }
+                else
+                {
+                    cur = par;
+                    state = FROM_CHILD;
+                }
+                continue;
+            }
+            par = cur; // an internal node the ray enters: walk its children, near first
+            pr = rn[cur];
+            cur = neg[pr.w] ? pr.y : pr.x;
+            state = FROM_PARENT;
+        }
+    }
     out_id[r] = best_id;
     out_d[r] = best_id == kNone ? __builtin_inf() : best;
 }
@@ -328,11 +428,15 @@ uint32_t *sort_perm(query_scratch &qs, uint32_t m, int bits, sort_scratch &sc, h
 }
 } // namespace
 
-void launch_ray_closest(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
-                        const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
+void launch_ray_nodes(const lbvh_dev &t, uint4 *rn, hipStream_t st)
+{
+    if (t.n >= 2) k_ray_nodes<<<grid(t.n - 1), kBlock, 0, st>>>(t, rn);
+}
+void launch_ray_closest(const lbvh_dev &t, const uint4 *rn, const bodies_dev &b, uint32_t nr, const double *org,
+                        const double *dir, const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
                         unsigned long long *fail, hipStream_t st)
 {
-    if (nr) k_ray_closest<<<grid(nr), kBlock, 0, st>>>(t, b, nr, org, dir, maxd, rw, out_id, out_d, fail);
+    if (nr) k_ray_closest<<<grid(nr), kBlock, 0, st>>>(t, rn, b, nr, org, dir, maxd, rw, out_id, out_d, fail);
 }
 void launch_ray_count(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
                       const double *maxd, const uint32_t *rw, uint32_t *cnt, unsigned long long *fail, hipStream_t st)

@@ -266,8 +266,10 @@ struct query_scratch
     double *hd;
     uint64_t *keys, *keys_alt;
 };
-void launch_ray_closest(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
-                        const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
+// per internal node (left, right, parent, split axis) for the ordered closest-hit walk
+void launch_ray_nodes(const lbvh_dev &t, uint4 *rn, hipStream_t st);
+void launch_ray_closest(const lbvh_dev &t, const uint4 *rn, const bodies_dev &b, uint32_t nr, const double *org,
+                        const double *dir, const double *maxd, const uint32_t *rw, uint32_t *out_id, double *out_d,
                         unsigned long long *fail, hipStream_t st);
 void launch_ray_count(const lbvh_dev &t, const bodies_dev &b, uint32_t nr, const double *org, const double *dir,
                       const double *maxd, const uint32_t *rw, uint32_t *cnt, unsigned long long *fail, hipStream_t st);
