@@ -123,6 +123,7 @@ struct pkg_ctx
 
     uint32_t n_shapes = 0, n_verts = 0;
     bool has_uv = false; // some shape is a tagged UV sphere
+    std::vector<uint32_t> shape_cost; // support candidates per shape: 12 for a tagged UV sphere, else its vertices
     dbuf<uint32_t> shape_off, shape_uv;
     dbuf<double> verts, shape_box;
 
@@ -162,6 +163,9 @@ struct pkg_ctx
 
     // narrowphase
     dbuf<uint32_t> hit_list, ovf_list;
+    // lanes per pair in GJK: 2 when the bodies' shapes average at most 14 support
+    // candidates (boxes, certified UV spheres, small hulls), else 4
+    int gjk_group = 4;
     dbuf<double> simplex, info;
     dbuf<uint8_t> status;
     dbuf<char> big_pool;
@@ -576,7 +580,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
     if (Pn)
     {
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, g, counters.p, st);
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);
@@ -850,6 +854,8 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
             uv[i] = det.tag(verts + 3 * static_cast<size_t>(offsets[i]), offsets[i + 1] - offsets[i]);
         upload(ctx->shape_uv, uv.data(), uv.size(), ctx->st);
         ctx->has_uv = std::any_of(uv.begin(), uv.end(), [](uint32_t t) { return t != 0; });
+        ctx->shape_cost.resize(n_shapes);
+        for (uint32_t i = 0; i < n_shapes; ++i) ctx->shape_cost[i] = uv[i] ? 12u : offsets[i + 1] - offsets[i];
         upload(ctx->shape_off, offsets, n_shapes + 1, ctx->st);
         upload(ctx->verts, verts, static_cast<size_t>(nv) * 3, ctx->st);
         ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
@@ -880,6 +886,15 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         }
     PKG_GUARD(ctx, {
         ctx->h_static.assign(d->is_static, d->is_static + n);
+        double cost = 0.0;
+        uint32_t live = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            if (d->is_static[i] != 2)
+            {
+                cost += ctx->shape_cost[d->shape[i]];
+                ++live;
+            }
+        ctx->gjk_group = (live && cost <= 14.0 * live) ? 2 : 4;
         ctx->jcoloured = false;
         hipStream_t st = ctx->st;
         ctx->n = n;

@@ -73,18 +73,19 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-// GJK, kGjkGroup lanes per pair: the lanes of a group read consecutive vertices of
-// the same shape (coalesced) and reduce the support search; the simplex logic runs
-// identically in every lane of the group, and lane 0 writes the results.
-constexpr int kGjkGroup = 8;
-template <bool UV>
+// GJK, G lanes per pair: the lanes of a group read consecutive vertices of the same
+// shape (coalesced) and reduce the support search; the simplex logic runs identically
+// in every lane of the group, and lane 0 writes the results. Fewer lanes suit shapes
+// with few support candidates (boxes, certified UV spheres): less redundant simplex
+// work per lane.
+template <int G, bool UV>
 __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
     k_gjk(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, gjk_out_dev g,
           unsigned long long *counters)
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t p = t / kGjkGroup;
-    int gl = static_cast<int>(t % kGjkGroup);
+    uint32_t p = t / G;
+    int gl = static_cast<int>(t % G);
     bool hit = false;
     simplex x;
     x.n = 0;
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
         uint64_t key = keys[p];
         gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
         gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-        hit = gjk<kGjkGroup, UV>(A, B, x, gl);
+        hit = gjk<G, UV>(A, B, x, gl);
         if (gl == 0) g.status[p] = 0;
     }
     bool lead = hit && gl == 0;
@@ -409,15 +410,25 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
-void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv,
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    unsigned gr = grid(static_cast<size_t>(n_pairs) * kGjkGroup);
-    if (uv)
-        k_gjk<true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
+    if (group == 2)
+    {
+        if (uv)
+            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+        else
+            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    }
     else
-        k_gjk<false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    {
+        if (uv)
+            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+        else
+            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+    }
 }
 
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,

@@ -165,7 +165,8 @@ struct gjk_out_dev
     uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
     uint32_t pair_cap;
 };
-void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv,
+// GJK over every pair with `group` (2 or 4) lanes per pair
+void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
