@@ -1318,9 +1318,10 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
     {
         const uint64_t none = ~0ull;
         std::vector<uint64_t> claim(bodies.size(), none);
+        std::vector<uint64_t> used(bodies.size(), 0); // colours taken at each dynamic body (bits 0-63)
         std::vector<int> col(M, -1);
         std::size_t left = M;
-        uint32_t c = 0;
+        uint32_t round = 0, ncol = 0;
         while (left > 0)
         {
             std::fill(claim.begin(), claim.end(), none);
@@ -1337,16 +1338,26 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
                 if (col[i] >= 0) continue;
                 const manifold &man = manifolds[midx[i]];
                 uint64_t pr = (static_cast<uint64_t>(colour_mix32(static_cast<uint32_t>(midx[i]))) << 32) | midx[i];
-                bool ok = (bodies[man.a].is_static || claim[man.a] == pr) &&
-                          (bodies[man.b].is_static || claim[man.b] == pr);
+                const bool sa = bodies[man.a].is_static, sb = bodies[man.b].is_static;
+                bool ok = (sa || claim[man.a] == pr) && (sb || claim[man.b] == pr);
                 if (ok)
                 {
-                    col[i] = static_cast<int>(c);
+                    // the smallest colour no coloured manifold at either dynamic body holds
+                    uint64_t u = (sa ? 0 : used[man.a]) | (sb ? 0 : used[man.b]);
+                    uint32_t k = ~u ? static_cast<uint32_t>(__builtin_ctzll(~u)) : 64u + round;
+                    col[i] = static_cast<int>(k);
+                    if (k < 64)
+                    {
+                        if (!sa) used[man.a] |= 1ull << k;
+                        if (!sb) used[man.b] |= 1ull << k;
+                    }
+                    ncol = std::max(ncol, k + 1);
                     --left;
                 }
             }
-            ++c;
+            ++round;
         }
+        const uint32_t c = ncol;
         counters.colours = c;
         for (uint32_t cc = 0; cc < c; ++cc)
             for (std::size_t i = 0; i < M; ++i)
@@ -1370,9 +1381,10 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
     {
         const uint64_t none = ~0ull;
         std::vector<uint64_t> claim(bodies.size(), none);
+        std::vector<uint64_t> used(bodies.size(), 0);
         std::vector<int> col(J, -1);
         std::size_t left = J;
-        uint32_t c = 0;
+        uint32_t round = 0, c = 0;
         while (left > 0)
         {
             std::fill(claim.begin(), claim.end(), none);
@@ -1389,13 +1401,22 @@ void world::stage_solve(double dt) // constraint.h:1052-1201
                 if (col[k] >= 0) continue;
                 const joint &jj = joints[k];
                 uint64_t pr = (static_cast<uint64_t>(colour_mix32(static_cast<uint32_t>(k))) << 32) | k;
-                if ((bodies[jj.a].is_static || claim[jj.a] == pr) && (bodies[jj.b].is_static || claim[jj.b] == pr))
+                const bool sa = bodies[jj.a].is_static, sb = bodies[jj.b].is_static;
+                if ((sa || claim[jj.a] == pr) && (sb || claim[jj.b] == pr))
                 {
-                    col[k] = static_cast<int>(c);
+                    uint64_t u = (sa ? 0 : used[jj.a]) | (sb ? 0 : used[jj.b]);
+                    uint32_t kc = ~u ? static_cast<uint32_t>(__builtin_ctzll(~u)) : 64u + round;
+                    col[k] = static_cast<int>(kc);
+                    if (kc < 64)
+                    {
+                        if (!sa) used[jj.a] |= 1ull << kc;
+                        if (!sb) used[jj.b] |= 1ull << kc;
+                    }
+                    c = std::max(c, kc + 1);
                     --left;
                 }
             }
-            ++c;
+            ++round;
         }
         for (uint32_t cc = 0; cc < c; ++cc)
             for (std::size_t k = 0; k < J; ++k)

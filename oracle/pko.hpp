@@ -224,10 +224,12 @@ inline uint64_t pair_key(uint32_t a, uint32_t b) // collision_phases.h:63-69
     return (static_cast<uint64_t>(lo) << 32) | hi;
 }
 
-// deterministic manifold colouring shared with the device: round c gives colour c to
-// every uncoloured manifold whose priority is the minimum, over all uncoloured
-// manifolds, at each of its dynamic bodies. priority = (mix32(p) << 32) | p, with p
-// the manifold's position in the sorted pair list.
+// deterministic manifold colouring shared with the device (Jones-Plassmann order with
+// the smallest free colour): each round, every uncoloured manifold whose priority is
+// the minimum, over all uncoloured manifolds, at each of its dynamic bodies takes the
+// smallest colour that no coloured manifold at those bodies holds (64 + the round
+// number when colours 0-63 are all taken there). priority = (mix32(p) << 32) | p, with
+// p the manifold's position in the sorted pair list.
 uint32_t colour_mix32(uint32_t x);
 
 struct world

@@ -179,7 +179,7 @@ struct pkg_ctx
     dbuf<uint64_t> ckeys, ckeys_alt;
     dbuf<double> rows_s;      // solver rows in solve order
     dbuf<uint8_t> row_valid_s;
-    dbuf<unsigned long long> claim;
+    dbuf<unsigned long long> claim, used; // colouring: per-body claims, colours taken per body
     uint32_t n_colours = 0;
 
     // joints, dense in key order; an add or remove rebuilds the records into the other set
@@ -370,17 +370,19 @@ void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
     jcol.reserve(J);
     claim.reserve(std::max<uint32_t>(n, 1));
     launch_fill_u32(jcol.p, kUncoloured, J, st);
-    uint32_t c = 0;
-    while (true)
+    used.reserve(std::max<uint32_t>(n, 1));
+    launch_fill_u64(used.p, 0ull, n, st);
+    reset_counter(C_NCOL);
+    for (uint32_t round = 0;; ++round)
     {
         launch_fill_u64(claim.p, ~0ull, n, st);
         launch_colour_claim(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, st);
         reset_counter(C_REMAIN);
-        launch_colour_assign(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, c, counters.p, st);
-        ++c;
+        launch_colour_assign(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, used.p, round, counters.p, st);
         if (read_counter(C_REMAIN) == 0) break;
-        if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
+        if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
     }
+    const uint32_t c = static_cast<uint32_t>(read_counter(C_NCOL));
     jorder.reserve(J);
     jorder_alt.reserve(J);
     jck.reserve(J);
@@ -634,16 +636,19 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                     wo = world_of.p;
                     pb = pair_base.p;
                 }
-                while (true)
+                used.reserve(std::max<uint32_t>(n, 1));
+                launch_fill_u64(used.p, 0ull, n, st);
+                reset_counter(C_NCOL);
+                for (uint32_t round = 0;; ++round)
                 {
                     launch_fill_u64(claim.p, ~0ull, n, st);
                     launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
                     reset_counter(C_REMAIN);
-                    launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, c, counters.p, st);
-                    ++c;
+                    launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, used.p, round, counters.p, st);
                     if (read_counter(C_REMAIN) == 0) break;
-                    if (c > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
+                    if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
+                c = static_cast<uint32_t>(read_counter(C_NCOL));
             }
         }
         n_colours = c;

@@ -4,8 +4,9 @@
 // all contact points. Here the manifolds are coloured so that no two manifolds of a
 // colour share a dynamic body; within a colour every manifold is independent, so one
 // thread per manifold reproduces, bit for bit, a sequential sweep in colour order.
-// Colouring: round c colours every uncoloured manifold whose priority
-// (mix32(p) << 32 | p) is the minimum claim at each of its dynamic bodies.
+// Colouring: each round, every uncoloured manifold whose priority (mix32(p) << 32 | p)
+// is the minimum claim at each of its dynamic bodies takes the smallest colour free at
+// those bodies.
 #include "pk_access.hpp"
 
 #include <cfloat>
@@ -320,9 +321,13 @@ __global__ void k_claim(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, co
     if (!b.is_static[ib]) atomicMin(&claim[ib], pr);
 }
 
+// A manifold holding the claim at each of its dynamic bodies takes the smallest colour
+// that no coloured manifold at those bodies holds (`used`: colours 0-63 taken at each
+// body), or 64 + the round number when all 64 are taken. Manifolds coloured in one
+// round share no dynamic body, so their choices are independent.
 __global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, uint32_t *colour,
                          const uint32_t *world_of, const uint32_t *pair_base, const unsigned long long *claim,
-                         uint32_t c, unsigned long long *counters)
+                         unsigned long long *used, uint32_t round, unsigned long long *counters)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool remain = false;
@@ -331,9 +336,20 @@ __global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, u
         uint64_t key = keys[p];
         uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
         unsigned long long pr = prio(local_index(p, ia, world_of, pair_base));
-        bool ok = (b.is_static[ia] || claim[ia] == pr) && (b.is_static[ib] || claim[ib] == pr);
+        const bool sa = b.is_static[ia] != 0, sb = b.is_static[ib] != 0;
+        bool ok = (sa || claim[ia] == pr) && (sb || claim[ib] == pr);
         if (ok)
-            colour[p] = c;
+        {
+            unsigned long long u = (sa ? 0ull : used[ia]) | (sb ? 0ull : used[ib]);
+            uint32_t k = ~u ? static_cast<uint32_t>(__ffsll(static_cast<long long>(~u)) - 1) : 64u + round;
+            colour[p] = k;
+            if (k < 64)
+            {
+                if (!sa) atomicOr(&used[ia], 1ull << k);
+                if (!sb) atomicOr(&used[ib], 1ull << k);
+            }
+            atomicMax(&counters[C_NCOL], static_cast<unsigned long long>(k) + 1ull);
+        }
         else
             remain = true;
     }
@@ -928,12 +944,12 @@ void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_de
 }
 void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                          const unsigned long long *claim, uint32_t c, unsigned long long *counters,
-                          hipStream_t st)
+                          const unsigned long long *claim, unsigned long long *used, uint32_t round,
+                          unsigned long long *counters, hipStream_t st)
 {
     if (n_pairs)
-        k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim, c,
-                                                   counters);
+        k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim, used,
+                                                   round, counters);
 }
 uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
                                uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,

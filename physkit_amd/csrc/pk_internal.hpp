@@ -76,6 +76,7 @@ enum counter_slot
     C_DEFER = 12,
     C_KEEP = 13,
     C_ISL_MAX = 14,
+    C_NCOL = 15,
     C_NCOUNTERS = 16
 };
 
@@ -211,10 +212,12 @@ void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *wo
 void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
                          const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                          unsigned long long *claim, hipStream_t st);
+// one colouring round: claim holders take the smallest colour free at their dynamic
+// bodies (`used`, zeroed before the first round; C_NCOL ends as the colour count)
 void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                          const unsigned long long *claim, uint32_t c, unsigned long long *counters,
-                          hipStream_t st);
+                          const unsigned long long *claim, unsigned long long *used, uint32_t round,
+                          unsigned long long *counters, hipStream_t st);
 // stable bucket of the solver manifolds by colour: returns the order array (one of
 // the two given) and colour_start[0..n_colours] (manifolds without rows come last)
 uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
