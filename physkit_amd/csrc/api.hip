@@ -141,6 +141,7 @@ struct pkg_ctx
     dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, visit, range_first, range_last, lastf, defer;
     dbuf<int32_t> left, right;
     dbuf<float4> inode_lo, inode_hi;
+    dbuf<uint4> nrec;
     dbuf<double> leaf_box;
     dbuf<uint8_t> leaf_moved, covered;
     dbuf<unsigned long long> scene;
@@ -309,6 +310,8 @@ struct pkg_ctx
 
 namespace
 {
+constexpr uint32_t kColourCheck = 4; // colouring rounds between host checks of the remaining count
+
 void record(hipEvent_t *ev, int k, hipStream_t st)
 {
     if (ev) hip_check(hipEventRecord(ev[k], st), "event record");
@@ -328,6 +331,7 @@ lbvh_dev pkg_ctx::tree()
     t.first = range_first.p;
     t.last = range_last.p;
     t.lastf = lastf.p;
+    t.nrec = nrec.p;
     t.inode_lo = inode_lo.p;
     t.inode_hi = inode_hi.p;
     t.parent = parent.p;
@@ -379,7 +383,7 @@ void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
         launch_colour_claim(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, st);
         reset_counter(C_REMAIN);
         launch_colour_assign(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, used.p, round, counters.p, st);
-        if (read_counter(C_REMAIN) == 0) break;
+        if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
         if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
     }
     const uint32_t c = static_cast<uint32_t>(read_counter(C_NCOL));
@@ -639,13 +643,15 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 used.reserve(std::max<uint32_t>(n, 1));
                 launch_fill_u64(used.p, 0ull, n, st);
                 reset_counter(C_NCOL);
+                // the host looks at the remaining count every kColourCheck rounds; rounds
+                // after the last manifold is coloured change nothing
                 for (uint32_t round = 0;; ++round)
                 {
                     launch_fill_u64(claim.p, ~0ull, n, st);
                     launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
                     reset_counter(C_REMAIN);
                     launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, used.p, round, counters.p, st);
-                    if (read_counter(C_REMAIN) == 0) break;
+                    if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
                     if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
                 c = static_cast<uint32_t>(read_counter(C_NCOL));
@@ -943,6 +949,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->defer.reserve(2 * static_cast<size_t>(n));
         ctx->inode_lo.reserve(n);
         ctx->inode_hi.reserve(n);
+        ctx->nrec.reserve(n);
         ctx->leaf_box.reserve(6 * static_cast<size_t>(n));
         launch_body_init(ctx->B(), ctx->S(), st);
         ctx->P[0] = ctx->P[1] = 0;

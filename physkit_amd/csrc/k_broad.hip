@@ -179,6 +179,8 @@ __global__ void k_build_internal(const uint64_t *k, int n, lbvh_dev t)
     t.right[i] = rc;
     t.first[i] = static_cast<uint32_t>(lo);
     t.last[i] = static_cast<uint32_t>(hi);
+    t.nrec[i] = make_uint4(static_cast<uint32_t>(lc), static_cast<uint32_t>(rc), static_cast<uint32_t>(lo),
+                           static_cast<uint32_t>(hi));
     t.parent[lc] = static_cast<uint32_t>(i);
     t.parent[rc] = static_cast<uint32_t>(i);
     if (i == 0) t.parent[0] = kNone;
@@ -252,17 +254,6 @@ __device__ __forceinline__ fbox child_box(const lbvh_dev &t, uint32_t c)
     return f;
 }
 
-__device__ __forceinline__ void finish_node(const lbvh_dev &t, uint32_t node)
-{
-    uint32_t l = static_cast<uint32_t>(t.left[node]), r = static_cast<uint32_t>(t.right[node]);
-    fbox a = child_box(t, l), b = child_box(t, r);
-    t.inode_lo[node] = make_float4(fminf(a.lo[0], b.lo[0]), fminf(a.lo[1], b.lo[1]), fminf(a.lo[2], b.lo[2]),
-                                   __uint_as_float(l));
-    t.inode_hi[node] = make_float4(fmaxf(a.hi[0], b.hi[0]), fmaxf(a.hi[1], b.hi[1]), fmaxf(a.hi[2], b.hi[2]),
-                                   __uint_as_float(t.escape[node]));
-    t.lastf[node] = t.last[node] | ((a.moved & b.moved) << 31);
-}
-
 __device__ __forceinline__ uint32_t wg_arrive(uint32_t *p)
 { return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
@@ -291,36 +282,25 @@ __device__ __forceinline__ fbox child_box_local(const lbvh_dev &t, uint32_t c, u
     return f;
 }
 
-// finish_node for a node inside the chunk: children from LDS (or the leaf boxes), the
-// result to LDS for the parent and to the node arrays for traversal
-__device__ __forceinline__ void finish_local(const lbvh_dev &t, uint32_t node, uint32_t c_lo, refit_lds &L)
+
+// Bottom-up refit in two launches. The second child to arrive at a node computes it.
+// Launch 1: a workgroup owns leaves [b*kChunk, (b+1)*kChunk) and refits the nodes whose
+// leaf range lies inside the chunk, synchronising only within the workgroup
+// (workgroup-scope acquire/release on LDS counters). A climb that reaches a node
+// spanning chunks stops there and lists its child (the top of a finished subtree).
+// Launch 2: one workgroup climbs from the listed tops through the spanning nodes to
+// the root; the launch boundary makes launch 1's boxes visible, so no arrival needs a
+// device-scope fence. The climbing thread carries its child's box in registers and
+// reads one record (children, leaf range) per level and only the sibling's box.
+__device__ __forceinline__ void store_node(const lbvh_dev &t, uint32_t node, const uint4 &rec, uint32_t esc,
+                                           const float *lo, const float *hi, uint32_t mv)
 {
-    uint32_t l = static_cast<uint32_t>(t.left[node]), r = static_cast<uint32_t>(t.right[node]);
-    fbox a = child_box_local(t, l, c_lo, L), b = child_box_local(t, r, c_lo, L);
-    float lo[3] = {fminf(a.lo[0], b.lo[0]), fminf(a.lo[1], b.lo[1]), fminf(a.lo[2], b.lo[2])};
-    float hi[3] = {fmaxf(a.hi[0], b.hi[0]), fmaxf(a.hi[1], b.hi[1]), fmaxf(a.hi[2], b.hi[2])};
-    uint32_t mv = a.moved & b.moved;
-    float *lb = L.box[node - c_lo];
-    for (int k = 0; k < 3; ++k)
-    {
-        lb[k] = lo[k];
-        lb[3 + k] = hi[k];
-    }
-    L.moved[node - c_lo] = static_cast<uint8_t>(mv);
-    t.inode_lo[node] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(l));
-    t.inode_hi[node] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(t.escape[node]));
-    t.lastf[node] = t.last[node] | (mv << 31);
+    t.inode_lo[node] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(rec.x));
+    t.inode_hi[node] = make_float4(hi[0], hi[1], hi[2], __uint_as_float(esc));
+    t.lastf[node] = rec.w | (mv << 31);
 }
 
-__device__ __forceinline__ uint32_t dev_arrive(uint32_t *p)
-{ return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT); }
-
-// Bottom-up refit: the second child to arrive at a node computes it. A workgroup owns
-// leaves [b*kChunk, (b+1)*kChunk); arrivals at nodes whose leaf range lies inside the
-// chunk only synchronise within the workgroup (workgroup-scope acquire/release, no
-// cache write-back), and only the few nodes spanning chunks pay a device-scope
-// acquire/release.
-__global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
+__global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t, unsigned long long *n_tops)
 {
     __shared__ refit_lds L;
     uint32_t n = t.n;
@@ -330,20 +310,71 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
     __syncthreads();
     for (uint32_t i = c_lo + threadIdx.x; i <= c_hi; i += blockDim.x)
     {
-        uint32_t node = t.parent[n - 1 + i];
+        uint32_t cur = n - 1 + i;
+        fbox mine = child_box(t, cur);
+        uint32_t node = t.parent[cur];
         for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
         {
-            bool local = t.first[node] >= c_lo && t.last[node] <= c_hi;
-            if (local)
+            const uint4 rec = t.nrec[node];
+            if (rec.z < c_lo || rec.w > c_hi)
             {
-                if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
-                finish_local(t, node, c_lo, L);
+                t.defer[atomicAdd(n_tops, 1ull)] = cur; // spanning node: finished in launch 2
+                break;
             }
-            else
+            const uint32_t esc = t.escape[node];
+            if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
+            const uint32_t sib = rec.x == cur ? rec.y : rec.x;
+            const fbox o = child_box_local(t, sib, c_lo, L);
+            float lo[3] = {fminf(mine.lo[0], o.lo[0]), fminf(mine.lo[1], o.lo[1]), fminf(mine.lo[2], o.lo[2])};
+            float hi[3] = {fmaxf(mine.hi[0], o.hi[0]), fmaxf(mine.hi[1], o.hi[1]), fmaxf(mine.hi[2], o.hi[2])};
+            const uint32_t mv = mine.moved & o.moved;
+            float *lb = L.box[node - c_lo];
+            for (int k = 0; k < 3; ++k)
             {
-                if (dev_arrive(&t.visit[node]) == 0) break;
-                finish_node(t, node);
+                lb[k] = lo[k];
+                lb[3 + k] = hi[k];
             }
+            L.moved[node - c_lo] = static_cast<uint8_t>(mv);
+            store_node(t, node, rec, esc, lo, hi, mv);
+            for (int k = 0; k < 3; ++k)
+            {
+                mine.lo[k] = lo[k];
+                mine.hi[k] = hi[k];
+            }
+            mine.moved = mv;
+            cur = node;
+            node = t.parent[node];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kRefitThreads) k_refit_top(lbvh_dev t, const unsigned long long *n_tops)
+{
+    const uint32_t n = t.n;
+    const uint32_t m = static_cast<uint32_t>(*n_tops);
+    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x)
+    {
+        uint32_t cur = t.defer[k];
+        fbox mine = child_box(t, cur);
+        uint32_t node = t.parent[cur];
+        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+        {
+            const uint4 rec = t.nrec[node];
+            const uint32_t esc = t.escape[node];
+            if (wg_arrive(&t.visit[node]) == 0) break;
+            const uint32_t sib = rec.x == cur ? rec.y : rec.x;
+            const fbox o = child_box(t, sib);
+            float lo[3] = {fminf(mine.lo[0], o.lo[0]), fminf(mine.lo[1], o.lo[1]), fminf(mine.lo[2], o.lo[2])};
+            float hi[3] = {fmaxf(mine.hi[0], o.hi[0]), fmaxf(mine.hi[1], o.hi[1]), fmaxf(mine.hi[2], o.hi[2])};
+            const uint32_t mv = mine.moved & o.moved;
+            store_node(t, node, rec, esc, lo, hi, mv);
+            for (int k = 0; k < 3; ++k)
+            {
+                mine.lo[k] = lo[k];
+                mine.hi[k] = hi[k];
+            }
+            mine.moved = mv;
+            cur = node;
             node = t.parent[node];
         }
     }
@@ -553,7 +584,9 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     if (n >= 2)
     {
         launch_fill_u32(t.visit, 0u, n - 1, st);
-        k_refit<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t);
+        launch_fill_u64(counters + C_DEFER, 0ull, 1, st);
+        k_refit<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t, counters + C_DEFER);
+        k_refit_top<<<1, kRefitThreads, 0, st>>>(t, counters + C_DEFER);
     }
 }
 

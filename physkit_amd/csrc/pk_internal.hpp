@@ -120,6 +120,7 @@ struct lbvh_dev
     uint32_t *first;     // n-1: leaf range of the node
     uint32_t *last;      // n-1
     uint32_t *lastf;     // n-1: last | (every leaf below is moved) << 31
+    uint4 *nrec;         // n-1: (left, right, first, last) in one record for the refit climb
     float4 *inode_lo;    // n-1: FP32 box min rounded down, w = left child bits
     float4 *inode_hi;    // n-1: FP32 box max rounded up, w = escape bits
     uint32_t *parent;    // 2n-1
