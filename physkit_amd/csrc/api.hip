@@ -181,6 +181,7 @@ struct pkg_ctx
     dbuf<double> rows_s;      // solver rows in solve order
     dbuf<uint8_t> row_valid_s;
     dbuf<unsigned long long> claim, used; // colouring: per-body claims, colours taken per body
+    dbuf<uint32_t> clist;                 // colouring: the row manifolds
     uint32_t n_colours = 0;
 
     // joints, dense in key order; an add or remove rebuilds the records into the other set
@@ -376,13 +377,14 @@ void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
     launch_fill_u32(jcol.p, kUncoloured, J, st);
     used.reserve(std::max<uint32_t>(n, 1));
     launch_fill_u64(used.p, 0ull, n, st);
+    launch_fill_u64(claim.p, 0ull, n, st);
     reset_counter(C_NCOL);
     for (uint32_t round = 0;; ++round)
     {
-        launch_fill_u64(claim.p, ~0ull, n, st);
-        launch_colour_claim(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, st);
+        launch_colour_claim(jd.keys, nullptr, J, b, jcol.p, nullptr, nullptr, claim.p, round, st);
         reset_counter(C_REMAIN);
-        launch_colour_assign(jd.keys, J, b, jcol.p, nullptr, nullptr, claim.p, used.p, round, counters.p, st);
+        launch_colour_assign(jd.keys, nullptr, J, b, jcol.p, nullptr, nullptr, claim.p, used.p, round, counters.p,
+                             st);
         if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
         if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
     }
@@ -641,16 +643,21 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                     pb = pair_base.p;
                 }
                 used.reserve(std::max<uint32_t>(n, 1));
+                clist.reserve(Pn);
                 launch_fill_u64(used.p, 0ull, n, st);
+                launch_fill_u64(claim.p, 0ull, n, st);
                 reset_counter(C_NCOL);
+                reset_counter(C_NLIST);
+                launch_colour_list(colour.p, Pn, clist.p, counters.p, st);
+                const uint32_t nl = static_cast<uint32_t>(read_counter(C_NLIST));
                 // the host looks at the remaining count every kColourCheck rounds; rounds
                 // after the last manifold is coloured change nothing
                 for (uint32_t round = 0;; ++round)
                 {
-                    launch_fill_u64(claim.p, ~0ull, n, st);
-                    launch_colour_claim(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, st);
+                    launch_colour_claim(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, round, st);
                     reset_counter(C_REMAIN);
-                    launch_colour_assign(keys[nxt].p, Pn, b, colour.p, wo, pb, claim.p, used.p, round, counters.p, st);
+                    launch_colour_assign(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, used.p, round,
+                                         counters.p, st);
                     if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
                     if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }

@@ -293,8 +293,13 @@ __global__ void __launch_bounds__(kBlock)
     wave_sum3(&counters[C_ROWS], nrows);
 }
 
-__device__ __forceinline__ uint64_t prio(uint32_t p)
-{ return (static_cast<uint64_t>(mix32(p)) << 32) | p; }
+// Claim of a manifold in a colouring round: the round (from 1) above the complement of
+// mix32(p). mix32 is a bijection, so ordering by it equals ordering by the priority
+// (mix32(p) << 32) | p; an atomic max keeps this round's lowest priority at each body,
+// and claims left from earlier rounds lose to any claim of the current one, so the
+// claim array is cleared once per colouring, not every round.
+__device__ __forceinline__ uint64_t claim_of(uint32_t p, uint32_t round)
+{ return (static_cast<uint64_t>(round + 1u) << 32) | (0xFFFFFFFFu - mix32(p)); }
 
 __device__ __forceinline__ uint32_t local_index(uint32_t p, uint32_t ia, const uint32_t *world_of,
                                                 const uint32_t *pair_base)
@@ -309,33 +314,47 @@ __global__ void k_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32
     if (p == 0 || world_of[static_cast<uint32_t>(keys[p - 1] >> 32)] != w) pair_base[w] = p;
 }
 
-__global__ void k_claim(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, const uint32_t *colour,
-                        const uint32_t *world_of, const uint32_t *pair_base, unsigned long long *claim)
+// the items coloured: list[0, n) (or 0..n-1 when list is null)
+__global__ void k_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, bodies_dev b, const uint32_t *colour,
+                        const uint32_t *world_of, const uint32_t *pair_base, unsigned long long *claim,
+                        uint32_t round)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs || colour[p] != kUncoloured) return;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = list ? list[i] : i;
+    if (colour[p] != kUncoloured) return;
     uint64_t key = keys[p];
     uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-    unsigned long long pr = prio(local_index(p, ia, world_of, pair_base));
-    if (!b.is_static[ia]) atomicMin(&claim[ia], pr);
-    if (!b.is_static[ib]) atomicMin(&claim[ib], pr);
+    unsigned long long pr = claim_of(local_index(p, ia, world_of, pair_base), round);
+    if (!b.is_static[ia]) atomicMax(&claim[ia], pr);
+    if (!b.is_static[ib]) atomicMax(&claim[ib], pr);
+}
+
+// the row manifolds (colour kUncoloured after contact setup), listed for the rounds
+__global__ void k_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool rows = p < n_pairs && colour[p] == kUncoloured;
+    unsigned long long slot = wave_append(&counters[C_NLIST], rows);
+    if (rows) list[slot] = p;
 }
 
 // A manifold holding the claim at each of its dynamic bodies takes the smallest colour
 // that no coloured manifold at those bodies holds (`used`: colours 0-63 taken at each
 // body), or 64 + the round number when all 64 are taken. Manifolds coloured in one
 // round share no dynamic body, so their choices are independent.
-__global__ void k_assign(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, uint32_t *colour,
+__global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, bodies_dev b, uint32_t *colour,
                          const uint32_t *world_of, const uint32_t *pair_base, const unsigned long long *claim,
                          unsigned long long *used, uint32_t round, unsigned long long *counters)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool remain = false;
-    if (p < n_pairs && colour[p] == kUncoloured)
+    uint32_t p = i < n ? (list ? list[i] : i) : 0u;
+    if (i < n && colour[p] == kUncoloured)
     {
         uint64_t key = keys[p];
         uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        unsigned long long pr = prio(local_index(p, ia, world_of, pair_base));
+        unsigned long long pr = claim_of(local_index(p, ia, world_of, pair_base), round);
         const bool sa = b.is_static[ia] != 0, sb = b.is_static[ib] != 0;
         bool ok = (sa || claim[ia] == pr) && (sb || claim[ib] == pr);
         if (ok)
@@ -936,20 +955,25 @@ void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *wo
 {
     if (n_pairs) k_pair_base<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, world_of, pair_base);
 }
-void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                         unsigned long long *claim, hipStream_t st)
+void launch_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters,
+                        hipStream_t st)
 {
-    if (n_pairs) k_claim<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim);
+    if (n_pairs) k_colour_list<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, list, counters);
 }
-void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
+                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                         unsigned long long *claim, uint32_t round, hipStream_t st)
+{
+    if (n) k_claim<<<grid(n), kBlock, 0, st>>>(keys, list, n, b, colour, world_of, pair_base, claim, round);
+}
+void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, unsigned long long *used, uint32_t round,
                           unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs)
-        k_assign<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, colour, world_of, pair_base, claim, used,
-                                                   round, counters);
+    if (n)
+        k_assign<<<grid(n), kBlock, 0, st>>>(keys, list, n, b, colour, world_of, pair_base, claim, used, round,
+                                             counters);
 }
 uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
                                uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,

@@ -77,7 +77,8 @@ enum counter_slot
     C_KEEP = 13,
     C_ISL_MAX = 14,
     C_NCOL = 15,
-    C_NCOUNTERS = 16
+    C_NLIST = 16,
+    C_NCOUNTERS = 20
 };
 
 // ---------------- k_body.hip ----------------
@@ -210,12 +211,16 @@ void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_d
 // world_of and pair_base are null for a single world.
 void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
                       hipStream_t st);
-void launch_colour_claim(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+// Colouring rounds over a list of items (null: all n). The claim array is cleared
+// once before the first round; `used` (colours 0-63 taken at each body) too; each round
+// the claim holders take the smallest colour free at their dynamic bodies, and C_NCOL
+// ends as the colour count.
+void launch_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters,
+                        hipStream_t st); // the row manifolds; count in C_NLIST
+void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
                          const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                         unsigned long long *claim, hipStream_t st);
-// one colouring round: claim holders take the smallest colour free at their dynamic
-// bodies (`used`, zeroed before the first round; C_NCOL ends as the colour count)
-void launch_colour_assign(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
+                         unsigned long long *claim, uint32_t round, hipStream_t st);
+void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, unsigned long long *used, uint32_t round,
                           unsigned long long *counters, hipStream_t st);
