@@ -265,7 +265,14 @@ __global__ void __launch_bounds__(kBlock)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t count = 0;
-    if (p < n_pairs)
+    // a pair with no old contacts and no new one stays empty: no body state is read
+    const bool live = p < n_pairs && (g.status[p] == 1 || (src[p] != kNone && prev.count[src[p]] > 0));
+    if (p < n_pairs && !live)
+    {
+        next.count[p] = 0;
+        events[p] = 0;
+    }
+    if (live)
     {
         const double distance2_eps = 0.005 * 0.005; // :212
         const double breaking = 0.05;               // :213
