@@ -138,9 +138,9 @@ struct pkg_ctx
 
     // broadphase
     dbuf<uint64_t> morton, morton_alt;
-    dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, visit, range_first, range_last, lastf, defer;
+    dbuf<uint32_t> ids, ids_alt, parent, escape, leaf_body, range_first, range_last, lastf;
     dbuf<int32_t> left, right;
-    dbuf<float4> inode_lo, inode_hi;
+    dbuf<float4> inode_lo, inode_hi, span_box, span_tab;
     dbuf<uint4> nrec;
     dbuf<double> leaf_box;
     dbuf<uint8_t> leaf_moved, covered;
@@ -340,8 +340,10 @@ lbvh_dev pkg_ctx::tree()
     t.leaf_box = leaf_box.p;
     t.leaf_body = leaf_body.p;
     t.leaf_moved = leaf_moved.p;
-    t.visit = visit.p;
-    t.defer = defer.p;
+    t.n_chunks = refit_chunks(n);
+    t.span_levels = refit_levels(n);
+    t.span_box = span_box.p;
+    t.span_tab = span_tab.p;
     t.scene = scene.p;
     t.n_worlds = n_worlds;
     t.wbits = id_bits(n_worlds);
@@ -951,9 +953,9 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->escape.reserve(2 * static_cast<size_t>(n));
         ctx->leaf_body.reserve(n);
         ctx->leaf_moved.reserve(n);
-        ctx->visit.reserve(n);
         for (auto *b : {&ctx->range_first, &ctx->range_last, &ctx->lastf}) b->reserve(n);
-        ctx->defer.reserve(2 * static_cast<size_t>(n));
+        ctx->span_box.reserve(4 * static_cast<size_t>(n));
+        ctx->span_tab.reserve(2 * static_cast<size_t>(refit_levels(n)) * refit_chunks(n));
         ctx->inode_lo.reserve(n);
         ctx->inode_hi.reserve(n);
         ctx->nrec.reserve(n);

@@ -6,7 +6,7 @@
 // The reference keeps two incremental SAH trees (src/bvh.cpp:235-514); the pair SET
 // does not depend on tree shape, so the device rebuilds a linear BVH every step:
 //   63-bit Morton keys of leaf centroids -> radix sort -> Karras hierarchy ->
-//   two-phase bottom-up refit -> escape indices for stackless traversal.
+//   escape indices for stackless traversal -> chunked bottom-up refit.
 // Internal nodes hold FP32 boxes rounded outward (conservative, 32 bytes per node
 // with the child and escape links); every leaf test uses the exact FP64 leaf box, so
 // each overlap decision is the reference's aabb::intersects.
@@ -17,7 +17,7 @@ namespace pk
 namespace
 {
 constexpr int kBlock = 256;
-constexpr int kChunk = 1024; // leaves per refit workgroup
+constexpr int kChunk = static_cast<int>(kRefitChunk); // leaves per refit workgroup
 constexpr int kRefitThreads = 1024;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
@@ -265,6 +265,7 @@ struct refit_lds
     uint32_t visit[kChunk];
     float box[kChunk][6];
     uint8_t moved[kChunk];
+    fbox wave_tot[2][kRefitThreads / 64]; // per-wavefront totals of the chunk scans
 };
 
 __device__ __forceinline__ fbox child_box_local(const lbvh_dev &t, uint32_t c, uint32_t c_lo, const refit_lds &L)
@@ -282,16 +283,75 @@ __device__ __forceinline__ fbox child_box_local(const lbvh_dev &t, uint32_t c, u
     return f;
 }
 
+__device__ __forceinline__ fbox empty_fbox()
+{
+    fbox f;
+    for (int k = 0; k < 3; ++k)
+    {
+        f.lo[k] = __builtin_inff();
+        f.hi[k] = -__builtin_inff();
+    }
+    f.moved = 1;
+    return f;
+}
+__device__ __forceinline__ void merge_into(fbox &a, const fbox &b)
+{
+    for (int k = 0; k < 3; ++k)
+    {
+        a.lo[k] = fminf(a.lo[k], b.lo[k]);
+        a.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    a.moved &= b.moved;
+}
+__device__ __forceinline__ void put_fbox(float4 *lo, float4 *hi, size_t i, const fbox &f)
+{
+    lo[i] = make_float4(f.lo[0], f.lo[1], f.lo[2], __uint_as_float(f.moved));
+    hi[i] = make_float4(f.hi[0], f.hi[1], f.hi[2], 0.0f);
+}
+__device__ __forceinline__ fbox get_fbox(const float4 *lo, const float4 *hi, size_t i)
+{
+    const float4 a = lo[i], b = hi[i];
+    fbox f;
+    f.lo[0] = a.x;
+    f.lo[1] = a.y;
+    f.lo[2] = a.z;
+    f.hi[0] = b.x;
+    f.hi[1] = b.y;
+    f.hi[2] = b.z;
+    f.moved = __float_as_uint(a.w);
+    return f;
+}
+// inclusive scan of boxes across the wavefront: toward higher lanes (UP) or lower
+template <bool UP> __device__ __forceinline__ fbox scan_wave(fbox f)
+{
+    const int lane = static_cast<int>(__lane_id());
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        fbox o;
+        for (int k = 0; k < 3; ++k)
+        {
+            o.lo[k] = UP ? __shfl_up(f.lo[k], off) : __shfl_down(f.lo[k], off);
+            o.hi[k] = UP ? __shfl_up(f.hi[k], off) : __shfl_down(f.hi[k], off);
+        }
+        o.moved = UP ? __shfl_up(f.moved, off) : __shfl_down(f.moved, off);
+        if (UP ? lane >= off : lane + off < 64) merge_into(f, o);
+    }
+    return f;
+}
 
-// Bottom-up refit in two launches. The second child to arrive at a node computes it.
-// Launch 1: a workgroup owns leaves [b*kChunk, (b+1)*kChunk) and refits the nodes whose
+// Bottom-up refit. The second child to arrive at a node computes it.
+// k_refit: a workgroup owns leaves [b*kChunk, (b+1)*kChunk) and refits the nodes whose
 // leaf range lies inside the chunk, synchronising only within the workgroup
-// (workgroup-scope acquire/release on LDS counters). A climb that reaches a node
-// spanning chunks stops there and lists its child (the top of a finished subtree).
-// Launch 2: one workgroup climbs from the listed tops through the spanning nodes to
-// the root; the launch boundary makes launch 1's boxes visible, so no arrival needs a
-// device-scope fence. The climbing thread carries its child's box in registers and
-// reads one record (children, leaf range) per level and only the sibling's box.
+// (workgroup-scope acquire/release on LDS counters). A climb stops at the first node
+// whose range crosses the chunk. The workgroup also writes, for every leaf of the chunk,
+// the union of the chunk's leaves up to it (prefix) and from it on (suffix), and the
+// chunk's total at level 0 of a sparse table over chunks.
+// k_span_table: one workgroup fills the table's levels (level k, chunk c: the union of
+// chunks [c, c + 2^k)).
+// k_span: a node whose range [lo, hi] crosses chunks is the suffix at lo, the prefix at
+// hi and at most two table entries covering the whole chunks between, so every such
+// node is computed on its own, with no arrivals and no device-scope fences. Min and
+// max are exact, so each box equals the union of its leaves' boxes in any order.
 __device__ __forceinline__ void store_node(const lbvh_dev &t, uint32_t node, const uint4 &rec, uint32_t esc,
                                            const float *lo, const float *hi, uint32_t mv)
 {
@@ -300,83 +360,107 @@ __device__ __forceinline__ void store_node(const lbvh_dev &t, uint32_t node, con
     t.lastf[node] = rec.w | (mv << 31);
 }
 
-__global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t, unsigned long long *n_tops)
+__global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
 {
+    static_assert(kRefitThreads == kChunk, "one thread per leaf of the chunk");
     __shared__ refit_lds L;
     uint32_t n = t.n;
     uint32_t c_lo = blockIdx.x * kChunk;
     uint32_t c_hi = min(c_lo + kChunk, n) - 1;
-    for (uint32_t i = threadIdx.x; i < kChunk; i += blockDim.x) L.visit[i] = 0;
+    const uint32_t tid = threadIdx.x;
+    L.visit[tid] = 0;
+    // chunk prefix and suffix unions of the leaf boxes
+    const uint32_t i = c_lo + tid;
+    const bool valid = i <= c_hi;
+    const fbox leaf = valid ? child_box(t, n - 1 + i) : empty_fbox();
+    fbox pre = scan_wave<true>(leaf);
+    fbox suf = scan_wave<false>(leaf);
+    const int wave = static_cast<int>(tid >> 6), lane = static_cast<int>(tid & 63);
+    if (lane == 63) L.wave_tot[0][wave] = pre;
+    if (lane == 0) L.wave_tot[1][wave] = suf;
     __syncthreads();
-    for (uint32_t i = c_lo + threadIdx.x; i <= c_hi; i += blockDim.x)
+    for (int w = 0; w < wave; ++w) merge_into(pre, L.wave_tot[0][w]);
+    for (int w = wave + 1; w < kRefitThreads / 64; ++w) merge_into(suf, L.wave_tot[1][w]);
+    if (valid)
     {
-        uint32_t cur = n - 1 + i;
-        fbox mine = child_box(t, cur);
-        uint32_t node = t.parent[cur];
-        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+        put_fbox(t.span_box, t.span_box + n, i, pre);
+        put_fbox(t.span_box + 2 * static_cast<size_t>(n), t.span_box + 3 * static_cast<size_t>(n), i, suf);
+        if (i == c_hi) put_fbox(t.span_tab, t.span_tab + t.span_levels * t.n_chunks, blockIdx.x, pre);
+    }
+    if (!valid) return;
+    uint32_t cur = n - 1 + i;
+    fbox mine = leaf;
+    uint32_t node = t.parent[cur];
+    for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+    {
+        const uint4 rec = t.nrec[node];
+        if (rec.z < c_lo || rec.w > c_hi) break; // spanning node: k_span computes it
+        const uint32_t esc = t.escape[node];
+        if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
+        const uint32_t sib = rec.x == cur ? rec.y : rec.x;
+        const fbox o = child_box_local(t, sib, c_lo, L);
+        float lo[3] = {fminf(mine.lo[0], o.lo[0]), fminf(mine.lo[1], o.lo[1]), fminf(mine.lo[2], o.lo[2])};
+        float hi[3] = {fmaxf(mine.hi[0], o.hi[0]), fmaxf(mine.hi[1], o.hi[1]), fmaxf(mine.hi[2], o.hi[2])};
+        const uint32_t mv = mine.moved & o.moved;
+        float *lb = L.box[node - c_lo];
+        for (int k = 0; k < 3; ++k)
         {
-            const uint4 rec = t.nrec[node];
-            if (rec.z < c_lo || rec.w > c_hi)
-            {
-                t.defer[atomicAdd(n_tops, 1ull)] = cur; // spanning node: finished in launch 2
-                break;
-            }
-            const uint32_t esc = t.escape[node];
-            if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
-            const uint32_t sib = rec.x == cur ? rec.y : rec.x;
-            const fbox o = child_box_local(t, sib, c_lo, L);
-            float lo[3] = {fminf(mine.lo[0], o.lo[0]), fminf(mine.lo[1], o.lo[1]), fminf(mine.lo[2], o.lo[2])};
-            float hi[3] = {fmaxf(mine.hi[0], o.hi[0]), fmaxf(mine.hi[1], o.hi[1]), fmaxf(mine.hi[2], o.hi[2])};
-            const uint32_t mv = mine.moved & o.moved;
-            float *lb = L.box[node - c_lo];
-            for (int k = 0; k < 3; ++k)
-            {
-                lb[k] = lo[k];
-                lb[3 + k] = hi[k];
-            }
-            L.moved[node - c_lo] = static_cast<uint8_t>(mv);
-            store_node(t, node, rec, esc, lo, hi, mv);
-            for (int k = 0; k < 3; ++k)
-            {
-                mine.lo[k] = lo[k];
-                mine.hi[k] = hi[k];
-            }
-            mine.moved = mv;
-            cur = node;
-            node = t.parent[node];
+            lb[k] = lo[k];
+            lb[3 + k] = hi[k];
+        }
+        L.moved[node - c_lo] = static_cast<uint8_t>(mv);
+        store_node(t, node, rec, esc, lo, hi, mv);
+        for (int k = 0; k < 3; ++k)
+        {
+            mine.lo[k] = lo[k];
+            mine.hi[k] = hi[k];
+        }
+        mine.moved = mv;
+        cur = node;
+        node = t.parent[node];
+    }
+}
+
+__global__ void __launch_bounds__(kRefitThreads) k_span_table(lbvh_dev t)
+{
+    const uint32_t nch = t.n_chunks;
+    float4 *lo = t.span_tab, *hi = t.span_tab + static_cast<size_t>(t.span_levels) * nch;
+    for (uint32_t k = 1; k < t.span_levels; ++k)
+    {
+        __syncthreads(); // level k - 1 is complete
+        const uint32_t half = 1u << (k - 1);
+        const size_t src = static_cast<size_t>(k - 1) * nch, dst = static_cast<size_t>(k) * nch;
+        for (uint32_t c = threadIdx.x; c + 2 * half <= nch; c += blockDim.x)
+        {
+            fbox a = get_fbox(lo, hi, src + c);
+            merge_into(a, get_fbox(lo, hi, src + c + half));
+            put_fbox(lo, hi, dst + c, a);
         }
     }
 }
 
-__global__ void __launch_bounds__(kRefitThreads) k_refit_top(lbvh_dev t, const unsigned long long *n_tops)
+__global__ void __launch_bounds__(kBlock) k_span(lbvh_dev t)
 {
-    const uint32_t n = t.n;
-    const uint32_t m = static_cast<uint32_t>(*n_tops);
-    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x)
+    const uint32_t n = t.n, nch = t.n_chunks;
+    const float4 *tlo = t.span_tab, *thi = t.span_tab + static_cast<size_t>(t.span_levels) * nch;
+    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node < n - 1)
     {
-        uint32_t cur = t.defer[k];
-        fbox mine = child_box(t, cur);
-        uint32_t node = t.parent[cur];
-        for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
+        const uint4 rec = t.nrec[node];
+        if (rec.z / kChunk == rec.w / kChunk) return; // inside one chunk: k_refit's
+        const uint32_t esc = t.escape[node];
+        fbox f = get_fbox(t.span_box + 2 * static_cast<size_t>(n), t.span_box + 3 * static_cast<size_t>(n), rec.z);
+        merge_into(f, get_fbox(t.span_box, t.span_box + n, rec.w));
+        const uint32_t c0 = rec.z / kChunk + 1, c1 = rec.w / kChunk; // whole chunks [c0, c1)
+        if (c0 < c1)
         {
-            const uint4 rec = t.nrec[node];
-            const uint32_t esc = t.escape[node];
-            if (wg_arrive(&t.visit[node]) == 0) break;
-            const uint32_t sib = rec.x == cur ? rec.y : rec.x;
-            const fbox o = child_box(t, sib);
-            float lo[3] = {fminf(mine.lo[0], o.lo[0]), fminf(mine.lo[1], o.lo[1]), fminf(mine.lo[2], o.lo[2])};
-            float hi[3] = {fmaxf(mine.hi[0], o.hi[0]), fmaxf(mine.hi[1], o.hi[1]), fmaxf(mine.hi[2], o.hi[2])};
-            const uint32_t mv = mine.moved & o.moved;
-            store_node(t, node, rec, esc, lo, hi, mv);
-            for (int k = 0; k < 3; ++k)
-            {
-                mine.lo[k] = lo[k];
-                mine.hi[k] = hi[k];
-            }
-            mine.moved = mv;
-            cur = node;
-            node = t.parent[node];
+            const uint32_t len = c1 - c0;
+            const uint32_t lv = 31u - static_cast<uint32_t>(__clz(len));
+            const size_t row = static_cast<size_t>(lv) * nch;
+            merge_into(f, get_fbox(tlo, thi, row + c0));
+            merge_into(f, get_fbox(tlo, thi, row + c1 - (1u << lv)));
         }
+        store_node(t, node, rec, esc, f.lo, f.hi, f.moved);
     }
 }
 
@@ -583,10 +667,12 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     k_escape<<<grid(2 * n - 1), kBlock, 0, st>>>(t);
     if (n >= 2)
     {
-        launch_fill_u32(t.visit, 0u, n - 1, st);
-        launch_fill_u64(counters + C_DEFER, 0ull, 1, st);
-        k_refit<<<grid(n, kChunk), kRefitThreads, 0, st>>>(t, counters + C_DEFER);
-        k_refit_top<<<1, kRefitThreads, 0, st>>>(t, counters + C_DEFER);
+        k_refit<<<t.n_chunks, kRefitThreads, 0, st>>>(t);
+        if (t.n_chunks > 1)
+        {
+            if (t.span_levels > 1) k_span_table<<<1, kRefitThreads, 0, st>>>(t);
+            k_span<<<grid(n - 1), kBlock, 0, st>>>(t);
+        }
     }
 }
 

@@ -73,7 +73,7 @@ enum counter_slot
     C_EPA_BIG_FAIL = 9,
     C_EVENTS = 10,
     C_TREE_FAIL = 11,
-    C_DEFER = 12,
+    // 12: unused
     C_KEEP = 13,
     C_ISL_MAX = 14,
     C_NCOL = 15,
@@ -129,8 +129,12 @@ struct lbvh_dev
     double *leaf_box;    // 6n: exact FP64 leaf bounds, leaf-major
     uint32_t *leaf_body; // n: body id of leaf i
     uint8_t *leaf_moved; // n: leaf queries this step (dynamic and moved)
-    uint32_t *visit;     // n-1 refit arrival counters
-    uint32_t *defer;     // 2n: refit arrivals at nodes spanning workgroup chunks
+    // refit chunks of kRefitChunk leaves: per leaf, the union of its chunk's leaf boxes
+    // up to it and from it on (4n: prefix lo, prefix hi, suffix lo, suffix hi; w of lo =
+    // every-leaf-moved flag), and a sparse table over chunk totals (2 * levels * chunks)
+    uint32_t n_chunks, span_levels;
+    float4 *span_box;
+    float4 *span_tab;
     unsigned long long *scene; // 6 ordered-integer bounds of leaf centroids
     // batched independent worlds (n_worlds > 1): bodies [wstart[w], wstart[w+1]) form
     // world w; Morton keys carry the world in their top wbits bits, so world w's leaves
@@ -140,6 +144,14 @@ struct lbvh_dev
     const uint32_t *world_of; // n: world of each body
     const uint32_t *wstart;   // n_worlds + 1
 };
+constexpr uint32_t kRefitChunk = 1024;
+inline uint32_t refit_chunks(uint32_t n) { return (n + kRefitChunk - 1) / kRefitChunk; }
+inline uint32_t refit_levels(uint32_t n)
+{
+    uint32_t c = refit_chunks(n), l = 1;
+    while ((2u << (l - 1)) <= c) ++l;
+    return l;
+}
 void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
                        unsigned long long *counters, hipStream_t st);
 // candidate keys (pairs with at least one moved leaf) appended at counters[C_COMBINED];
