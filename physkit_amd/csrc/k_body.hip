@@ -107,7 +107,7 @@ __global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy
         st3(b.torque, b.n, i, mk(0.0, 0.0, 0.0));
     }
     if (i < b.n) b.moved[i] = moved ? 1 : 0;
-    wave_append(&counters[C_MOVED], moved);
+    block_add(&counters[C_MOVED], moved ? 1u : 0u);
 }
 
 // src/world.cpp:50-55, integrate.h:42-46, particle.h:40-44

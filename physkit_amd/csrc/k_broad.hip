@@ -468,8 +468,8 @@ __global__ void __launch_bounds__(kBlock) k_span(lbvh_dev t)
 // j > i, or when j is not a moved leaf (j then never queries): each pair once. A
 // subtree whose leaves all lie at or below i and are all moved holds no such j.
 // Pairs collect in a per-thread buffer; the output space is reserved once per
-// wavefront after the walk (a full buffer is flushed on its own), so the shared
-// cursor sees a few thousand atomics instead of one per emitting iteration.
+// workgroup after the walk (a full buffer is flushed on its own), so the shared
+// cursor sees one atomic per 256 leaves instead of one per emitting iteration.
 constexpr int kEmitBuf = 32;
 
 __device__ __forceinline__ void flush_own(uint64_t *buf, uint32_t cnt, uint64_t *out_keys, uint32_t cap,
@@ -560,18 +560,8 @@ __global__ void __launch_bounds__(kBlock)
         }
         node = next;
     }
-    // converged again: one reservation per wavefront for the buffered pairs
-    int lane = __lane_id();
-    uint32_t incl = nbuf;
-    for (int off = 1; off < 64; off <<= 1)
-    {
-        uint32_t v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    uint32_t total = __shfl(incl, 63);
-    unsigned long long base = 0;
-    if (lane == 63 && total) base = atomicAdd(&counters[C_COMBINED], static_cast<unsigned long long>(total));
-    base = __shfl(base, 63) + (incl - nbuf);
+    // converged again: one reservation per workgroup for the buffered pairs
+    const unsigned long long base = block_reserve(&counters[C_COMBINED], nbuf);
     for (uint32_t k = 0; k < nbuf; ++k)
         if (base + k < cap) out_keys[base + k] = buf[k];
 }

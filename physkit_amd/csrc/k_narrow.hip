@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
         if (gl == 0) g.status[p] = 0;
     }
     bool lead = hit && gl == 0;
-    unsigned long long slot = wave_append(&counters[C_HITS], lead);
+    unsigned long long slot = block_append(&counters[C_HITS], lead);
     if (lead)
     {
         g.hit_list[slot] = p;
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(kBlock)
         events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
         count = static_cast<uint32_t>(nm.n);
     }
-    wave_sum3(&counters[C_CONTACTS], count);
+    block_add(&counters[C_CONTACTS], count);
 }
 
 // ---- standalone batch ----

@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         colour[p] = nrows ? kUncoloured : kNone;
     }
-    wave_sum3(&counters[C_ROWS], nrows);
+    block_add(&counters[C_ROWS], nrows);
 }
 
 // Claim of a manifold in a colouring round: the round (from 1) above the complement of
@@ -335,7 +335,7 @@ __global__ void k_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool rows = p < n_pairs && colour[p] == kUncoloured;
-    unsigned long long slot = wave_append(&counters[C_NLIST], rows);
+    unsigned long long slot = block_append(&counters[C_NLIST], rows);
     if (rows) list[slot] = p;
 }
 
@@ -349,6 +349,7 @@ __global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n,
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool remain = false;
+    uint32_t ncol = 0;
     uint32_t p = i < n ? (list ? list[i] : i) : 0u;
     if (i < n && colour[p] == kUncoloured)
     {
@@ -367,12 +368,13 @@ __global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n,
                 if (!sa) atomicOr(&used[ia], 1ull << k);
                 if (!sb) atomicOr(&used[ib], 1ull << k);
             }
-            atomicMax(&counters[C_NCOL], static_cast<unsigned long long>(k) + 1ull);
+            ncol = k + 1;
         }
         else
             remain = true;
     }
-    wave_append(&counters[C_REMAIN], remain);
+    block_add(&counters[C_REMAIN], remain ? 1u : 0u);
+    block_max(&counters[C_NCOL], ncol);
 }
 
 // colour buckets by a stable radix sort of (colour, manifold index); manifolds
