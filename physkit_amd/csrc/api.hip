@@ -296,6 +296,14 @@ struct pkg_ctx
         return m;
     }
 
+    // counters [lo, hi] in one copy and one synchronisation
+    void read_counters(int lo, int hi)
+    {
+        hip_check(hipMemcpyAsync(h_counters + lo, counters.p + lo, sizeof(unsigned long long) * (hi - lo + 1),
+                                 hipMemcpyDeviceToHost, st),
+                  "counter copy");
+        hip_check(hipStreamSynchronize(st), "stream sync");
+    }
     unsigned long long read_counter(int slot)
     {
         hip_check(hipMemcpyAsync(h_counters + slot, counters.p + slot, sizeof(unsigned long long),
@@ -508,8 +516,9 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         uint32_t cap = static_cast<uint32_t>(std::min<size_t>(comb_keys.cap, 0xFFFFFFF0u));
         reset_counter(C_COMBINED);
         launch_pair_query(t, b, comb_keys.p, cap, counters.p, st);
-        unsigned long long cnt = read_counter(C_COMBINED);
-        if (read_counter(C_TREE_FAIL)) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
+        read_counters(C_COMBINED, C_TREE_FAIL);
+        const unsigned long long cnt = h_counters[C_COMBINED];
+        if (h_counters[C_TREE_FAIL]) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
         if (cnt <= cap)
         {
             M = static_cast<uint32_t>(cnt);

@@ -36,16 +36,14 @@ __global__ void __launch_bounds__(kThreads) k_rs_hist(const uint64_t *keys, uint
     hist[static_cast<size_t>(tid) * nblocks + blockIdx.x] = h[tid];
 }
 
-__global__ void __launch_bounds__(kThreads)
-    k_rs_scatter(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint32_t n,
-                 int shift, const uint32_t *offs, uint32_t nblocks)
+// Stable scatter of one tile of kTile keys starting at `base`. run[d]: the output
+// position of the tile's first key with digit d (advanced in place); wc: LDS workspace.
+__device__ __forceinline__ void scatter_tile(const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
+                                             uint32_t *vout, uint32_t n, int shift, size_t base, uint32_t *run,
+                                             uint32_t (*wc)[256])
 {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[4][256];
     int tid = threadIdx.x;
     int wave = tid >> 6;
-    run[tid] = offs[static_cast<size_t>(tid) * nblocks + blockIdx.x];
-    size_t base = static_cast<size_t>(blockIdx.x) * kTile;
     for (int it = 0; it < kItems; ++it)
     {
         size_t idx = base + static_cast<size_t>(it) * kThreads + tid;
@@ -87,6 +85,16 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_rs_scatter(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint32_t n,
+                 int shift, const uint32_t *offs, uint32_t nblocks)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[4][256];
+    run[threadIdx.x] = offs[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x];
+    scatter_tile(kin, vin, kout, vout, n, shift, static_cast<size_t>(blockIdx.x) * kTile, run, wc);
 }
 
 // ---- scan ----
@@ -168,6 +176,66 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// n <= kTile: every pass in one workgroup (histogram, scan of the 256 digit counts,
+// scatter), the passes ping-ponging through the two global buffers with workgroup
+// barriers between them: one launch instead of five per pass
+struct shift_list
+{
+    int s[16];
+    int n;
+};
+__global__ void __launch_bounds__(kThreads)
+    k_rs_small(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, uint32_t n, shift_list sl)
+{
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[4][256];
+    const int tid = threadIdx.x;
+    bool alt = false;
+    for (int p = 0; p < sl.n; ++p)
+    {
+        const uint64_t *ki = alt ? k_alt : k;
+        const uint32_t *vi = alt ? v_alt : v;
+        uint64_t *ko = alt ? k : k_alt;
+        uint32_t *vo = alt ? v : v_alt;
+        const int shift = sl.s[p];
+        h[tid] = 0;
+        __syncthreads();
+        for (int it = 0; it < kItems; ++it)
+        {
+            uint32_t idx = static_cast<uint32_t>(it * kThreads + tid);
+            if (idx < n) atomicAdd(&h[(ki[idx] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        uint32_t tot;
+        run[tid] = block_excl_scan(h[tid], &tot);
+        __syncthreads();
+        scatter_tile(ki, vi, ko, vo, n, shift, 0, run, wc); // ends with a barrier
+        alt = !alt;
+    }
+}
+
+// n <= kScanTile: the three scan launches in one
+__global__ void __launch_bounds__(kThreads) k_scan_small(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total)
+{
+    const uint32_t base = threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, &tot);
+    for (int k = 0; k < kScanItems; ++k)
+    {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
 uint32_t nblk(size_t n, size_t tile) { return static_cast<uint32_t>((n + tile - 1) / tile); }
 
 void check(hipError_t e)
@@ -181,8 +249,12 @@ size_t scan_block_sums(uint32_t n) { return nblk(n, kScanTile) + 1; }
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *block_sums,
                         uint32_t *total, hipStream_t st)
 {
+    if (n <= static_cast<uint32_t>(kScanTile))
+    {
+        k_scan_small<<<1, kThreads, 0, st>>>(in, out, n, total);
+        return;
+    }
     uint32_t nb = nblk(n, kScanTile);
-    if (nb == 0) nb = 1;
     k_scan_reduce<<<nb, kThreads, 0, st>>>(in, n, block_sums);
     k_scan_top<<<1, kThreads, 0, st>>>(block_sums, nb, total);
     k_scan_down<<<nb, kThreads, 0, st>>>(in, out, n, block_sums);
@@ -194,6 +266,14 @@ bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, 
                     const int *shifts, int n_shifts, sort_scratch &sc, hipStream_t st)
 {
     if (n <= 1 || n_shifts == 0) return false;
+    if (n <= static_cast<uint32_t>(kTile) && n_shifts <= 16)
+    {
+        shift_list sl{};
+        for (int p = 0; p < n_shifts; ++p) sl.s[p] = shifts[p];
+        sl.n = n_shifts;
+        k_rs_small<<<1, kThreads, 0, st>>>(k, v, k_alt, v_alt, n, sl);
+        return (n_shifts & 1) != 0;
+    }
     uint32_t nb = nblk(n, kTile);
     size_t entries = static_cast<size_t>(256) * nb;
     size_t need = 2 * entries + scan_block_sums(static_cast<uint32_t>(entries)) + 1;
