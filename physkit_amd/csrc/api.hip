@@ -124,7 +124,7 @@ struct pkg_ctx
     uint32_t n_shapes = 0, n_verts = 0;
     bool has_uv = false; // some shape is a tagged UV sphere
     std::vector<uint32_t> shape_cost; // support candidates per shape: 12 for a tagged UV sphere, else its vertices
-    dbuf<uint32_t> shape_off, shape_uv;
+    dbuf<uint32_t> shape_off, shape_voff, shape_uv;
     dbuf<double> verts, shape_box;
 
     uint32_t n = 0;
@@ -282,6 +282,7 @@ struct pkg_ctx
         shapes_dev s;
         s.n = n_shapes;
         s.off = shape_off.p;
+        s.voff = shape_voff.p;
         s.verts = verts.p;
         s.local_box = shape_box.p;
         s.uv = shape_uv.p;
@@ -886,7 +887,22 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
         ctx->shape_cost.resize(n_shapes);
         for (uint32_t i = 0; i < n_shapes; ++i) ctx->shape_cost[i] = uv[i] ? 12u : offsets[i + 1] - offsets[i];
         upload(ctx->shape_off, offsets, n_shapes + 1, ctx->st);
-        upload(ctx->verts, verts, static_cast<size_t>(nv) * 3, ctx->st);
+        // device vertex blocks: even start, even count (an odd block repeats its last
+        // vertex, which is never scanned), so vertex pairs load as aligned 16-byte words
+        std::vector<uint32_t> voff(std::max<uint32_t>(n_shapes, 1), 0u);
+        std::vector<double> pv;
+        pv.reserve(3 * (static_cast<size_t>(nv) + n_shapes));
+        for (uint32_t i = 0; i < n_shapes; ++i)
+        {
+            voff[i] = static_cast<uint32_t>(pv.size() / 3);
+            const uint32_t c = offsets[i + 1] - offsets[i];
+            const double *src = verts + 3 * static_cast<size_t>(offsets[i]);
+            pv.insert(pv.end(), src, src + 3 * static_cast<size_t>(c));
+            if (c & 1u) pv.insert(pv.end(), src + 3 * static_cast<size_t>(c - 1), src + 3 * static_cast<size_t>(c));
+        }
+        if (pv.empty()) pv.assign(6, 0.0);
+        upload(ctx->shape_voff, voff.data(), voff.size(), ctx->st);
+        upload(ctx->verts, pv.data(), pv.size(), ctx->st);
         ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
         ctx->n_shapes = n_shapes;
         ctx->n_verts = nv;

@@ -14,7 +14,7 @@ __global__ void k_shape_boxes(shapes_dev s) // aabb::from_points, bounds.h:33-49
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.n) return;
-    uint32_t b = s.off[i], e = s.off[i + 1];
+    uint32_t b = s.voff[i], e = b + (s.off[i + 1] - s.off[i]);
     v3 p0 = mk(s.verts[3 * b], s.verts[3 * b + 1], s.verts[3 * b + 2]);
     box3 r;
     r.min = p0;

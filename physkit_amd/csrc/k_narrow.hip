@@ -28,10 +28,10 @@ template <bool UV> __device__ __forceinline__ gshape body_shape(const bodies_dev
     g.b = mk(0.0, 0.0, 0.0);
     g.q = ldq(b.quat, b.n, id);
     uint32_t sh = b.shape[id];
-    uint32_t o = s.off[sh];
-    g.v = s.verts + 3 * static_cast<size_t>(o);
-    g.nv = s.off[sh + 1] - o;
+    g.v = s.verts + 3 * static_cast<size_t>(s.voff[sh]);
+    g.nv = s.off[sh + 1] - s.off[sh];
     g.uv = UV ? s.uv[sh] : 0u;
+    g.paired = 1;
     return g;
 }
 
@@ -354,6 +354,7 @@ __device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint3
     g.v = nullptr;
     g.nv = 0;
     g.uv = 0;
+    g.paired = 0;
     g.b = mk(0.0, 0.0, 0.0);
     g.q = mkq(1.0, 0.0, 0.0, 0.0);
     if (g.kind == 0)

@@ -29,7 +29,8 @@ struct gshape
     qt q;
     const double *v;
     uint32_t nv;
-    uint32_t uv; // mesh built as a UV sphere: (stacks << 16) | sectors, else 0
+    uint32_t uv;     // mesh built as a UV sphere: (stacks << 16) | sectors, else 0
+    uint32_t paired; // v is 16-byte aligned and readable up to an even vertex count
 };
 
 __device__ __forceinline__ v3 ldv(const double *v, uint32_t i) { return mk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
@@ -128,16 +129,45 @@ template <int G> __device__ __forceinline__ uint32_t support_idx(const gshape &s
         double bv = 0.0;
         uint32_t bi = kNoIdx;
         bool nan0 = false;
-        for (uint32_t i = static_cast<uint32_t>(gl); i < s.nv; i += G)
+        if (s.paired)
         {
-            double t = dot(ldv(s.v, i), ld);
-            if (i == 0) nan0 = t != t;
-            if (t == t && (bi == kNoIdx || t > bv))
+            // lane gl scans vertex pairs gl, gl+G, ...: both vertices of a pair in three
+            // aligned 16-byte loads, taken in index order
+            const uint32_t np = (s.nv + 1) >> 1;
+            for (uint32_t j = static_cast<uint32_t>(gl); j < np; j += G)
             {
-                bv = t;
-                bi = i;
+                const double2 *q = reinterpret_cast<const double2 *>(s.v + 6 * static_cast<size_t>(j));
+                const double2 a = q[0], b = q[1], c = q[2];
+                const uint32_t i = 2 * j;
+                double t = dot(mk(a.x, a.y, b.x), ld);
+                if (i == 0) nan0 = t != t;
+                if (t == t && (bi == kNoIdx || t > bv))
+                {
+                    bv = t;
+                    bi = i;
+                }
+                if (i + 1 < s.nv)
+                {
+                    t = dot(mk(b.y, c.x, c.y), ld);
+                    if (t == t && (bi == kNoIdx || t > bv))
+                    {
+                        bv = t;
+                        bi = i + 1;
+                    }
+                }
             }
         }
+        else
+            for (uint32_t i = static_cast<uint32_t>(gl); i < s.nv; i += G)
+            {
+                double t = dot(ldv(s.v, i), ld);
+                if (i == 0) nan0 = t != t;
+                if (t == t && (bi == kNoIdx || t > bv))
+                {
+                    bv = t;
+                    bi = i;
+                }
+            }
         for (int m = 1; m < G; m <<= 1)
         {
             double ov = __shfl_xor(bv, m, G);
@@ -185,6 +215,7 @@ __device__ __forceinline__ gshape sel_shape(bool c, const gshape &a, const gshap
     r.v = c ? a.v : b.v;
     r.nv = c ? a.nv : b.nv;
     r.uv = c ? a.uv : b.uv;
+    r.paired = c ? a.paired : b.paired;
     return r;
 }
 

@@ -41,8 +41,12 @@ struct bodies_dev
 struct shapes_dev
 {
     uint32_t n;
-    const uint32_t *off; // n+1 vertex offsets
-    const double *verts; // xyz interleaved (a support scan reads one shape's vertices in order)
+    const uint32_t *off; // n+1 vertex offsets (the caller's numbering: counts)
+    const uint32_t *voff; // n: first vertex of each shape's block in verts. Blocks start at
+                          // an even vertex (16-byte aligned) and hold an even count, the
+                          // last vertex repeated when odd, so a scan reads vertex pairs
+                          // as three aligned 16-byte loads
+    const double *verts;  // xyz interleaved
     double *local_box;   // 6 planes over shapes
     const uint32_t *uv;  // n: UV-sphere tag ((stacks << 16) | sectors) or 0
 };
