@@ -12,22 +12,24 @@ namespace pk
 namespace
 {
 constexpr int kThreads = 256;
-constexpr int kItems = 16;
-constexpr int kTile = kThreads * kItems; // keys per block
+constexpr int kItems = 16;               // keys per thread and tile, at most
+constexpr int kTile = kThreads * kItems; // keys per tile, at most
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
+// a tile is kThreads * items keys; fewer items per thread give more tiles (workgroups)
+// when n is small, so a pass fills the device
 __global__ void __launch_bounds__(kThreads) k_rs_hist(const uint64_t *keys, uint32_t n, int shift,
-                                                      uint32_t *hist, uint32_t nblocks)
+                                                      uint32_t *hist, uint32_t nblocks, int items)
 {
     __shared__ uint32_t h[256];
     int tid = threadIdx.x;
     h[tid] = 0;
     __syncthreads();
-    size_t base = static_cast<size_t>(blockIdx.x) * kTile;
-    for (int it = 0; it < kItems; ++it)
+    size_t base = static_cast<size_t>(blockIdx.x) * kThreads * items;
+    for (int it = 0; it < items; ++it)
     {
         size_t idx = base + static_cast<size_t>(it) * kThreads + tid;
         if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
@@ -36,15 +38,16 @@ __global__ void __launch_bounds__(kThreads) k_rs_hist(const uint64_t *keys, uint
     hist[static_cast<size_t>(tid) * nblocks + blockIdx.x] = h[tid];
 }
 
-// Stable scatter of one tile of kTile keys starting at `base`. run[d]: the output
-// position of the tile's first key with digit d (advanced in place); wc: LDS workspace.
+// Stable scatter of one tile of kThreads * items keys starting at `base`. run[d]: the
+// output position of the tile's first key with digit d (advanced in place); wc: LDS
+// workspace.
 __device__ __forceinline__ void scatter_tile(const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
-                                             uint32_t *vout, uint32_t n, int shift, size_t base, uint32_t *run,
-                                             uint32_t (*wc)[256])
+                                             uint32_t *vout, uint32_t n, int shift, size_t base, int items,
+                                             uint32_t *run, uint32_t (*wc)[256])
 {
     int tid = threadIdx.x;
     int wave = tid >> 6;
-    for (int it = 0; it < kItems; ++it)
+    for (int it = 0; it < items; ++it)
     {
         size_t idx = base + static_cast<size_t>(it) * kThreads + tid;
         bool valid = idx < n;
@@ -89,12 +92,13 @@ __device__ __forceinline__ void scatter_tile(const uint64_t *kin, const uint32_t
 
 __global__ void __launch_bounds__(kThreads)
     k_rs_scatter(const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint32_t n,
-                 int shift, const uint32_t *offs, uint32_t nblocks)
+                 int shift, const uint32_t *offs, uint32_t nblocks, int items)
 {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wc[4][256];
     run[threadIdx.x] = offs[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x];
-    scatter_tile(kin, vin, kout, vout, n, shift, static_cast<size_t>(blockIdx.x) * kTile, run, wc);
+    scatter_tile(kin, vin, kout, vout, n, shift, static_cast<size_t>(blockIdx.x) * kThreads * items, items, run,
+                 wc);
 }
 
 // ---- scan ----
@@ -210,7 +214,7 @@ __global__ void __launch_bounds__(kThreads)
         uint32_t tot;
         run[tid] = block_excl_scan(h[tid], &tot);
         __syncthreads();
-        scatter_tile(ki, vi, ko, vo, n, shift, 0, run, wc); // ends with a barrier
+        scatter_tile(ki, vi, ko, vo, n, shift, 0, kItems, run, wc); // ends with a barrier
         alt = !alt;
     }
 }
@@ -260,7 +264,15 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
     k_scan_down<<<nb, kThreads, 0, st>>>(in, out, n, block_sums);
 }
 
-size_t radix_hist_entries(uint32_t n) { return static_cast<size_t>(256) * nblk(n, kTile); }
+// keys per thread for a pass over n keys: the most (up to kItems) that still give at
+// least kFillTiles tiles
+constexpr uint32_t kFillTiles = 512;
+int tile_items(uint32_t n)
+{
+    int items = kItems;
+    while (items > 1 && nblk(n, static_cast<size_t>(kThreads) * items) < kFillTiles) items >>= 1;
+    return items;
+}
 
 bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, uint32_t n,
                     const int *shifts, int n_shifts, sort_scratch &sc, hipStream_t st)
@@ -274,7 +286,8 @@ bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, 
         k_rs_small<<<1, kThreads, 0, st>>>(k, v, k_alt, v_alt, n, sl);
         return (n_shifts & 1) != 0;
     }
-    uint32_t nb = nblk(n, kTile);
+    const int items = tile_items(n);
+    uint32_t nb = nblk(n, static_cast<size_t>(kThreads) * items);
     size_t entries = static_cast<size_t>(256) * nb;
     size_t need = 2 * entries + scan_block_sums(static_cast<uint32_t>(entries)) + 1;
     if (sc.hist_cap < need)
@@ -295,9 +308,9 @@ bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, 
         const uint32_t *vi = alt ? v_alt : v;
         uint64_t *ko = alt ? k : k_alt;
         uint32_t *vo = alt ? v : v_alt;
-        k_rs_hist<<<nb, kThreads, 0, st>>>(ki, n, shifts[p], hist, nb);
+        k_rs_hist<<<nb, kThreads, 0, st>>>(ki, n, shifts[p], hist, nb, items);
         exclusive_scan_u32(hist, hscan, static_cast<uint32_t>(entries), bsums, total, st);
-        k_rs_scatter<<<nb, kThreads, 0, st>>>(ki, vi, ko, vo, n, shifts[p], hscan, nb);
+        k_rs_scatter<<<nb, kThreads, 0, st>>>(ki, vi, ko, vo, n, shifts[p], hscan, nb, items);
         alt = !alt;
     }
     return alt;

@@ -104,7 +104,6 @@ struct sort_scratch
 };
 bool radix_sort_u64(uint64_t *k, uint32_t *v, uint64_t *k_alt, uint32_t *v_alt, uint32_t n,
                     const int *shifts, int n_shifts, sort_scratch &sc, hipStream_t st);
-size_t radix_hist_entries(uint32_t n);
 // exclusive scan of n uint32 flags into out, total written to *total (device)
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *block_sums,
                         uint32_t *total, hipStream_t st);
