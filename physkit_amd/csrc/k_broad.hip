@@ -5,7 +5,7 @@
 //                  && (active(a,b) || moved_a || moved_b)
 // The reference keeps two incremental SAH trees (src/bvh.cpp:235-514); the pair SET
 // does not depend on tree shape, so the device rebuilds a linear BVH every step:
-//   63-bit Morton keys of leaf centroids -> radix sort -> Karras hierarchy ->
+//   48-bit Morton keys of leaf centroids -> radix sort -> Karras hierarchy ->
 //   escape indices for stackless traversal -> chunked bottom-up refit.
 // Internal nodes hold FP32 boxes rounded outward (conservative, 32 bytes per node
 // with the child and escape links); every leaf test uses the exact FP64 leaf box, so
@@ -128,12 +128,12 @@ __global__ void k_morton(bodies_dev b, lbvh_dev t, const unsigned long long *sce
         double v = (b.fat[static_cast<size_t>(k) * b.n + i] + b.fat[static_cast<size_t>(k + 3) * b.n + i]) * 0.5;
         double u = hi > lo ? (v - lo) / (hi - lo) : 0.0;
         u = fmin(fmax(u, 0.0), 1.0);
-        uint64_t x = static_cast<uint64_t>(u * 2097152.0);
-        q[k] = x > 2097151ull ? 2097151ull : x;
+        uint64_t x = static_cast<uint64_t>(u * 65536.0);
+        q[k] = x > 65535ull ? 65535ull : x;
     }
     uint64_t m = (split3(q[0]) << 2) | (split3(q[1]) << 1) | split3(q[2]);
     if (t.n_worlds > 1) // world id in the top bits: each world's leaves sort together
-        m = (static_cast<uint64_t>(t.world_of[i]) << (63 - t.wbits)) | (m >> t.wbits);
+        m = (static_cast<uint64_t>(t.world_of[i]) << (kMortonBits - t.wbits)) | (m >> t.wbits);
     keys[i] = m;
     ids[i] = i;
 }
@@ -642,8 +642,8 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     unsigned sb = grid(n) < 512 ? grid(n) : 512;
     k_scene_bounds<<<sb, kBlock, 0, st>>>(b, t.scene);
     k_morton<<<grid(n), kBlock, 0, st>>>(b, t, t.scene, t.morton, t.ids);
-    static const int shifts[8] = {0, 8, 16, 24, 32, 40, 48, 56};
-    if (radix_sort_u64(t.morton, t.ids, t.morton_alt, t.ids_alt, n, shifts, 8, sc, st))
+    static const int shifts[kMortonBits / 8] = {0, 8, 16, 24, 32, 40};
+    if (radix_sort_u64(t.morton, t.ids, t.morton_alt, t.ids_alt, n, shifts, kMortonBits / 8, sc, st))
     {
         uint64_t *tk = t.morton;
         t.morton = t.morton_alt;

@@ -154,7 +154,7 @@ __global__ void k_ray_nodes(lbvh_dev t, uint4 *rn)
     {
         int bit = 63 - __clzll(static_cast<long long>(x));
         int wb = t.n_worlds > 1 ? t.wbits : 0;
-        if (bit < 63 - wb) axis = static_cast<uint32_t>(2 - (bit + wb) % 3);
+        if (bit < kMortonBits - wb) axis = static_cast<uint32_t>(2 - (bit + wb) % 3);
     }
     rn[i] = make_uint4(lc, rc, t.parent[i], axis);
 }

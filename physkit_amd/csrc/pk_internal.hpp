@@ -147,6 +147,9 @@ struct lbvh_dev
     const uint32_t *world_of; // n: world of each body
     const uint32_t *wstart;   // n_worlds + 1
 };
+// Morton keys: 16 bits per axis (bit 3k+2 x, 3k+1 y, 3k z), six 8-bit sort passes; a
+// batched context puts the world in the top bits and drops as many low Morton bits
+constexpr int kMortonBits = 48;
 constexpr uint32_t kRefitChunk = 1024;
 inline uint32_t refit_chunks(uint32_t n) { return (n + kRefitChunk - 1) / kRefitChunk; }
 inline uint32_t refit_levels(uint32_t n)
