@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -135,6 +136,11 @@ struct pkg_ctx
 
     dbuf<unsigned long long> counters;
     unsigned long long *h_counters = nullptr;
+    uint32_t candidates = 0; // candidate pairs of the last step
+    // leaf count above which the pair query writes kPairRegions regions; the environment
+    // variable PKG_PAIR_REGION_MIN_LEAVES, read at pkg_create, overrides it (tests use it
+    // to take the region path on small scenes)
+    uint32_t region_min_leaves = 1u << 20;
 
     // broadphase
     dbuf<uint64_t> morton, morton_alt;
@@ -509,30 +515,51 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     const uint32_t Pprev = P[cur];
     const int nxt = 1 - cur;
     // candidates: pairs with at least one moved leaf, each once
-    uint32_t M = 0;
+    // Above about a million leaves (16,384 wavefronts, each reserving once) the
+    // reservations would serialise on one cursor: the query then writes kPairRegions
+    // regions of comb_keys_alt, concatenated into comb_keys afterwards.
+    const uint32_t nreg = n > region_min_leaves ? kPairRegions : 1u;
+    dbuf<uint64_t> &qout = nreg > 1 ? comb_keys_alt : comb_keys;
+    uint32_t M = 0, rcap = 0;
+    region_counts rc{};
     for (int attempt = 0;; ++attempt)
     {
-        size_t want = std::max<size_t>(comb_keys.cap, 4 * static_cast<size_t>(n) + 1024);
-        comb_keys.reserve(want);
-        uint32_t cap = static_cast<uint32_t>(std::min<size_t>(comb_keys.cap, 0xFFFFFFF0u));
-        reset_counter(C_COMBINED);
-        launch_pair_query(t, b, comb_keys.p, cap, counters.p, st);
-        read_counters(C_COMBINED, C_TREE_FAIL);
-        const unsigned long long cnt = h_counters[C_COMBINED];
+        size_t want = std::max<size_t>(qout.cap, 4 * static_cast<size_t>(n) + 1024 * nreg);
+        qout.reserve(want);
+        rcap = static_cast<uint32_t>(std::min<size_t>(qout.cap / nreg, 0xFFFFFFF0u / nreg));
+        launch_fill_u64(counters.p + C_QCUR, 0ull, static_cast<size_t>(nreg) * kCursorStride, st);
+        launch_pair_query(t, b, qout.p, rcap, nreg, counters.p, st);
+        read_counters(C_TREE_FAIL, C_QCUR + static_cast<int>((nreg - 1) * kCursorStride));
         if (h_counters[C_TREE_FAIL]) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
-        if (cnt <= cap)
+        unsigned long long cnt = 0, most = 0;
+        for (uint32_t r = 0; r < nreg; ++r)
+        {
+            const unsigned long long c = h_counters[C_QCUR + r * kCursorStride];
+            rc.off[r] = static_cast<uint32_t>(std::min<unsigned long long>(cnt, 0xFFFFFFFFull));
+            cnt += c;
+            most = std::max(most, c);
+        }
+        if (cnt > 0xFFFFFFF0ull) throw pk_error(PKG_OVERFLOW_PAIRS, "candidate pair count exceeds 32-bit indexing");
+        if (most <= rcap)
         {
             M = static_cast<uint32_t>(cnt);
+            for (uint32_t r = nreg; r <= kPairRegions; ++r) rc.off[r] = M;
             break;
         }
-        if (cnt > 0xFFFFFFF0ull || attempt > 2)
-            throw pk_error(PKG_OVERFLOW_PAIRS, "candidate pair count exceeds 32-bit indexing");
-        comb_keys.release();
-        comb_keys.reserve(static_cast<size_t>(cnt + cnt / 4 + 1024));
+        if (attempt > 2) throw pk_error(PKG_OVERFLOW_PAIRS, "candidate pair buffer could not be sized");
+        qout.release();
+        qout.reserve(static_cast<size_t>(nreg) * static_cast<size_t>(most + most / 4 + 1024));
     }
+    candidates = M;
     comb_vals.reserve(M);
-    comb_keys_alt.reserve(M);
     comb_vals_alt.reserve(M);
+    if (nreg > 1)
+    {
+        comb_keys.reserve(std::max<uint32_t>(M, 1));
+        launch_pair_concat(comb_keys_alt.p, rcap, rc, comb_keys.p, st);
+    }
+    else
+        comb_keys_alt.reserve(M);
     uint64_t *ck = comb_keys.p;
     if (M > 1)
     {
@@ -799,7 +826,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     last.rows = h_counters[C_ROWS];
     last.colours = n_colours;
     last.epa_overflow = h_counters[C_EPA_OVF];
-    last.candidates = h_counters[C_COMBINED];
+    last.candidates = candidates;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -840,6 +867,8 @@ pkg_status pkg_create(pkg_ctx **out, const pkg_config *cfg)
         hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), C_NCOUNTERS * sizeof(unsigned long long)),
                   "pinned alloc");
         c->counters.reserve(C_NCOUNTERS);
+        if (const char *e = std::getenv("PKG_PAIR_REGION_MIN_LEAVES"))
+            c->region_min_leaves = static_cast<uint32_t>(std::strtoul(e, nullptr, 10));
         c->scene.reserve(6);
     }
     catch (const pk_error &e)

@@ -467,9 +467,11 @@ __global__ void __launch_bounds__(kBlock) k_span(lbvh_dev t)
 // one thread per sorted leaf i; a moved dynamic leaf emits its pair with leaf j when
 // j > i, or when j is not a moved leaf (j then never queries): each pair once. A
 // subtree whose leaves all lie at or below i and are all moved holds no such j.
-// Pairs collect in a per-thread buffer; the output space is reserved once per
-// workgroup after the walk (a full buffer is flushed on its own), so the shared
-// cursor sees one atomic per 256 leaves instead of one per emitting iteration.
+// Pairs collect in a per-thread buffer; the output space is reserved after the walk
+// (a full buffer is flushed on its own). Same-address atomics serialise across the
+// device (about 12 ns each): with one output region the workgroup reserves once; on
+// large scenes workgroups write to several regions with a cursor each and reserve per
+// wavefront, so no wavefront waits for its workgroup's longest walk.
 constexpr int kEmitBuf = 32;
 
 __device__ __forceinline__ void flush_own(uint64_t *buf, uint32_t cnt, uint64_t *out_keys, uint32_t cap,
@@ -481,8 +483,12 @@ __device__ __forceinline__ void flush_own(uint64_t *buf, uint32_t cnt, uint64_t 
 }
 
 __global__ void __launch_bounds__(kBlock)
-    k_pair_query(lbvh_dev t, bodies_dev b, uint64_t *out_keys, uint32_t cap, unsigned long long *counters)
+    k_pair_query(lbvh_dev t, bodies_dev b, uint64_t *regions, uint32_t cap, uint32_t n_regions,
+                 unsigned long long *counters)
 {
+    const uint32_t region = blockIdx.x % n_regions;
+    uint64_t *out_keys = regions + static_cast<size_t>(region) * cap;
+    unsigned long long *cursor = counters + C_QCUR + region * kCursorStride;
     uint64_t buf[kEmitBuf];
     uint32_t nbuf = 0;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,7 +557,7 @@ __global__ void __launch_bounds__(kBlock)
                 {
                     if (nbuf == kEmitBuf)
                     {
-                        flush_own(buf, nbuf, out_keys, cap, &counters[C_COMBINED]);
+                        flush_own(buf, nbuf, out_keys, cap, cursor);
                         nbuf = 0;
                     }
                     buf[nbuf++] = pair_key(a, t.leaf_body[j]);
@@ -560,8 +566,25 @@ __global__ void __launch_bounds__(kBlock)
         }
         node = next;
     }
-    // converged again: one reservation per workgroup for the buffered pairs
-    const unsigned long long base = block_reserve(&counters[C_COMBINED], nbuf);
+    // converged again: one reservation for the buffered pairs, per workgroup with one
+    // region (fewer atomics), per wavefront with several (no wait for the workgroup's
+    // longest walk)
+    unsigned long long base = 0;
+    if (n_regions == 1)
+        base = block_reserve(cursor, nbuf);
+    else
+    {
+        int lane = __lane_id();
+        uint32_t incl = nbuf;
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        uint32_t total = __shfl(incl, 63);
+        if (lane == 63 && total) base = atomicAdd(cursor, static_cast<unsigned long long>(total));
+        base = __shfl(base, 63) + (incl - nbuf);
+    }
     for (uint32_t k = 0; k < nbuf; ++k)
         if (base + k < cap) out_keys[base + k] = buf[k];
 }
@@ -666,11 +689,27 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     }
 }
 
+__global__ void k_pair_concat(const uint64_t *regions, uint32_t rcap, region_counts rc, uint64_t *dst)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rc.off[kPairRegions]) return;
+    uint32_t r = 0;
+    while (e >= rc.off[r + 1]) ++r;
+    dst[e] = regions[static_cast<size_t>(r) * rcap + (e - rc.off[r])];
+}
+
+void launch_pair_concat(const uint64_t *regions, uint32_t rcap, const region_counts &rc, uint64_t *dst,
+                        hipStream_t st)
+{
+    const uint32_t m = rc.off[kPairRegions];
+    if (m) k_pair_concat<<<grid(m), kBlock, 0, st>>>(regions, rcap, rc, dst);
+}
+
 void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_keys, uint32_t cap,
-                       unsigned long long *counters, hipStream_t st)
+                       uint32_t n_regions, unsigned long long *counters, hipStream_t st)
 {
     if (t.n < 2) return;
-    k_pair_query<<<grid(t.n), kBlock, 0, st>>>(t, b, out_keys, cap, counters);
+    k_pair_query<<<grid(t.n), kBlock, 0, st>>>(t, b, out_keys, cap, n_regions, counters);
 }
 
 void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,

@@ -67,7 +67,7 @@ enum counter_slot
 {
     C_MOVED = 0,
     C_CAND = 1,
-    C_COMBINED = 2,
+    // 2: unused
     C_PAIRS = 3,
     C_HITS = 4,
     C_CONTACTS = 5,
@@ -82,8 +82,14 @@ enum counter_slot
     C_ISL_MAX = 14,
     C_NCOL = 15,
     C_NLIST = 16,
-    C_NCOUNTERS = 20
+    // pair query cursors: one per output region, each in its own 256-byte line, so
+    // wavefronts reserving output space spread their atomics over kPairRegions
+    // addresses instead of serialising on one
+    C_QCUR = 32,
+    C_NCOUNTERS = 32 + 8 * 32
 };
+constexpr uint32_t kPairRegions = 8;
+constexpr uint32_t kCursorStride = 32;
 
 // ---------------- k_body.hip ----------------
 void launch_body_init(const bodies_dev &b, const shapes_dev &s, hipStream_t st);
@@ -160,10 +166,19 @@ inline uint32_t refit_levels(uint32_t n)
 }
 void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
                        unsigned long long *counters, hipStream_t st);
-// candidate keys (pairs with at least one moved leaf) appended at counters[C_COMBINED];
+// candidate keys (pairs with at least one moved leaf), appended per region (below);
 // a malformed tree is reported in counters[C_TREE_FAIL]
-void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_keys, uint32_t cap,
-                       unsigned long long *counters, hipStream_t st);
+// workgroup w writes its candidates to region r = w % n_regions: out_keys[r * rcap, ...)
+// with the count at counters[C_QCUR + r * kCursorStride]
+void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_keys, uint32_t rcap,
+                       uint32_t n_regions, unsigned long long *counters, hipStream_t st);
+// concatenates the regions' counts[r] keys into dst (off: exclusive prefix of counts)
+struct region_counts
+{
+    uint32_t off[kPairRegions + 1];
+};
+void launch_pair_concat(const uint64_t *regions, uint32_t rcap, const region_counts &rc, uint64_t *dst,
+                        hipStream_t st);
 // src[e] = index of candidate e in the sorted previous list (kNone if new); marks it covered
 void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev,
                        uint32_t *src, uint8_t *covered, hipStream_t st);
