@@ -169,7 +169,7 @@ struct pkg_ctx
     dbuf<uint8_t> events;
 
     // narrowphase
-    dbuf<uint32_t> hit_list, ovf_list;
+    dbuf<uint32_t> hit_list, ovf_list, ovf2_list;
     // lanes per pair in GJK: 2 when the bodies' shapes average at most 14 support
     // candidates (boxes, certified UV spheres, small hulls), else 4
     int gjk_group = 4;
@@ -633,12 +633,18 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         record(ev, 9, st);
         launch_epa(keys[nxt].p, hits, b, s, has_uv, g, ovf_list.p, counters.p, st);
         uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
-        if (novf) // rerun with the 4096-face store; capacity never changes a result
+        if (novf) // rerun with larger stores; capacity never changes a result
         {
-            big_pool.reserve(static_cast<size_t>(novf) * epa_big_bytes());
-            launch_epa_big(keys[nxt].p, ovf_list.p, novf, b, s, g, big_pool.p, counters.p, st);
-            if (read_counter(C_EPA_BIG_FAIL))
-                throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
+            ovf2_list.reserve(novf);
+            launch_epa_lds(keys[nxt].p, ovf_list.p, novf, b, s, g, ovf2_list.p, counters.p, st);
+            const uint32_t n2 = static_cast<uint32_t>(read_counter(C_EPA_OVF2));
+            if (n2)
+            {
+                big_pool.reserve(static_cast<size_t>(n2) * epa_big_bytes());
+                launch_epa_big(keys[nxt].p, ovf2_list.p, n2, b, s, g, big_pool.p, counters.p, st);
+                if (read_counter(C_EPA_BIG_FAIL))
+                    throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
+            }
         }
         record(ev, 5, st);
         launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p,

@@ -139,6 +139,34 @@ __global__ void __launch_bounds__(kEpaBlock)
     if (ovf) ovf_list[slot] = h;
 }
 
+// Second tier: one workgroup per hit that overflowed the private store, its polytope in
+// LDS (896 faces): LDS round trips instead of global-memory ones on the solve's long
+// chain of dependent accesses. A solve that needs more goes on to k_epa_big.
+__global__ void __launch_bounds__(64)
+    k_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b, shapes_dev s,
+              gjk_out_dev g, uint32_t *ovf2_list, unsigned long long *counters)
+{
+    __shared__ epa_store_lds store;
+    const uint32_t t = blockIdx.x;
+    if (t >= n_ovf || threadIdx.x != 0) return;
+    const uint32_t h = ovf_list[t];
+    const uint32_t p = g.hit_list[h];
+    const uint64_t key = keys[p];
+    gshape A = body_shape<true>(b, s, static_cast<uint32_t>(key >> 32));
+    gshape B = body_shape<true>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+    simplex x;
+    load_simplex(g.simplex, g.cap, h, x);
+    double out[10];
+    const int r = epa_solve<true>(A, B, x, store, out);
+    if (r == 3)
+    {
+        ovf2_list[atomicAdd(&counters[C_EPA_OVF2], 1ull)] = h;
+        return;
+    }
+    if (r == 1) write_info(g.info, g.pair_cap, p, out);
+    g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : 2);
+}
+
 template <bool UV>
 __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b,
                           shapes_dev s, gjk_out_dev g, epa_store_big *pool, unsigned long long *counters)
@@ -447,6 +475,13 @@ void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, cons
         k_epa<true><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
     else
         k_epa<false><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+}
+
+void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, const bodies_dev &b,
+                    const shapes_dev &s, const gjk_out_dev &g, uint32_t *ovf2_list, unsigned long long *counters,
+                    hipStream_t st)
+{
+    if (n_ovf) k_epa_lds<<<n_ovf, 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g, ovf2_list, counters);
 }
 
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,

@@ -502,7 +502,9 @@ template <int FC, int PC, class I> struct epa_store
     I stk[FC]; // silhouette stack
 };
 using epa_store_fast = epa_store<192, 68, uint8_t>;    // every hull solve seen so far fits
-using epa_store_big = epa_store<4096, 68, uint16_t>;   // overflow path, global memory
+using epa_store_lds = epa_store<896, 68, uint16_t>;    // overflow path, in LDS
+using epa_store_big = epa_store<4096, 68, uint16_t>;   // last overflow path, global memory
+static_assert(sizeof(epa_store_lds) <= 65536, "the LDS store must fit one workgroup's 64 KiB");
 
 template <class S> struct epa_solver
 {

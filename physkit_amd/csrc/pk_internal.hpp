@@ -77,7 +77,7 @@ enum counter_slot
     C_EPA_BIG_FAIL = 9,
     C_EVENTS = 10,
     C_TREE_FAIL = 11,
-    // 12: unused
+    C_EPA_OVF2 = 12, // overflows of the LDS store, rerun in global memory
     C_KEEP = 13,
     C_ISL_MAX = 14,
     C_NCOL = 15,
@@ -206,6 +206,11 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
                 gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
+// overflowed hits rerun with an LDS store; its own overflows are listed in ovf2_list
+// (count at counters[C_EPA_OVF2]) for launch_epa_big
+void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, const bodies_dev &b,
+                    const shapes_dev &s, const gjk_out_dev &g, uint32_t *ovf2_list, unsigned long long *counters,
+                    hipStream_t st);
 size_t epa_big_bytes();
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
                     const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,

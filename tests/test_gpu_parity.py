@@ -280,8 +280,10 @@ def test_batched_worlds_bitexact(pk, oracle):
         worlds.append((o, b0, b1))
     print(f"[parity] batched worlds sizes={np.diff(fb).tolist()}", flush=True)
     total_rows = 0
+    overflows = 0
     for k in range(90):
         g.step(s["dt"])
+        overflows += g.counters()["epa_overflow"]
         gb = g.bodies()
         gk = g.pairs()
         gc, gd = g.manifolds()
@@ -307,6 +309,7 @@ def test_batched_worlds_bitexact(pk, oracle):
             _assert_same((w, k), "colours", ck_col[cm], oc_col)
             total_rows += o.counters()["rows"]
     g.close()
+    print(f"[parity] batched worlds: EPA solves rerun in larger stores: {overflows}", flush=True)
     assert total_rows > 0
 
 
