@@ -126,7 +126,7 @@ struct pkg_ctx
     bool has_uv = false; // some shape is a tagged UV sphere
     std::vector<uint32_t> shape_cost; // support candidates per shape: 12 for a tagged UV sphere, else its vertices
     dbuf<uint32_t> shape_off, shape_voff, shape_uv;
-    dbuf<double> verts, shape_box;
+    dbuf<double> verts, shape_box, shape_rad;
 
     uint32_t n = 0;
     dbuf<double> pos, vel, ang, acc, torque, quat, mass, inv_mass, I_local, Iinv_local, Iw, Iw_inv, rest,
@@ -169,7 +169,7 @@ struct pkg_ctx
     dbuf<uint8_t> events;
 
     // narrowphase
-    dbuf<uint32_t> hit_list, ovf_list, ovf2_list;
+    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list;
     // lanes per pair in GJK: 2 when the bodies' shapes average at most 14 support
     // candidates (boxes, certified UV spheres, small hulls), else 4
     int gjk_group = 4;
@@ -290,6 +290,7 @@ struct pkg_ctx
         s.off = shape_off.p;
         s.voff = shape_voff.p;
         s.verts = verts.p;
+        s.rad = shape_rad.p;
         s.local_box = shape_box.p;
         s.uv = shape_uv.p;
         return s;
@@ -627,7 +628,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
     if (Pn)
     {
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, counters.p, st);
+        gjk_list.reserve(Pn);
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);
@@ -925,6 +927,7 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
         // device vertex blocks: even start, even count (an odd block repeats its last
         // vertex, which is never scanned), so vertex pairs load as aligned 16-byte words
         std::vector<uint32_t> voff(std::max<uint32_t>(n_shapes, 1), 0u);
+        std::vector<double> rad(std::max<uint32_t>(n_shapes, 1), 0.0);
         std::vector<double> pv;
         pv.reserve(3 * (static_cast<size_t>(nv) + n_shapes));
         for (uint32_t i = 0; i < n_shapes; ++i)
@@ -934,9 +937,16 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
             const double *src = verts + 3 * static_cast<size_t>(offsets[i]);
             pv.insert(pv.end(), src, src + 3 * static_cast<size_t>(c));
             if (c & 1u) pv.insert(pv.end(), src + 3 * static_cast<size_t>(c - 1), src + 3 * static_cast<size_t>(c));
+            for (uint32_t k = 0; k < c; ++k)
+            {
+                const double *q = src + 3 * static_cast<size_t>(k);
+                const double r = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                rad[i] = std::isfinite(r) ? std::max(rad[i], r) : std::numeric_limits<double>::infinity();
+            }
         }
         if (pv.empty()) pv.assign(6, 0.0);
         upload(ctx->shape_voff, voff.data(), voff.size(), ctx->st);
+        upload(ctx->shape_rad, rad.data(), rad.size(), ctx->st);
         upload(ctx->verts, pv.data(), pv.size(), ctx->st);
         ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
         ctx->n_shapes = n_shapes;

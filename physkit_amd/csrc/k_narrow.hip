@@ -35,6 +35,21 @@ template <bool UV> __device__ __forceinline__ gshape body_shape(const bodies_dev
     return g;
 }
 
+// Bodies whose bounding spheres (radius: the largest vertex distance from the shape
+// origin) are apart by more than 1e-6 m plus 1e-6 of the radii. Every point of the
+// Minkowski difference then has norm above 1e-6, so GJK's first-support test
+// (|p|^2 < 1e-12, collision.cpp:170) cannot fire, and with a gap this far above the
+// rounding of its normals the simplex cases report no enclosure (a tetrahedron is only
+// formed with the new point past the origin from the previous triangle's plane), so
+// gjk_collision returns false: the verdict is the reference's without running it.
+// A non-finite radius or position never compares apart.
+__device__ __forceinline__ bool spheres_apart(v3 pa, v3 pb, double r)
+{
+    const v3 d = pa - pb;
+    const double dist = sqrt(dot(d, d));
+    return dist > r + 1e-6 * r + 1e-6;
+}
+
 // GJK simplex hand-off, planar: slot 0 = point count, then per point p.xyz and the
 // two support features packed in one 64-bit word (kSimplexWords doubles per hit).
 __device__ __forceinline__ void st_spt(double *d, uint32_t cap, uint32_t slot, int k, const spt &p)
@@ -78,19 +93,41 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
 // in every lane of the group, and lane 0 writes the results. Fewer lanes suit shapes
 // with few support candidates (boxes, certified UV spheres): less redundant simplex
 // work per lane.
+// one thread per pair: a pair whose bounding spheres are apart is separated (status 0)
+// without GJK; the others are listed for k_gjk
+__global__ void __launch_bounds__(kBlock)
+    k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, uint8_t *status, uint32_t *list,
+               unsigned long long *counters)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    if (p < n_pairs)
+    {
+        const uint64_t key = keys[p];
+        const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        keep = !spheres_apart(ld3(b.pos, b.n, ia), ld3(b.pos, b.n, ib), s.rad[b.shape[ia]] + s.rad[b.shape[ib]]);
+        if (!keep) status[p] = 0;
+    }
+    const unsigned long long slot = block_append(&counters[C_GJK_LIST], keep);
+    if (keep) list[slot] = p;
+}
+
 template <int G, bool UV>
 __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
-    k_gjk(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, gjk_out_dev g,
+    k_gjk(const uint64_t *keys, const uint32_t *list, bodies_dev b, shapes_dev s, gjk_out_dev g,
           unsigned long long *counters)
 {
+    const uint32_t n_list = static_cast<uint32_t>(counters[C_GJK_LIST]);
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t p = t / G;
+    uint32_t i = t / G;
     int gl = static_cast<int>(t % G);
     bool hit = false;
     simplex x;
     x.n = 0;
-    if (p < n_pairs)
+    uint32_t p = 0;
+    if (i < n_list)
     {
+        p = list[i];
         uint64_t key = keys[p];
         gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
         gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
@@ -447,23 +484,25 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, unsigned long long *counters, hipStream_t st)
+                gjk_out_dev &g, uint32_t *list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
+    k_gjk_cull<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, g.status, list, counters);
+    // sized for every pair; groups past the listed count (read on the device) exit at once
     unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
     if (group == 2)
     {
         if (uv)
-            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
         else
-            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
     }
     else
     {
         if (uv)
-            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
         else
-            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, n_pairs, b, s, g, counters);
+            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
     }
 }
 

@@ -47,6 +47,7 @@ struct shapes_dev
                           // last vertex repeated when odd, so a scan reads vertex pairs
                           // as three aligned 16-byte loads
     const double *verts;  // xyz interleaved
+    const double *rad;    // n: largest vertex distance from the shape origin
     double *local_box;   // 6 planes over shapes
     const uint32_t *uv;  // n: UV-sphere tag ((stacks << 16) | sectors) or 0
 };
@@ -82,6 +83,7 @@ enum counter_slot
     C_ISL_MAX = 14,
     C_NCOL = 15,
     C_NLIST = 16,
+    C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
     // addresses instead of serialising on one
@@ -201,9 +203,11 @@ struct gjk_out_dev
     uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
     uint32_t pair_cap;
 };
-// GJK over every pair with `group` (2 or 4) lanes per pair
+// GJK over every pair with `group` (2 or 4) lanes per pair; pairs whose bounding
+// spheres are clearly apart are written separated first and the rest listed in `list`
+// (capacity n_pairs), so GJK wavefronts run only pairs that need it
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, unsigned long long *counters, hipStream_t st);
+                gjk_out_dev &g, uint32_t *list, unsigned long long *counters, hipStream_t st);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 // overflowed hits rerun with an LDS store; its own overflows are listed in ovf2_list
