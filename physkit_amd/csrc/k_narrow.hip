@@ -35,19 +35,83 @@ template <bool UV> __device__ __forceinline__ gshape body_shape(const bodies_dev
     return g;
 }
 
-// Bodies whose bounding spheres (radius: the largest vertex distance from the shape
-// origin) are apart by more than 1e-6 m plus 1e-6 of the radii. Every point of the
-// Minkowski difference then has norm above 1e-6, so GJK's first-support test
-// (|p|^2 < 1e-12, collision.cpp:170) cannot fire, and with a gap this far above the
-// rounding of its normals the simplex cases report no enclosure (a tetrahedron is only
-// formed with the new point past the origin from the previous triangle's plane), so
-// gjk_collision returns false: the verdict is the reference's without running it.
-// A non-finite radius or position never compares apart.
-__device__ __forceinline__ bool spheres_apart(v3 pa, v3 pb, double r)
+// GJK cull. Two bodies are apart when their bounding spheres (radius: the largest
+// vertex distance from the shape origin) or their oriented local boxes (the shape's
+// local AABB under the body's rotation; separating-axis test over 15 axes) are apart by
+// more than the margin 1e-6 m + 1e-6 of the radii. Every point of the Minkowski
+// difference then has norm above 1e-6, so GJK's first-support test (|p|^2 < 1e-12,
+// collision.cpp:170) cannot fire, and with a gap this far above the rounding of its
+// normals the simplex cases report no enclosure (a tetrahedron is only formed with the
+// new point past the origin from the previous triangle's plane), so gjk_collision
+// returns false: the verdict is the reference's without running it. The tests assume
+// the support map rotate(q, .) is a rotation: a quaternion whose squared norm is off 1
+// by more than 1e-9 (rotate then scales lengths by up to 1 + 2|q.q - 1|) is never
+// culled, nor is anything non-finite (every comparison with NaN is false).
+struct cull_body
 {
-    const v3 d = pa - pb;
-    const double dist = sqrt(dot(d, d));
-    return dist > r + 1e-6 * r + 1e-6;
+    v3 pos, c, e; // position, oriented box centre and half extents
+    v3 ax[3];     // rotate(q, unit axes)
+    double r, qn;
+};
+__device__ __forceinline__ cull_body cull_of(const bodies_dev &b, const shapes_dev &s, uint32_t id)
+{
+    cull_body o;
+    const uint32_t sh = b.shape[id];
+    const qt q = ldq(b.quat, b.n, id);
+    o.qn = ((q.w * q.w + q.x * q.x) + q.y * q.y) + q.z * q.z;
+    o.pos = ld3(b.pos, b.n, id);
+    o.r = s.rad[sh];
+    const box3 lb = ldbox(s.local_box, s.n, sh);
+    o.ax[0] = rotate(q, mk(1.0, 0.0, 0.0));
+    o.ax[1] = rotate(q, mk(0.0, 1.0, 0.0));
+    o.ax[2] = rotate(q, mk(0.0, 0.0, 1.0));
+    const v3 lc = (lb.min + lb.max) * 0.5;
+    o.e = (lb.max - lb.min) * 0.5;
+    o.c = o.pos + ((o.ax[0] * lc.x + o.ax[1] * lc.y) + o.ax[2] * lc.z);
+    return o;
+}
+// OBB-OBB separating axes (Ericson, Real-Time Collision Detection, 4.4.1); an edge
+// axis is not normalised and is at most unit length, so its test is conservative
+__device__ __forceinline__ bool obbs_apart(const cull_body &A, const cull_body &B, double m)
+{
+    double R[3][3], AR[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            R[i][j] = dot(A.ax[i], B.ax[j]);
+            AR[i][j] = fabs(R[i][j]) + 1e-12;
+        }
+    const v3 d = B.c - A.c;
+    const double t[3] = {dot(d, A.ax[0]), dot(d, A.ax[1]), dot(d, A.ax[2])};
+    const double ea[3] = {A.e.x, A.e.y, A.e.z}, eb[3] = {B.e.x, B.e.y, B.e.z};
+    for (int i = 0; i < 3; ++i)
+        if (fabs(t[i]) > ea[i] + ((eb[0] * AR[i][0] + eb[1] * AR[i][1]) + eb[2] * AR[i][2]) + m) return true;
+    for (int j = 0; j < 3; ++j)
+    {
+        const double tj = (t[0] * R[0][j] + t[1] * R[1][j]) + t[2] * R[2][j];
+        if (fabs(tj) > ((ea[0] * AR[0][j] + ea[1] * AR[1][j]) + ea[2] * AR[2][j]) + eb[j] + m) return true;
+    }
+    for (int i = 0; i < 3; ++i)
+    {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        for (int j = 0; j < 3; ++j)
+        {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const double ra = ea[i1] * AR[i2][j] + ea[i2] * AR[i1][j];
+            const double rb = eb[j1] * AR[i][j2] + eb[j2] * AR[i][j1];
+            if (fabs(t[i2] * R[i1][j] - t[i1] * R[i2][j]) > ra + rb + m) return true;
+        }
+    }
+    return false;
+}
+__device__ __forceinline__ bool bodies_apart(const cull_body &A, const cull_body &B)
+{
+    if (!(fabs(A.qn - 1.0) <= 1e-9 && fabs(B.qn - 1.0) <= 1e-9)) return false;
+    const double r = A.r + B.r;
+    const double m = 1e-6 * r + 1e-6;
+    const v3 d = A.pos - B.pos;
+    if (sqrt(dot(d, d)) > r + m) return true;
+    return obbs_apart(A, B, m);
 }
 
 // GJK simplex hand-off, planar: slot 0 = point count, then per point p.xyz and the
@@ -93,7 +157,7 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
 // in every lane of the group, and lane 0 writes the results. Fewer lanes suit shapes
 // with few support candidates (boxes, certified UV spheres): less redundant simplex
 // work per lane.
-// one thread per pair: a pair whose bounding spheres are apart is separated (status 0)
+// one thread per pair: a pair whose bodies are apart (above) is separated (status 0)
 // without GJK; the others are listed for k_gjk
 __global__ void __launch_bounds__(kBlock)
     k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, uint8_t *status, uint32_t *list,
@@ -104,8 +168,9 @@ __global__ void __launch_bounds__(kBlock)
     if (p < n_pairs)
     {
         const uint64_t key = keys[p];
-        const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        keep = !spheres_apart(ld3(b.pos, b.n, ia), ld3(b.pos, b.n, ib), s.rad[b.shape[ia]] + s.rad[b.shape[ib]]);
+        const cull_body A = cull_of(b, s, static_cast<uint32_t>(key >> 32));
+        const cull_body B = cull_of(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+        keep = !bodies_apart(A, B);
         if (!keep) status[p] = 0;
     }
     const unsigned long long slot = block_append(&counters[C_GJK_LIST], keep);
