@@ -152,11 +152,6 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-// GJK, G lanes per pair: the lanes of a group read consecutive vertices of the same
-// shape (coalesced) and reduce the support search; the simplex logic runs identically
-// in every lane of the group, and lane 0 writes the results. Fewer lanes suit shapes
-// with few support candidates (boxes, certified UV spheres): less redundant simplex
-// work per lane.
 // one thread per pair: a pair whose bodies are apart (above) is separated (status 0)
 // without GJK; the others are listed for k_gjk
 __global__ void __launch_bounds__(kBlock)
@@ -177,6 +172,11 @@ __global__ void __launch_bounds__(kBlock)
     if (keep) list[slot] = p;
 }
 
+// GJK over the listed pairs, G lanes per pair: the lanes of a group read consecutive
+// vertex pairs of the same shape and reduce the support search; the simplex logic runs
+// identically in every lane of the group, and lane 0 writes the results. Fewer lanes
+// suit shapes with few support candidates (boxes, certified UV spheres): less redundant
+// simplex work per lane.
 template <int G, bool UV>
 __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
     k_gjk(const uint64_t *keys, const uint32_t *list, bodies_dev b, shapes_dev s, gjk_out_dev g,
