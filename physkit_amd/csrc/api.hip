@@ -126,7 +126,7 @@ struct pkg_ctx
     bool has_uv = false; // some shape is a tagged UV sphere
     std::vector<uint32_t> shape_cost; // support candidates per shape: 12 for a tagged UV sphere, else its vertices
     dbuf<uint32_t> shape_off, shape_voff, shape_uv;
-    dbuf<double> verts, shape_box, shape_rad;
+    dbuf<double> verts, shape_box, shape_rad, cull_rec;
 
     uint32_t n = 0;
     dbuf<double> pos, vel, ang, acc, torque, quat, mass, inv_mass, I_local, Iinv_local, Iw, Iw_inv, rest,
@@ -629,7 +629,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     if (Pn)
     {
         gjk_list.reserve(Pn);
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, counters.p, st);
+        cull_rec.reserve(20 * static_cast<size_t>(n));
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, cull_rec.p, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);

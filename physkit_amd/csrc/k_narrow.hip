@@ -104,6 +104,61 @@ __device__ __forceinline__ bool obbs_apart(const cull_body &A, const cull_body &
     }
     return false;
 }
+// Per-body cull record, written once per step: 20 doubles (160 bytes, 16-byte
+// aligned) so a pair reads two contiguous records instead of about twenty scattered
+// planar words: pos.xyz, r | c.xyz, 0 | e.xyz, ax[0].x | ax[0].yz, ax[1].xy |
+// ax[1].z, ax[2].xyz. A body whose quaternion is off unit norm gets r = infinity, which
+// makes every test of its pairs fail.
+constexpr int kCullRec = 20;
+__device__ __forceinline__ void put_cull(double *rec, const cull_body &o)
+{
+    double2 *q = reinterpret_cast<double2 *>(rec);
+    const bool unit = fabs(o.qn - 1.0) <= 1e-9;
+    q[0] = make_double2(o.pos.x, o.pos.y);
+    q[1] = make_double2(o.pos.z, unit ? o.r : __builtin_inf());
+    q[2] = make_double2(o.c.x, o.c.y);
+    q[3] = make_double2(o.c.z, 0.0);
+    q[4] = make_double2(o.e.x, o.e.y);
+    q[5] = make_double2(o.e.z, o.ax[0].x);
+    q[6] = make_double2(o.ax[0].y, o.ax[0].z);
+    q[7] = make_double2(o.ax[1].x, o.ax[1].y);
+    q[8] = make_double2(o.ax[1].z, o.ax[2].x);
+    q[9] = make_double2(o.ax[2].y, o.ax[2].z);
+}
+__device__ __forceinline__ void get_sphere(const double *rec, cull_body &o)
+{
+    const double2 *q = reinterpret_cast<const double2 *>(rec);
+    const double2 a = q[0], b = q[1];
+    o.pos = mk(a.x, a.y, b.x);
+    o.r = b.y;
+}
+__device__ __forceinline__ void get_box(const double *rec, cull_body &o)
+{
+    const double2 *q = reinterpret_cast<const double2 *>(rec);
+    const double2 c0 = q[2], c1 = q[3], e0 = q[4], e1 = q[5], a0 = q[6], a1 = q[7], a2 = q[8], a3 = q[9];
+    o.c = mk(c0.x, c0.y, c1.x);
+    o.e = mk(e0.x, e0.y, e1.x);
+    o.ax[0] = mk(e1.y, a0.x, a0.y);
+    o.ax[1] = mk(a1.x, a1.y, a2.x);
+    o.ax[2] = mk(a2.y, a3.x, a3.y);
+}
+// the sphere test on the first 32 bytes of each record; the box test reads the rest
+// only for pairs the spheres do not separate
+__device__ __forceinline__ bool records_apart(const double *ra, const double *rb)
+{
+    cull_body A, B;
+    get_sphere(ra, A);
+    get_sphere(rb, B);
+    const double r = A.r + B.r;
+    const double m = 1e-6 * r + 1e-6; // infinite for a body off unit norm: nothing passes
+    const v3 d = A.pos - B.pos;
+    if (sqrt(dot(d, d)) > r + m) return true;
+    get_box(ra, A);
+    get_box(rb, B);
+    return obbs_apart(A, B, m);
+}
+
+// the same tests from the planar body arrays (few pairs per body: no records)
 __device__ __forceinline__ bool bodies_apart(const cull_body &A, const cull_body &B)
 {
     if (!(fabs(A.qn - 1.0) <= 1e-9 && fabs(B.qn - 1.0) <= 1e-9)) return false;
@@ -112,6 +167,12 @@ __device__ __forceinline__ bool bodies_apart(const cull_body &A, const cull_body
     const v3 d = A.pos - B.pos;
     if (sqrt(dot(d, d)) > r + m) return true;
     return obbs_apart(A, B, m);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cull_prep(bodies_dev b, shapes_dev s, double *rec)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.n) put_cull(rec + static_cast<size_t>(kCullRec) * i, cull_of(b, s, i));
 }
 
 // GJK simplex hand-off, planar: slot 0 = point count, then per point p.xyz and the
@@ -154,18 +215,21 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
 
 // one thread per pair: a pair whose bodies are apart (above) is separated (status 0)
 // without GJK; the others are listed for k_gjk
+template <bool REC>
 __global__ void __launch_bounds__(kBlock)
-    k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, uint8_t *status, uint32_t *list,
-               unsigned long long *counters)
+    k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec, uint8_t *status,
+               uint32_t *list, unsigned long long *counters)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool keep = false;
     if (p < n_pairs)
     {
         const uint64_t key = keys[p];
-        const cull_body A = cull_of(b, s, static_cast<uint32_t>(key >> 32));
-        const cull_body B = cull_of(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-        keep = !bodies_apart(A, B);
+        const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        if (REC)
+            keep = !records_apart(rec + static_cast<size_t>(kCullRec) * ia, rec + static_cast<size_t>(kCullRec) * ib);
+        else
+            keep = !bodies_apart(cull_of(b, s, ia), cull_of(b, s, ib));
         if (!keep) status[p] = 0;
     }
     const unsigned long long slot = block_append(&counters[C_GJK_LIST], keep);
@@ -549,10 +613,18 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, unsigned long long *counters, hipStream_t st)
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    k_gjk_cull<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, g.status, list, counters);
+    // per-body records pay off with several pairs per body (hull clusters: 4; the
+    // batched worlds, 0.6, read the planar arrays faster)
+    if (static_cast<size_t>(n_pairs) >= 2 * static_cast<size_t>(b.n))
+    {
+        k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec);
+        k_gjk_cull<true><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, g.status, list, counters);
+    }
+    else
+        k_gjk_cull<false><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, g.status, list, counters);
     // sized for every pair; groups past the listed count (read on the device) exit at once
     unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
     if (group == 2)

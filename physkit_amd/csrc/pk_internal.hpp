@@ -205,9 +205,10 @@ struct gjk_out_dev
 };
 // GJK over every pair with `group` (2 or 4) lanes per pair; pairs whose bounding
 // spheres are clearly apart are written separated first and the rest listed in `list`
-// (capacity n_pairs), so GJK wavefronts run only pairs that need it
+// (capacity n_pairs), so GJK wavefronts run only pairs that need it; cull_rec: 20
+// doubles per body of scratch for the per-body cull records
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, unsigned long long *counters, hipStream_t st);
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, unsigned long long *counters, hipStream_t st);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 // overflowed hits rerun with an LDS store; its own overflows are listed in ovf2_list
