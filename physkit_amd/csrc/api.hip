@@ -668,18 +668,30 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     {
         const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
         uint32_t c = 0; // manifold colours (0: no contact rows)
-        rows_dev r{};
-        if (Pn)
+        const double thr = 2.0 * gnorm * dt;
+        // Rows go straight into solve order (rs, built below from `ord`); the FP32 solve
+        // converts them from rows in pair order.
+        auto write_rows = [&](const uint32_t *ord, uint32_t n_act, const rows_dev &rs, uint32_t scap)
         {
+            if (!f32)
+            {
+                launch_contact_setup(keys[nxt].p, n_act, ord, scap, b, MF(nxt), rs, dt, thr, st);
+                return;
+            }
             rows.reserve(static_cast<size_t>(4) * ncap * kRowF);
             row_valid.reserve(static_cast<size_t>(4) * ncap);
-            colour.reserve(ncap);
-            claim.reserve(std::max<uint32_t>(n, 1));
+            rows_dev r{};
             r.cap4 = static_cast<uint32_t>(std::min(rows.cap / kRowF, row_valid.cap));
             r.d = rows.p;
             r.valid = row_valid.p;
-            double thr = 2.0 * gnorm * dt;
-            launch_contact_setup(keys[nxt].p, Pn, b, MF(nxt), r, dt, thr, colour.p, counters.p, st);
+            launch_contact_setup(keys[nxt].p, Pn, nullptr, 0, b, MF(nxt), r, dt, thr, st);
+            launch_rows_gather32(ord, n_act, r, rs, scap, st);
+        };
+        if (Pn)
+        {
+            colour.reserve(ncap);
+            claim.reserve(std::max<uint32_t>(n, 1));
+            launch_row_flags(keys[nxt].p, Pn, b, MF(nxt), colour.p, counters.p, st);
             if (read_counter(C_ROWS) > 0)
             {
                 const uint32_t *wo = nullptr, *pb = nullptr;
@@ -746,10 +758,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 rs.cap4 = 4 * scap;
                 rs.d = rows_s.p;
                 rs.valid = row_valid_s.p;
-                if (f32)
-                    launch_rows_gather32(is.ord, n_act, r, rs, scap, st);
-                else
-                    launch_rows_gather(is.ord, n_act, r, rs, scap, st);
+                write_rows(is.ord, n_act, rs, scap);
                 launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
                                      cfg.solver_iterations, f32, st);
                 if (f32)
@@ -788,10 +797,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 rs.cap4 = 4 * scap;
                 rs.d = rows_s.p;
                 rs.valid = row_valid_s.p;
-                if (f32)
-                    launch_rows_gather32(ord, n_act, r, rs, scap, st);
-                else
-                    launch_rows_gather(ord, n_act, r, rs, scap, st);
+                write_rows(ord, n_act, rs, scap);
             }
             const joints_dev jd = JD(jset);
             const uint32_t jc = n_joints ? static_cast<uint32_t>(jcs_h.size()) - 1 : 0;

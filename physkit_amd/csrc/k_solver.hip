@@ -199,15 +199,70 @@ __device__ __forceinline__ void wf3(const rows_dev &r, int f, uint32_t slot, v3 
     rf(r, f + 2, slot) = v.z;
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_setup(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, manifolds_dev m, rows_dev r, double dt,
-            double thr, uint32_t *colour, unsigned long long *counters)
+// Contact c of manifold p: world normal n and lever arms as build_contact_jacobian
+// forms them (:883-890) and whether the contact makes rows (a penetration that is not
+// <= 0, so a NaN one does too, as in the reference's `if (penetration <= 0) continue`).
+__device__ __forceinline__ bool contact_frame(const manifolds_dev &m, uint32_t p, int c, v3 pa, v3 pb, qt qa, qt qb,
+                                              v3 &n, v3 &ra, v3 &rb, double &pen)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
+    const size_t s = m.cap;
+    const v3 nw = normalized(mk(d[0], d[s], d[2 * s]));
+    const v3 la = mk(d[3 * s], d[4 * s], d[5 * s]);
+    const v3 lb = mk(d[6 * s], d[7 * s], d[8 * s]);
+    const v3 lnb = rotate(conj(qb), nw);
+    ra = rotate(qa, la);
+    rb = rotate(qb, lb);
+    n = rotate(qb, lnb);
+    pen = dot((pb + rb) - (pa + ra), n);
+    return !(pen <= 0.0);
+}
+
+// which manifolds make solver rows: colour[p] = kUncoloured (to be coloured) or kNone,
+// and the row count; the rows themselves are written by k_setup once the solve order
+// is known
+__global__ void __launch_bounds__(kBlock)
+    k_row_flags(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, manifolds_dev m, uint32_t *colour,
+                unsigned long long *counters)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nrows = 0;
     if (p < n_pairs)
     {
-        for (int c = 0; c < 4; ++c) r.valid[4 * p + c] = 0;
+        const int cnt = m.count[p];
+        if (cnt > 0)
+        {
+            const uint64_t key = keys[p];
+            const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+            const v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
+            const qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+            for (int c = 0; c < cnt; ++c)
+            {
+                v3 n, ra, rb;
+                double pen;
+                if (contact_frame(m, p, c, pa, pb, qa, qb, n, ra, rb, pen)) ++nrows;
+            }
+        }
+        colour[p] = nrows ? kUncoloured : kNone;
+    }
+    block_add(&counters[C_ROWS], nrows);
+}
+
+// Rows of manifold ord[k] in solve order: contact c at slot c * cap + k of every row
+// plane, so the lanes solving one colour read each field at consecutive addresses.
+// Without ord, manifold k's contact c at slot 4k + c (pair order).
+__device__ __forceinline__ uint32_t row_slot(const uint32_t *ord, uint32_t cap, uint32_t k, int c)
+{ return ord ? static_cast<uint32_t>(c) * cap + k : 4 * k + static_cast<uint32_t>(c); }
+
+__global__ void __launch_bounds__(kBlock)
+    k_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, bodies_dev b, manifolds_dev m,
+            rows_dev r, double dt, double thr)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+    {
+        const uint32_t p = ord ? ord[k] : k;
+        for (int c = 0; c < 4; ++c) r.valid[row_slot(ord, cap, k, c)] = 0;
         int cnt = m.count[p];
         if (cnt > 0)
         {
@@ -223,17 +278,11 @@ __global__ void __launch_bounds__(kBlock)
             for (int c = 0; c < cnt; ++c)
             {
                 const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-                size_t s = m.cap;
-                v3 nw = normalized(mk(d[0], d[s], d[2 * s]));
-                v3 la = mk(d[3 * s], d[4 * s], d[5 * s]);
-                v3 lb = mk(d[6 * s], d[7 * s], d[8 * s]);
+                const size_t s = m.cap;
                 // build_contact_jacobian, :874-953
-                v3 lnb = rotate(conj(qb), nw);
-                v3 ra = rotate(qa, la);
-                v3 rb = rotate(qb, lb);
-                v3 n = rotate(qb, lnb);
-                double pen = dot((pb + rb) - (pa + ra), n);
-                if (pen <= 0.0) continue;
+                v3 n, ra, rb;
+                double pen;
+                if (!contact_frame(m, p, c, pa, pb, qa, qb, n, ra, rb, pen)) continue;
                 rowv nr = make_row(A, B, n, ra, rb);
                 v3 vca = A.vel + cross(A.ang, ra);
                 v3 vcb = B.vel + cross(B.ang, rb);
@@ -263,7 +312,7 @@ __global__ void __launch_bounds__(kBlock)
                     i22 = r2.Meff;
                     i12 = 0.0;
                 }
-                uint32_t slot = 4 * p + static_cast<uint32_t>(c);
+                const uint32_t slot = row_slot(ord, cap, k, c);
                 wf3(r, F_NJV, slot, nr.Jv);
                 wf3(r, F_NJWA, slot, nr.Jwa);
                 wf3(r, F_NJWB, slot, nr.Jwb);
@@ -285,12 +334,9 @@ __global__ void __launch_bounds__(kBlock)
                 rf(r, F_I12, slot) = i12;
                 rf(r, F_I22, slot) = i22;
                 r.valid[slot] = 1;
-                ++nrows;
             }
         }
-        colour[p] = nrows ? kUncoloured : kNone;
     }
-    block_add(&counters[C_ROWS], nrows);
 }
 
 // Claim of a manifold in a colouring round: the round (from 1) above the complement of
@@ -405,25 +451,6 @@ __global__ void k_colour_bounds(const uint64_t *sorted, uint32_t n, uint32_t n_c
     start[c] = lo;
 }
 
-// the rows of the solver manifolds gathered into solve order: position t of `order`
-// (manifold order[t]) keeps contact c at slot c * cap + t of every row plane, so the
-// lanes solving one colour read each row field with consecutive addresses
-__global__ void k_rows_gather(const uint32_t *order, uint32_t n_act, rows_dev src, rows_dev dst, uint32_t cap)
-{
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_act) return;
-    uint32_t p = order[t];
-    for (int c = 0; c < 4; ++c)
-    {
-        uint32_t s = 4 * p + static_cast<uint32_t>(c), d = static_cast<uint32_t>(c) * cap + t;
-        uint8_t v = src.valid[s];
-        dst.valid[d] = v;
-        if (!v) continue;
-        for (int f = 0; f < kRowF; ++f)
-            dst.d[static_cast<size_t>(f) * dst.cap4 + d] = src.d[static_cast<size_t>(f) * src.cap4 + s];
-    }
-}
-
 __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, int f_bias, int f_acc, uint32_t slot)
 {
     rowv v;
@@ -436,7 +463,7 @@ __device__ __forceinline__ rowv ld_row(const rows_dev &r, int f_jv, int f_meff, 
     return v;
 }
 
-// the manifold at position t of the solve order (rows gathered by k_rows_gather)
+// the manifold at position t of the solve order (rows written in that order by k_setup)
 template <class V = vel_global>
 __device__ __forceinline__ void solve_one(const uint64_t *keys, const uint32_t *order, uint32_t t, const bodies_dev &b,
                                           const rows_dev &r, uint32_t cap, int warm, const V &vs = V(),
@@ -946,11 +973,15 @@ __global__ void __launch_bounds__(kIslandThreads)
 }
 } // namespace
 
-void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                          manifolds_dev m, rows_dev r, double dt, double threshold,
-                          uint32_t *colour, unsigned long long *counters, hipStream_t st)
+void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, manifolds_dev m, uint32_t *colour,
+                      unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs) k_setup<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, r, dt, threshold, colour, counters);
+    if (n_pairs) k_row_flags<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, colour, counters);
+}
+void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
+                          manifolds_dev m, rows_dev r, double dt, double threshold, hipStream_t st)
+{
+    if (n) k_setup<<<grid(n), kBlock, 0, st>>>(keys, n, ord, cap, b, m, r, dt, threshold);
 }
 void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
                       hipStream_t st)
@@ -992,11 +1023,6 @@ uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_
     const uint64_t *sk = alt ? keys_alt : keys;
     k_colour_bounds<<<grid(n_colours + 1), kBlock, 0, st>>>(sk, n_pairs, n_colours, colour_start);
     return alt ? order_alt : order;
-}
-void launch_rows_gather(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst, uint32_t cap,
-                        hipStream_t st)
-{
-    if (n_act) k_rows_gather<<<grid(n_act), kBlock, 0, st>>>(order, n_act, src, dst, cap);
 }
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                          const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st)

@@ -246,9 +246,14 @@ struct rows_dev
 };
 constexpr int kRowF = 38;
 constexpr uint32_t kUncoloured = 0xFFFFFFFEu; // colour of an item still to be coloured
-void launch_contact_setup(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b,
-                          manifolds_dev m, rows_dev r, double dt, double threshold,
-                          uint32_t *colour, unsigned long long *counters, hipStream_t st);
+// colour[p] = kUncoloured for manifolds that make solver rows (else kNone); their row
+// count at counters[C_ROWS]
+void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, manifolds_dev m, uint32_t *colour,
+                      unsigned long long *counters, hipStream_t st);
+// rows of manifold ord[k], k < n, at slots c * cap + k of r (solve order); with ord
+// null, of manifold k at slots 4k + c (pair order)
+void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
+                          manifolds_dev m, rows_dev r, double dt, double threshold, hipStream_t st);
 // With batched worlds, a manifold's colouring priority uses its index within its own
 // world's pair list (p - pair_base[world]), so each world colours as if stepped alone;
 // world_of and pair_base are null for a single world.
@@ -272,8 +277,6 @@ void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n
 uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
                                uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,
                                sort_scratch &sc, hipStream_t st);
-void launch_rows_gather(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst, uint32_t cap,
-                        hipStream_t st);
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                          const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,
