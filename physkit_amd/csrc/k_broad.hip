@@ -474,6 +474,18 @@ __global__ void __launch_bounds__(kBlock) k_span(lbvh_dev t)
 // wavefront, so no wavefront waits for its workgroup's longest walk.
 constexpr int kEmitBuf = 32;
 
+// The node whose leaf range is exactly [lo, hi]: a world's leaves share the world bits
+// at the top of their keys, so they form one subtree, whose root (a Karras node sits at
+// one end of its range) is node lo or node hi, or the leaf itself for a one-body world.
+// kNone if neither matches.
+__device__ __forceinline__ uint32_t range_root(const lbvh_dev &t, uint32_t lo, uint32_t hi)
+{
+    if (lo == hi) return t.n - 1 + lo;
+    if (t.first[lo] == lo && t.last[lo] == hi) return lo;
+    if (hi < t.n - 1 && t.first[hi] == lo && t.last[hi] == hi) return hi;
+    return kNone;
+}
+
 __device__ __forceinline__ void flush_own(uint64_t *buf, uint32_t cnt, uint64_t *out_keys, uint32_t cap,
                                           unsigned long long *cursor)
 {
@@ -510,18 +522,28 @@ __global__ void __launch_bounds__(kBlock)
             qhi[k] = f_up(p[3 + k]);
         }
     }
-    // leaves of other worlds are never visited: the node's leaf range must meet the
-    // sorted range of the querying leaf's world
+    // leaves of other worlds are never visited: the walk starts at the root of the
+    // querying leaf's world and ends at that subtree's escape, or (no such root) the
+    // node's leaf range must meet the sorted range of the world
     const bool batched = t.n_worlds > 1;
     uint32_t wlo = 0, whi = n - 1;
+    uint32_t node = active ? 0u : kNone, stop = kNone;
+    bool ranged = false;
     if (active && batched)
     {
         uint32_t w = t.world_of[a];
         wlo = t.wstart[w];
         whi = t.wstart[w + 1] - 1;
+        const uint32_t root = range_root(t, wlo, whi);
+        if (root != kNone)
+        {
+            node = root;
+            stop = t.escape[root];
+        }
+        else
+            ranged = true;
     }
-    uint32_t node = active ? 0u : kNone;
-    for (uint32_t guard = 0; node != kNone; ++guard)
+    for (uint32_t guard = 0; node != stop; ++guard)
     {
         if (guard >= 2 * n)
         {
@@ -535,7 +557,7 @@ __global__ void __launch_bounds__(kBlock)
             bool ov = lo.x <= qhi[0] && hi.x >= qlo[0] && lo.y <= qhi[1] && hi.y >= qlo[1] && lo.z <= qhi[2] &&
                       hi.z >= qlo[2];
             next = __float_as_uint(hi.w);
-            if (ov && batched) ov = t.first[node] <= whi && (t.lastf[node] & 0x7fffffffu) >= wlo;
+            if (ov && ranged) ov = t.first[node] <= whi && (t.lastf[node] & 0x7fffffffu) >= wlo;
             if (ov)
             {
                 uint32_t lf = t.lastf[node];
