@@ -68,17 +68,6 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long *co
     return base + static_cast<unsigned long long>(__popcll(below));
 }
 
-// adds the sum of v (0 <= v < 8) over the active lanes with one atomic per wavefront
-__device__ __forceinline__ void wave_sum3(unsigned long long *counter, uint32_t v)
-{
-    unsigned long long mask = __ballot(true);
-    unsigned long long s = static_cast<unsigned long long>(__popcll(__ballot((v & 1u) != 0))) +
-                           2ull * static_cast<unsigned long long>(__popcll(__ballot((v & 2u) != 0))) +
-                           4ull * static_cast<unsigned long long>(__popcll(__ballot((v & 4u) != 0)));
-    int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    if (__lane_id() == leader && s) atomicAdd(counter, s);
-}
-
 // Workgroup-aggregated counters. Device-scope atomics on one address serialise across
 // the device at about 12 ns each on MI355X (65,536 wavefronts adding once each to one
 // counter take 0.79 ms; one add per 256-thread workgroup, 0.20 ms), so a kernel that
