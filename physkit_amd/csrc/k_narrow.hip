@@ -16,6 +16,10 @@ namespace
 {
 constexpr int kBlock = 256;
 constexpr int kEpaBlock = 64;
+// up to this many hits, each solve runs in its own workgroup with the LDS store: on a
+// near-idle device the longest solve's chain of LDS round trips ends sooner than one
+// through private memory
+constexpr uint32_t kEpaSmall = 512;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 // UV: the shapes may include tagged UV spheres (a kernel built without it carries no
@@ -310,12 +314,12 @@ __global__ void __launch_bounds__(kEpaBlock)
 // chain of dependent accesses. A solve that needs more goes on to k_epa_big.
 __global__ void __launch_bounds__(64)
     k_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b, shapes_dev s,
-              gjk_out_dev g, uint32_t *ovf2_list, unsigned long long *counters)
+              gjk_out_dev g, uint32_t *ovf2_list, unsigned long long *counters, int out_slot)
 {
     __shared__ epa_store_lds store;
     const uint32_t t = blockIdx.x;
     if (t >= n_ovf || threadIdx.x != 0) return;
-    const uint32_t h = ovf_list[t];
+    const uint32_t h = ovf_list ? ovf_list[t] : t;
     const uint32_t p = g.hit_list[h];
     const uint64_t key = keys[p];
     gshape A = body_shape<true>(b, s, static_cast<uint32_t>(key >> 32));
@@ -326,7 +330,7 @@ __global__ void __launch_bounds__(64)
     const int r = epa_solve<true>(A, B, x, store, out);
     if (r == 3)
     {
-        ovf2_list[atomicAdd(&counters[C_EPA_OVF2], 1ull)] = h;
+        ovf2_list[atomicAdd(&counters[out_slot], 1ull)] = h;
         return;
     }
     if (r == 1) write_info(g.info, g.pair_cap, p, out);
@@ -647,7 +651,9 @@ void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, cons
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_hits) return;
-    if (uv)
+    if (n_hits <= kEpaSmall)
+        k_epa_lds<<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters, C_EPA_OVF);
+    else if (uv)
         k_epa<true><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
     else
         k_epa<false><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
@@ -657,7 +663,7 @@ void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                     const shapes_dev &s, const gjk_out_dev &g, uint32_t *ovf2_list, unsigned long long *counters,
                     hipStream_t st)
 {
-    if (n_ovf) k_epa_lds<<<n_ovf, 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g, ovf2_list, counters);
+    if (n_ovf) k_epa_lds<<<n_ovf, 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g, ovf2_list, counters, C_EPA_OVF2);
 }
 
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
