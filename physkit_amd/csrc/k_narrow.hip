@@ -312,22 +312,23 @@ __global__ void __launch_bounds__(kEpaBlock)
 // Second tier: one workgroup per hit that overflowed the private store, its polytope in
 // LDS (896 faces): LDS round trips instead of global-memory ones on the solve's long
 // chain of dependent accesses. A solve that needs more goes on to k_epa_big.
+template <class S, bool UV>
 __global__ void __launch_bounds__(64)
     k_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, bodies_dev b, shapes_dev s,
               gjk_out_dev g, uint32_t *ovf2_list, unsigned long long *counters, int out_slot)
 {
-    __shared__ epa_store_lds store;
+    __shared__ S store;
     const uint32_t t = blockIdx.x;
     if (t >= n_ovf || threadIdx.x != 0) return;
     const uint32_t h = ovf_list ? ovf_list[t] : t;
     const uint32_t p = g.hit_list[h];
     const uint64_t key = keys[p];
-    gshape A = body_shape<true>(b, s, static_cast<uint32_t>(key >> 32));
-    gshape B = body_shape<true>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+    gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
+    gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
     simplex x;
     load_simplex(g.simplex, g.cap, h, x);
     double out[10];
-    const int r = epa_solve<true>(A, B, x, store, out);
+    const int r = epa_solve<UV>(A, B, x, store, out);
     if (r == 3)
     {
         ovf2_list[atomicAdd(&counters[out_slot], 1ull)] = h;
@@ -651,8 +652,12 @@ void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, cons
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_hits) return;
-    if (n_hits <= kEpaSmall)
-        k_epa_lds<<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters, C_EPA_OVF);
+    if (n_hits <= kEpaSmall && uv)
+        k_epa_lds<epa_store_lds, true><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters,
+                                                             C_EPA_OVF);
+    else if (n_hits <= kEpaSmall)
+        k_epa_lds<epa_store_lds, false><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters,
+                                                              C_EPA_OVF);
     else if (uv)
         k_epa<true><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
     else
@@ -663,7 +668,9 @@ void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                     const shapes_dev &s, const gjk_out_dev &g, uint32_t *ovf2_list, unsigned long long *counters,
                     hipStream_t st)
 {
-    if (n_ovf) k_epa_lds<<<n_ovf, 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g, ovf2_list, counters, C_EPA_OVF2);
+    if (n_ovf)
+        k_epa_lds<epa_store_lds, true><<<n_ovf, 64, 0, st>>>(keys, ovf_list, n_ovf, b, s, g, ovf2_list, counters,
+                                                            C_EPA_OVF2);
 }
 
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
