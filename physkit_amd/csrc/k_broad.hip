@@ -266,6 +266,10 @@ struct refit_lds
     float box[kChunk][6];
     uint8_t moved[kChunk];
     fbox wave_tot[2][kRefitThreads / 64]; // per-wavefront totals of the chunk scans
+    // the chunk's internal node records, escapes and parents, and its leaves' parents,
+    // read once (coalesced) so the climbs touch only LDS for the tree links
+    uint4 rec[kChunk];
+    uint32_t esc[kChunk], ipar[kChunk], lpar[kChunk];
 };
 
 __device__ __forceinline__ fbox child_box_local(const lbvh_dev &t, uint32_t c, uint32_t c_lo, const refit_lds &L)
@@ -372,6 +376,13 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
     // chunk prefix and suffix unions of the leaf boxes
     const uint32_t i = c_lo + tid;
     const bool valid = i <= c_hi;
+    if (i < n - 1 && i <= c_hi)
+    {
+        L.rec[tid] = t.nrec[i];
+        L.esc[tid] = t.escape[i];
+        L.ipar[tid] = t.parent[i];
+    }
+    if (valid) L.lpar[tid] = t.parent[n - 1 + i];
     const fbox leaf = valid ? child_box(t, n - 1 + i) : empty_fbox();
     fbox pre = scan_wave<true>(leaf);
     fbox suf = scan_wave<false>(leaf);
@@ -390,12 +401,15 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
     if (!valid) return;
     uint32_t cur = n - 1 + i;
     fbox mine = leaf;
-    uint32_t node = t.parent[cur];
+    uint32_t node = L.lpar[tid];
+    // a node whose range lies in the chunk has its index there (Karras nodes sit at one
+    // end of their range): an index outside is a spanning node
     for (uint32_t guard = 0; node != kNone && guard < n; ++guard)
     {
-        const uint4 rec = t.nrec[node];
-        if (rec.z < c_lo || rec.w > c_hi) break; // spanning node: k_span computes it
-        const uint32_t esc = t.escape[node];
+        if (node < c_lo || node > c_hi) break; // spanning node: k_span computes it
+        const uint4 rec = L.rec[node - c_lo];
+        if (rec.z < c_lo || rec.w > c_hi) break;
+        const uint32_t esc = L.esc[node - c_lo];
         if (wg_arrive(&L.visit[node - c_lo]) == 0) break;
         const uint32_t sib = rec.x == cur ? rec.y : rec.x;
         const fbox o = child_box_local(t, sib, c_lo, L);
@@ -417,7 +431,7 @@ __global__ void __launch_bounds__(kRefitThreads) k_refit(lbvh_dev t)
         }
         mine.moved = mv;
         cur = node;
-        node = t.parent[node];
+        node = L.ipar[node - c_lo];
     }
 }
 
