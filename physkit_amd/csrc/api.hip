@@ -692,8 +692,14 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             colour.reserve(ncap);
             claim.reserve(std::max<uint32_t>(n, 1));
             launch_row_flags(keys[nxt].p, Pn, b, MF(nxt), colour.p, counters.p, st);
-            if (read_counter(C_ROWS) > 0)
+            // the manifolds with rows are listed before the host looks at the row count,
+            // so one copy returns both (C_NCOL and C_NLIST are zero from the step's start)
+            clist.reserve(Pn);
+            launch_colour_list(colour.p, Pn, clist.p, counters.p, st);
+            read_counters(C_ROWS, C_NLIST);
+            if (h_counters[C_ROWS] > 0)
             {
+                const uint32_t nl = static_cast<uint32_t>(h_counters[C_NLIST]);
                 const uint32_t *wo = nullptr, *pb = nullptr;
                 if (n_worlds > 1)
                 {
@@ -703,25 +709,25 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                     pb = pair_base.p;
                 }
                 used.reserve(std::max<uint32_t>(n, 1));
-                clist.reserve(Pn);
                 launch_fill_u64(used.p, 0ull, n, st);
                 launch_fill_u64(claim.p, 0ull, n, st);
-                reset_counter(C_NCOL);
-                reset_counter(C_NLIST);
-                launch_colour_list(colour.p, Pn, clist.p, counters.p, st);
-                const uint32_t nl = static_cast<uint32_t>(read_counter(C_NLIST));
-                // the host looks at the remaining count every kColourCheck rounds; rounds
-                // after the last manifold is coloured change nothing
+                // the host looks at the remaining count every kColourCheck rounds (with the
+                // colour count in the same copy); rounds after the last manifold is
+                // coloured change nothing
                 for (uint32_t round = 0;; ++round)
                 {
                     launch_colour_claim(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, round, st);
                     reset_counter(C_REMAIN);
                     launch_colour_assign(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, used.p, round,
                                          counters.p, st);
-                    if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
+                    if (round % kColourCheck == kColourCheck - 1)
+                    {
+                        read_counters(C_REMAIN, C_NCOL);
+                        if (h_counters[C_REMAIN] == 0) break;
+                    }
                     if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
-                c = static_cast<uint32_t>(read_counter(C_NCOL));
+                c = static_cast<uint32_t>(h_counters[C_NCOL]);
             }
         }
         n_colours = c;
@@ -744,12 +750,13 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             island_sort is = launch_island_sort(keys[nxt].p, colour.p, Pn, c, b, par.p, ik.p, iv.p, ik_alt.p,
                                                 iv_alt.p, counters.p + C_ISL_MAX, sc, st);
             launch_island_segments(is, Pn, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
+            // island count, row manifold count and the longest island in one wait
+            launch_island_maxlen(istarts.p, total.p, Pn, counters.p + C_ISL_MAX, st);
             uint32_t h2[2] = {0, 0};
             hip_check(hipMemcpyAsync(h2, total.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "island count");
-            hip_check(hipStreamSynchronize(st), "stream sync");
+            read_counters(C_ISL_MAX, C_ISL_MAX);
             const uint32_t n_isl = h2[0], n_act = h2[1];
-            launch_island_maxlen(istarts.p, n_isl, n_act, counters.p + C_ISL_MAX, st);
-            if (read_counter(C_ISL_MAX) <= kIslandMaxManifolds)
+            if (h_counters[C_ISL_MAX] <= kIslandMaxManifolds)
             {
                 const uint32_t scap = std::max<uint32_t>(n_act, 1);
                 rows_s.reserve(static_cast<size_t>(4) * scap * kRowF);

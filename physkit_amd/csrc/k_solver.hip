@@ -793,8 +793,11 @@ __global__ void k_island_starts(const uint32_t *flags, const uint32_t *pos, uint
     if (t < n && flags[t]) starts[pos[t]] = t;
 }
 
-__global__ void k_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx)
+// totals = (island count, row manifold count), read on the device; the grid covers an
+// upper bound of the island count
+__global__ void k_island_maxlen(const uint32_t *starts, const uint32_t *totals, unsigned long long *mx)
 {
+    const uint32_t n_isl = totals[0], n_act = totals[1];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t len = 0;
     if (i < n_isl) len = (i + 1 < n_isl ? starts[i + 1] : n_act) - starts[i];
@@ -1069,10 +1072,10 @@ void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *f
     exclusive_scan_u32(flags, pos, n_pairs, bsums, total, st);
     k_island_starts<<<grid(n_pairs), kBlock, 0, st>>>(flags, pos, n_pairs, starts);
 }
-void launch_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx,
-                          hipStream_t st)
+void launch_island_maxlen(const uint32_t *starts, const uint32_t *totals, uint32_t max_islands,
+                          unsigned long long *mx, hipStream_t st)
 {
-    if (n_isl) k_island_maxlen<<<grid(n_isl), kBlock, 0, st>>>(starts, n_isl, n_act, mx);
+    if (max_islands) k_island_maxlen<<<grid(max_islands), kBlock, 0, st>>>(starts, totals, mx);
 }
 void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
                           uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,

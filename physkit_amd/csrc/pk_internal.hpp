@@ -300,8 +300,8 @@ island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uin
 // island starts (positions where the island changes) and the row-manifold count
 void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *flags, uint32_t *pos, uint32_t *bsums,
                             uint32_t *total, uint32_t *starts, uint32_t *n_act, hipStream_t st);
-void launch_island_maxlen(const uint32_t *starts, uint32_t n_isl, uint32_t n_act, unsigned long long *mx,
-                          hipStream_t st);
+void launch_island_maxlen(const uint32_t *starts, const uint32_t *totals, uint32_t max_islands,
+                          unsigned long long *mx, hipStream_t st);
 void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
                           uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
                           uint32_t iters, bool f32, hipStream_t st);
