@@ -49,22 +49,55 @@ WORKLOADS = {
 RAYS = 1 << 22
 
 
-def algorithmic_bytes(stage, s, pairs, hits_frac):
-    """Compulsory HBM bytes of one launch of the narrowphase kernels (FP64).
-    Per pair, GJK reads the key (8 B), each body's pose (pos 24 + quat 32), shape id and
-    vertex offsets (12) and every vertex of both shapes once (24 B each), and writes a
-    status byte; a hit also writes its hit-list entry (4) and simplex (17 x 8: count, then
-    per point p.xyz and the packed support features).
-    EPA re-reads, per hit, the hit-list entry, key, poses, offsets, vertices and
-    simplex, and writes 10 doubles of contact data and a status byte."""
+def support_vertices(s):
+    """Vertices a support search of each shape must read: all of them, except for a
+    UV-sphere mesh (2 + (stacks-1) * sectors vertices, poles on the y axis), whose
+    search evaluates 12 candidates around the analytic optimum."""
     off = s["shape_offsets"].astype(np.int64)
-    nv = (off[1:] - off[:-1])[s["shape"]]
+    v = s["verts"]
+    nv = off[1:] - off[:-1]
+    cost = nv.astype(np.float64)
+    big = np.nonzero(nv > 64)[0]
+    if len(big):
+        top, bot = v[off[big]], v[off[big + 1] - 1]
+        first, second = v[off[big] + 1], v[off[big] + 2]
+        uv = ((top[:, 1] > 0) & (top[:, 0] == 0) & (top[:, 2] == 0) & (bot[:, 0] == 0) & (bot[:, 2] == 0)
+              & (bot[:, 1] == -top[:, 1]) & (first[:, 1] == second[:, 1]))
+        cost[big[uv]] = 12.0
+    return cost
+
+
+def shape_radii(s):
+    """Largest vertex distance from each shape's origin."""
+    off = s["shape_offsets"].astype(np.int64)
+    r2 = (s["verts"] ** 2).sum(1)
+    return np.sqrt(np.maximum.reduceat(r2, off[:-1]))
+
+
+def algorithmic_bytes(stage, s, pairs, hits_frac, pos=None):
+    """Compulsory HBM bytes of one launch of the narrowphase stage (FP64).
+    GJK stage: a pair whose bounding spheres are apart (the cull's first test) needs its
+    key (8 B), both bodies' positions and radii (2 x 32) and its status byte. Any other
+    pair also reads each body's quaternion (32), shape id and vertex offsets (12) and the
+    support vertices of both shapes once (24 B each; 12 candidates for a UV sphere); a
+    hit writes its hit-list entry (4) and simplex (17 x 8: count, then per point p.xyz
+    and the packed support features).
+    EPA re-reads, per hit, the hit-list entry, key, poses, offsets, support vertices and
+    simplex, and writes 10 doubles of contact data and a status byte."""
+    cost = support_vertices(s)[s["shape"]]
     a = (pairs >> np.uint64(32)).astype(np.int64)
     b = (pairs & np.uint64(0xFFFFFFFF)).astype(np.int64)
-    verts = 24.0 * (nv[a] + nv[b]).astype(np.float64)
+    verts = 24.0 * (cost[a] + cost[b])
     per_pair_fixed = 8 + 2 * (24 + 32 + 12)
     if stage == "gjk":
-        return float(np.sum(per_pair_fixed + verts + 1) + hits_frac * len(pairs) * (4 + 17 * 8))
+        rr = shape_radii(s)[s["shape"]]
+        p = s["pos"] if pos is None else pos
+        d = np.linalg.norm(p[a] - p[b], axis=1)
+        r = rr[a] + rr[b]
+        apart = d > r + 1e-6 * r + 1e-6
+        culled = float(np.count_nonzero(apart))
+        rest = np.sum((per_pair_fixed + verts + 1)[~apart])
+        return float(culled * (8 + 2 * 32 + 1) + rest + hits_frac * len(pairs) * (4 + 17 * 8))
     if stage == "epa":
         return float(hits_frac * np.sum(4 + per_pair_fixed + verts + 17 * 8 + 80 + 1))
     raise ValueError(stage)
@@ -417,7 +450,7 @@ def main():
     # dominant kernel of the narrowphase and its roofline
     dom = max(("gjk", "epa"), key=lambda k: stage[k])
     t_launch = stage[dom] / 1e3 / args.steps
-    alg = algorithmic_bytes(dom, s, pairs, hits_frac) if len(pairs) else 0.0
+    alg = algorithmic_bytes(dom, s, pairs, hits_frac, g.bodies()["pos"]) if len(pairs) else 0.0
     achieved = alg / t_launch / 1e9 if t_launch > 0 else 0.0
     roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
