@@ -658,6 +658,12 @@ __global__ void k_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, 
 
 // previous pairs that no candidate covers stay when their leaves still overlap
 // (collision_phases.h:383-403)
+// A previous pair no candidate covers stays while its fat boxes overlap
+// (collision_phases.h:383-403). The candidates are every overlapping pair with a moved
+// leaf, so an uncovered pair with a moved body no longer overlaps; with neither body
+// moved, both fat boxes are the ones under which the pair was active last step (a fat
+// box changes only when its body moves, and an upload clears the pair list), so it
+// still overlaps. The overlap test thus reduces to the two moved flags.
 __global__ void k_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered, bodies_dev b,
                             uint32_t *flag)
 {
@@ -665,7 +671,7 @@ __global__ void k_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t
     if (p >= n_prev) return;
     uint64_t key = prev[p];
     uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-    flag[p] = (!covered[p] && box_intersects(ldbox(b.fat, b.n, a), ldbox(b.fat, b.n, c))) ? 1u : 0u;
+    flag[p] = (!covered[p] && !b.moved[a] && !b.moved[c]) ? 1u : 0u;
 }
 
 __global__ void k_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag, const uint32_t *pos,
