@@ -548,17 +548,43 @@ template <class S> struct epa_solver
         m.g[fi].nz = nrm.z;
         m.g[fi].d = dot(nrm, pi);
     }
+    // init_face(fi, s, e, k, -1), the link to adj across (s, e) (:305-313) and push(fi)
+    // for the new face (s, e, k) on a horizon edge. k is the new point, whose coordinates
+    // pk are still in registers. The face's vertices stay in the order (s, e, k), so s is
+    // at index 0 and the link to adj is a[0].
+    __device__ __forceinline__ void new_face(int fi, int s, int e, int k, v3 pk, int adj)
+    {
+        v3 ps = P(s);
+        v3 ab = P(e) - ps;
+        v3 ac = pk - ps;
+        v3 nrm = cross(ab, ac);
+        if (sqn(nrm) < 1e-12)
+            nrm = mk(0.0, 0.0, 0.0);
+        else
+            nrm = normalized(nrm);
+        const double d = dot(nrm, ps);
+        typename S::meta mt;
+        mt.v[0] = static_cast<typename S::idx_t>(s);
+        mt.v[1] = static_cast<typename S::idx_t>(e);
+        mt.v[2] = static_cast<typename S::idx_t>(k);
+        mt.a[0] = static_cast<typename S::idx_t>(adj);
+        mt.a[1] = S::null;
+        mt.a[2] = S::null;
+        mt.obs = 0;
+        mt.pad = 0;
+        m.fm[fi] = mt;
+        m.g[fi].nx = nrm.x;
+        m.g[fi].ny = nrm.y;
+        m.g[fi].nz = nrm.z;
+        m.g[fi].d = d;
+        m.fm[adj].a[vidx(adj, e)] = static_cast<typename S::idx_t>(fi);
+        ++nheap;
+        sift_up(nheap - 1, fi, d);
+    }
     __device__ __forceinline__ int vidx(int f, int va) const
     {
         typename S::meta mt = m.fm[f];
         return (mt.v[0] == va) ? 0 : (mt.v[1] == va ? 1 : 2);
-    }
-    __device__ __forceinline__ void link(int f1, int f2, int va, int vb) // :305-313
-    {
-        int e1 = vidx(f1, va);
-        int e2 = vidx(f2, vb);
-        m.fm[f1].a[e1] = static_cast<typename S::idx_t>(f2);
-        m.fm[f2].a[e2] = static_cast<typename S::idx_t>(f1);
     }
     // min-heap on plane distance, in the sift order of libstdc++'s push_heap/pop_heap
     __device__ __forceinline__ void sift_up(int hole, int vf, double vd)
@@ -760,13 +786,12 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         int base = e.nfaces;
         for (int h = 0; h < nh; ++h)
         {
-            int f = base + h;
             typename S::edge ed = store.eg[h];
-            e.init_face(f, ed.s, ed.e, pi, -1);
-            e.link(f, ed.adj, ed.s, ed.e);
-            e.push(f);
+            e.new_face(base + h, ed.s, ed.e, pi, p.p, ed.adj);
         }
         e.nfaces += nh;
+        // link(base + i, base + j, he_i, pi) and its mirror: a new face is (s, e, pi), so
+        // its edge (e, pi) is a[1] and its edge (pi, s) is a[2]
         for (int i = 0; i < nh; ++i)
         {
             typename S::edge ei = store.eg[i];
@@ -776,9 +801,15 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
                 typename S::edge ej = store.eg[j];
                 int hs_j = ej.s, he_j = ej.e;
                 if (he_i == hs_j)
-                    e.link(base + i, base + j, he_i, pi);
+                {
+                    store.fm[base + i].a[1] = static_cast<typename S::idx_t>(base + j);
+                    store.fm[base + j].a[2] = static_cast<typename S::idx_t>(base + i);
+                }
                 else if (hs_i == he_j)
-                    e.link(base + j, base + i, he_j, pi);
+                {
+                    store.fm[base + j].a[1] = static_cast<typename S::idx_t>(base + i);
+                    store.fm[base + i].a[2] = static_cast<typename S::idx_t>(base + j);
+                }
             }
         }
     }
