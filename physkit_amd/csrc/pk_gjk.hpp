@@ -500,6 +500,25 @@ template <int FC, int PC, class I> struct epa_store
     } eg[FC];
     I hf[FC];  // heap: face in each slot
     I stk[FC]; // silhouette stack
+
+    __device__ __forceinline__ v3 nrm(int f) const { return mk(g[f].nx, g[f].ny, g[f].nz); }
+    __device__ __forceinline__ double dist(int f) const { return g[f].d; }
+    __device__ __forceinline__ void set_face(int f, v3 n, double d)
+    {
+        g[f].nx = n.x;
+        g[f].ny = n.y;
+        g[f].nz = n.z;
+        g[f].d = d;
+    }
+    __device__ __forceinline__ double hkey(int i) const { return hd[i]; }
+    __device__ __forceinline__ void set_hkey(int i, double v) { hd[i] = v; }
+    __device__ __forceinline__ v3 pt(int i) const { return mk(px[i], py[i], pz[i]); }
+    __device__ __forceinline__ void set_pt(int i, v3 p)
+    {
+        px[i] = p.x;
+        py[i] = p.y;
+        pz[i] = p.z;
+    }
 };
 using epa_store_fast = epa_store<192, 68, uint8_t>;    // every hull solve seen so far fits
 using epa_store_lds = epa_store<896, 68, uint16_t>;    // overflow path, in LDS
@@ -512,8 +531,8 @@ template <class S> struct epa_solver
     int npoly = 0, nfaces = 0, nheap = 0;
     __device__ explicit epa_solver(S &store) : m(store) {}
 
-    __device__ __forceinline__ v3 P(int i) const { return mk(m.px[i], m.py[i], m.pz[i]); }
-    __device__ __forceinline__ v3 N(int f) const { return mk(m.g[f].nx, m.g[f].ny, m.g[f].nz); }
+    __device__ __forceinline__ v3 P(int i) const { return m.pt(i); }
+    __device__ __forceinline__ v3 N(int f) const { return m.nrm(f); }
 
     __device__ void init_face(int fi, int i, int j, int k, int opp) // :273-297
     {
@@ -543,10 +562,7 @@ template <class S> struct epa_solver
         mt.obs = 0;
         mt.pad = 0;
         m.fm[fi] = mt;
-        m.g[fi].nx = nrm.x;
-        m.g[fi].ny = nrm.y;
-        m.g[fi].nz = nrm.z;
-        m.g[fi].d = dot(nrm, pi);
+        m.set_face(fi, nrm, dot(nrm, pi));
     }
     // init_face(fi, s, e, k, -1), the link to adj across (s, e) (:305-313) and push(fi)
     // for the new face (s, e, k) on a horizon edge. k is the new point, whose coordinates
@@ -573,10 +589,7 @@ template <class S> struct epa_solver
         mt.obs = 0;
         mt.pad = 0;
         m.fm[fi] = mt;
-        m.g[fi].nx = nrm.x;
-        m.g[fi].ny = nrm.y;
-        m.g[fi].nz = nrm.z;
-        m.g[fi].d = d;
+        m.set_face(fi, nrm, d);
         m.fm[adj].a[vidx(adj, e)] = static_cast<typename S::idx_t>(fi);
         ++nheap;
         sift_up(nheap - 1, fi, d);
@@ -592,20 +605,20 @@ template <class S> struct epa_solver
         int parent = (hole - 1) / 2;
         while (hole > 0)
         {
-            double pd = m.hd[parent];
+            double pd = m.hkey(parent);
             if (!(pd > vd)) break;
-            m.hd[hole] = pd;
+            m.set_hkey(hole, pd);
             m.hf[hole] = m.hf[parent];
             hole = parent;
             parent = (hole - 1) / 2;
         }
-        m.hd[hole] = vd;
+        m.set_hkey(hole, vd);
         m.hf[hole] = static_cast<typename S::idx_t>(vf);
     }
     __device__ __forceinline__ void push(int f)
     {
         ++nheap;
-        sift_up(nheap - 1, f, m.g[f].d);
+        sift_up(nheap - 1, f, m.dist(f));
     }
     __device__ int pop()
     {
@@ -616,26 +629,26 @@ template <class S> struct epa_solver
             if (len > 1)
             {
                 int vf = m.hf[len - 1];
-                double vd = m.hd[len - 1];
+                double vd = m.hkey(len - 1);
                 int n = len - 1;
                 int hole = 0, child = 0;
                 while (child < (n - 1) / 2)
                 {
                     child = 2 * (child + 1);
-                    double rd = m.hd[child], ld = m.hd[child - 1];
+                    double rd = m.hkey(child), ld = m.hkey(child - 1);
                     if (rd > ld)
                     {
                         child--;
                         rd = ld;
                     }
-                    m.hd[hole] = rd;
+                    m.set_hkey(hole, rd);
                     m.hf[hole] = m.hf[child];
                     hole = child;
                 }
                 if ((n & 1) == 0 && child == (n - 2) / 2)
                 {
                     child = 2 * (child + 1);
-                    m.hd[hole] = m.hd[child - 1];
+                    m.set_hkey(hole, m.hkey(child - 1));
                     m.hf[hole] = m.hf[child - 1];
                     hole = child - 1;
                 }
@@ -662,7 +675,7 @@ template <class S> struct epa_solver
                 int nb = cm.a[i];
                 if (nb == S::null) continue;
                 if (m.fm[nb].obs) continue;
-                if (dot(N(nb), p) > m.g[nb].d + tol)
+                if (dot(N(nb), p) > m.dist(nb) + tol)
                 {
                     m.fm[nb].obs = 1;
                     m.stk[sp++] = static_cast<typename S::idx_t>(nb); // each face is pushed at most once
@@ -716,7 +729,7 @@ template <class S> struct epa_solver
         const v3 p0a = support_at(A, m.pia[i0]), p1a = support_at(A, m.pia[i1]), p2a = support_at(A, m.pia[i2]);
         const v3 p0b = support_at(B, m.pib[i0]), p1b = support_at(B, m.pib[i1]), p2b = support_at(B, m.pib[i2]);
         v3 nrm = N(fi);
-        double fd = m.g[fi].d;
+        double fd = m.dist(fi);
         v3 pm = nrm * fd;
         v3 v0 = q1 - q0, v1 = q2 - q0, v2 = pm - q0;
         double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
@@ -740,9 +753,7 @@ template <class S> struct epa_solver
     }
     __device__ __forceinline__ void put_point(int i, const spt &p)
     {
-        m.px[i] = p.p.x;
-        m.py[i] = p.p.y;
-        m.pz[i] = p.p.z;
+        m.set_pt(i, p.p);
         m.pia[i] = p.ia;
         m.pib[i] = p.ib;
     }
@@ -768,7 +779,7 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         int mf = e.pop();
         if (mf < 0) break;
         v3 nrm = e.N(mf);
-        double fd = store.g[mf].d;
+        double fd = store.dist(mf);
         spt p = msupport<1, UV>(A, B, nrm, 0);
         double pd = dot(nrm, p.p);
         if (pd - fd < tol)
