@@ -664,11 +664,11 @@ template <class S> struct epa_solver
     {
         const double tol = 1e-6;
         int sp = 0, nh = 0;
-        m.stk[sp++] = static_cast<typename S::idx_t>(f0);
         m.fm[f0].obs = 1;
-        while (sp > 0)
+        // f0 first, then the faces on the stack (f0 needs no trip through the stack)
+        int cur = f0;
+        for (;;)
         {
-            int cur = m.stk[--sp];
             typename S::meta cm = m.fm[cur];
             for (int i = 0; i < 3; ++i)
             {
@@ -692,6 +692,8 @@ template <class S> struct epa_solver
                     ++nh;
                 }
             }
+            if (sp == 0) break;
+            cur = m.stk[--sp];
         }
         return nh;
     }
@@ -765,6 +767,7 @@ template <class S> struct epa_solver
 template <bool UV = true, class S>
 __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store, double *out)
 {
+    static_assert(S::pcap <= 256, "point indices are packed in bytes");
     if (x.n < 4 && !pad_simplex<1, UV>(A, B, x, 0)) return 0;
     epa_solver<S> e(store);
     e.put_point(0, x.s0);
@@ -795,22 +798,50 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
         e.put_point(pi, p);
         // new faces on the horizon, linked and pushed in edge order (:481-498)
         int base = e.nfaces;
+        // the first 8 horizon edges' end points (point indices, below 256) are also kept
+        // in registers, one byte each, for the linking loop below
+        uint64_t hs8 = 0, he8 = 0;
         for (int h = 0; h < nh; ++h)
         {
             typename S::edge ed = store.eg[h];
             e.new_face(base + h, ed.s, ed.e, pi, p.p, ed.adj);
+            if (h < 8)
+            {
+                hs8 |= static_cast<uint64_t>(ed.s) << (8 * h);
+                he8 |= static_cast<uint64_t>(ed.e) << (8 * h);
+            }
         }
         e.nfaces += nh;
         // link(base + i, base + j, he_i, pi) and its mirror: a new face is (s, e, pi), so
         // its edge (e, pi) is a[1] and its edge (pi, s) is a[2]
         for (int i = 0; i < nh; ++i)
         {
-            typename S::edge ei = store.eg[i];
-            int hs_i = ei.s, he_i = ei.e;
+            int hs_i, he_i;
+            if (nh <= 8)
+            {
+                hs_i = static_cast<int>((hs8 >> (8 * i)) & 0xFFu);
+                he_i = static_cast<int>((he8 >> (8 * i)) & 0xFFu);
+            }
+            else
+            {
+                typename S::edge ei = store.eg[i];
+                hs_i = ei.s;
+                he_i = ei.e;
+            }
             for (int j = i + 1; j < nh; ++j)
             {
-                typename S::edge ej = store.eg[j];
-                int hs_j = ej.s, he_j = ej.e;
+                int hs_j, he_j;
+                if (nh <= 8)
+                {
+                    hs_j = static_cast<int>((hs8 >> (8 * j)) & 0xFFu);
+                    he_j = static_cast<int>((he8 >> (8 * j)) & 0xFFu);
+                }
+                else
+                {
+                    typename S::edge ej = store.eg[j];
+                    hs_j = ej.s;
+                    he_j = ej.e;
+                }
                 if (he_i == hs_j)
                 {
                     store.fm[base + i].a[1] = static_cast<typename S::idx_t>(base + j);
