@@ -534,40 +534,10 @@ template <class S> struct epa_solver
     __device__ __forceinline__ v3 P(int i) const { return m.pt(i); }
     __device__ __forceinline__ v3 N(int f) const { return m.nrm(f); }
 
-    __device__ void init_face(int fi, int i, int j, int k, int opp) // :273-297
-    {
-        int v1 = j, v2 = k;
-        v3 pi = P(i);
-        v3 ab = P(j) - pi;
-        v3 ac = P(k) - pi;
-        v3 nrm = cross(ab, ac);
-        if (sqn(nrm) < 1e-12)
-            nrm = mk(0.0, 0.0, 0.0);
-        else
-            nrm = normalized(nrm);
-        if (opp >= 0 && dot(nrm, P(opp) - pi) > 0.0)
-        {
-            int t = v1;
-            v1 = v2;
-            v2 = t;
-            nrm = -nrm;
-        }
-        typename S::meta mt;
-        mt.v[0] = static_cast<typename S::idx_t>(i);
-        mt.v[1] = static_cast<typename S::idx_t>(v1);
-        mt.v[2] = static_cast<typename S::idx_t>(v2);
-        mt.a[0] = S::null;
-        mt.a[1] = S::null;
-        mt.a[2] = S::null;
-        mt.obs = 0;
-        mt.pad = 0;
-        m.fm[fi] = mt;
-        m.set_face(fi, nrm, dot(nrm, pi));
-    }
-    // init_face(fi, s, e, k, -1), the link to adj across (s, e) (:305-313) and push(fi)
-    // for the new face (s, e, k) on a horizon edge. k is the new point, whose coordinates
-    // pk are still in registers. The face's vertices stay in the order (s, e, k), so s is
-    // at index 0 and the link to adj is a[0].
+    // init_face(fi, s, e, k, -1) (:273-297), the link to adj across (s, e) (:305-313)
+    // and push(fi) for the new face (s, e, k) on a horizon edge. k is the new point,
+    // whose coordinates pk are still in registers. The face's vertices stay in the order
+    // (s, e, k), so s is at index 0 and the link to adj is a[0].
     __device__ __forceinline__ void new_face(int fi, int s, int e, int k, v3 pk, int adj)
     {
         v3 ps = P(s);
@@ -697,30 +667,77 @@ template <class S> struct epa_solver
         }
         return nh;
     }
-    __device__ void initial_tetrahedron() // :355-388
+    // :355-388, from the four points q (already stored as points 0-3), held in registers:
+    // the faces' vertices and links are worked out in registers and each face record is
+    // written once
+    __device__ void initial_tetrahedron(const v3 (&q)[4])
     {
         const int def[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
+        int fv[4][3];
+#pragma unroll
         for (int f = 0; f < 4; ++f)
         {
-            init_face(f, def[f][0], def[f][1], def[f][2], def[f][3]);
+            // init_face(f, def[f][0], def[f][1], def[f][2], opposite point def[f][3]) (:273-297)
+            const int i = def[f][0];
+            int v1 = def[f][1], v2 = def[f][2];
+            const v3 pi = q[i];
+            v3 nrm = cross(q[v1] - pi, q[v2] - pi);
+            if (sqn(nrm) < 1e-12)
+                nrm = mk(0.0, 0.0, 0.0);
+            else
+                nrm = normalized(nrm);
+            if (dot(nrm, q[def[f][3]] - pi) > 0.0)
+            {
+                const int t = v1;
+                v1 = v2;
+                v2 = t;
+                nrm = -nrm;
+            }
+            fv[f][0] = i;
+            fv[f][1] = v1;
+            fv[f][2] = v2;
+            const double d = dot(nrm, pi);
+            m.set_face(f, nrm, d);
             nfaces = f + 1;
-            push(f);
+            ++nheap;
+            sift_up(nheap - 1, f, d);
         }
+        int fa[4][3];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            fa[f][0] = fa[f][1] = fa[f][2] = S::null;
+#pragma unroll
         for (int i = 0; i < 4; ++i)
+#pragma unroll
             for (int j = i + 1; j < 4; ++j)
+#pragma unroll
                 for (int e1 = 0; e1 < 3; ++e1)
                 {
-                    int u1 = m.fm[i].v[e1], v1 = m.fm[i].v[(e1 + 1) % 3];
+                    const int u1 = fv[i][e1], w1 = fv[i][(e1 + 1) % 3];
+#pragma unroll
                     for (int e2 = 0; e2 < 3; ++e2)
                     {
-                        int u2 = m.fm[j].v[e2], v2 = m.fm[j].v[(e2 + 1) % 3];
-                        if (u1 == v2 && v1 == u2)
+                        const int u2 = fv[j][e2], w2 = fv[j][(e2 + 1) % 3];
+                        if (u1 == w2 && w1 == u2)
                         {
-                            m.fm[i].a[e1] = static_cast<typename S::idx_t>(j);
-                            m.fm[j].a[e2] = static_cast<typename S::idx_t>(i);
+                            fa[i][e1] = j;
+                            fa[j][e2] = i;
                         }
                     }
                 }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+        {
+            typename S::meta mt;
+            for (int c = 0; c < 3; ++c)
+            {
+                mt.v[c] = static_cast<typename S::idx_t>(fv[f][c]);
+                mt.a[c] = static_cast<typename S::idx_t>(fa[f][c]);
+            }
+            mt.obs = 0;
+            mt.pad = 0;
+            m.fm[f] = mt;
+        }
     }
     // :424-454; out = normal3, world_a3, world_b3, depth
     __device__ void barycentric(int fi, const gshape &A, const gshape &B, double *out) const
@@ -775,7 +792,8 @@ __device__ int epa_solve(const gshape &A, const gshape &B, simplex &x, S &store,
     e.put_point(2, x.s2);
     e.put_point(3, x.s3);
     e.npoly = 4;
-    e.initial_tetrahedron();
+    const v3 q[4] = {x.s0.p, x.s1.p, x.s2.p, x.s3.p};
+    e.initial_tetrahedron(q);
     const double tol = 1e-6;
     for (int it = 0; it < 64; ++it)
     {
