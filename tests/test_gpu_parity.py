@@ -239,6 +239,35 @@ def test_gjk_epa_batch_random_bitexact(pk, oracle):
     g.close()
 
 
+def test_gjk_epa_batch_deep_spheres_bitexact(pk, oracle):
+    """Deeply overlapping 16x32 UV spheres: long EPA runs (about 36 expansions per
+    solve) over polytopes of many faces, with horizons of up to 8 edges."""
+    rng = np.random.default_rng(23)
+    sph = oracle.sphere_verts(0.5, 16, 32)
+    A, B = [], []
+    for _ in range(300):
+        ca = rng.uniform(-0.1, 0.1, 3)
+        u = rng.normal(size=3)
+        cb = ca + u / np.linalg.norm(u) * rng.uniform(0.0, 0.7)
+        A.append((2, np.concatenate([ca, _rand_quat(rng)]), sph))
+        B.append((2, np.concatenate([cb, _rand_quat(rng)]), sph))
+    g = pk.GpuWorld()
+    hit, out = g.gjk_epa_batch(A, B)
+    n_contact = 0
+    for i in range(len(A)):
+        r = oracle.gjk_epa(_oracle_shape(oracle, A[i]), _oracle_shape(oracle, B[i]))
+        assert r is not None, i  # centres at most 0.7 apart: the hulls (inradius 0.49) overlap
+        if r == "degenerate":
+            assert hit[i] == 2, i
+        else:
+            n_contact += 1
+            assert hit[i] == 1, i
+            exp = np.concatenate([r["normal"], r["world_a"], r["world_b"], [r["depth"]]])
+            _assert_same(i, "gjk_epa", out[i], exp)
+    assert n_contact > 250
+    g.close()
+
+
 def test_gjk_epa_batch_known_depths(pk, oracle):
     """Axis-aligned boxes: the EPA depth is the smallest face distance of the
     Minkowski box A-B (the reference's tests/epa/epa_test.cpp:57-158 rely on it)."""
