@@ -529,4 +529,73 @@ void pko_inverse3(const double *m, double *out)
         for (int c = 0; c < 3; ++c) out[3 * r + c] = b.m[r][c];
 }
 
+// ---- stage-level hooks for the contact-path pins ----
+// One stage of world::step (src/world.cpp:20-56): 0 integrate_vel and leaf update,
+// 1 pair-set update, 2 narrowphase, 3 contact solve, 4 integrate_pos.
+int pko_run_stage(void *h, int stage, double dt)
+{
+    auto *w = static_cast<world *>(h);
+    switch (stage)
+    {
+    case 0: w->stage_integrate_vel(dt); return 1;
+    case 1: w->stage_pairs(); return 1;
+    case 2: w->stage_narrow(); return 1;
+    case 3: w->stage_solve(dt); return 1;
+    case 4: w->stage_integrate_pos(dt); return 1;
+    default: return 0;
+    }
+}
+
+namespace
+{
+contact ld_contact(const double *d) // the 13-double layout of pko_get_manifolds
+{
+    contact k;
+    k.normal = ld3(d);
+    k.local_a = ld3(d + 3);
+    k.local_b = ld3(d + 6);
+    k.depth = d[9];
+    k.ln = d[10];
+    k.lt1 = d[11];
+    k.lt2 = d[12];
+    return k;
+}
+void st_contact(double *d, const contact &k)
+{
+    st3(d, k.normal);
+    st3(d + 3, k.local_a);
+    st3(d + 6, k.local_b);
+    d[9] = k.depth;
+    d[10] = k.ln;
+    d[11] = k.lt1;
+    d[12] = k.lt2;
+}
+} // namespace
+
+// Replaces the cached manifold of the active pair `key` by n contacts (13 doubles each,
+// as pko_get_manifolds writes them). Returns 0 if the pair is not active or n > 4.
+int pko_set_manifold(void *h, uint64_t key, int n, const double *data)
+{
+    auto *w = static_cast<world *>(h);
+    auto it = std::lower_bound(w->active.begin(), w->active.end(), key);
+    if (it == w->active.end() || *it != key || n < 0 || n > 4) return 0;
+    manifold &m = w->manifolds[static_cast<std::size_t>(it - w->active.begin())];
+    m.n = n;
+    for (int i = 0; i < n; ++i) m.c[static_cast<std::size_t>(i)] = ld_contact(data + 13 * i);
+    return 1;
+}
+
+// manifold::add_contact (collision_phases.h:124-130) on a manifold holding n_old
+// contacts; writes the result (up to 4 contacts) to out and returns its count.
+int pko_manifold_add(int n_old, const double *old_contacts, const double *new_contact, double *out)
+{
+    if (n_old < 0 || n_old > 4) return -1;
+    manifold m;
+    m.n = n_old;
+    for (int i = 0; i < n_old; ++i) m.c[static_cast<std::size_t>(i)] = ld_contact(old_contacts + 13 * i);
+    m.add_contact(ld_contact(new_contact));
+    for (int i = 0; i < m.n; ++i) st_contact(out + 13 * i, m.c[static_cast<std::size_t>(i)]);
+    return m.n;
+}
+
 } // extern "C"

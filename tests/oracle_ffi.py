@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# PKO_LIB names another build of the oracle: the planted-mistake test
+# (tests/test_oracle_mutants.py) runs the pin tests against altered copies
+LIB_PATH = os.environ.get("PKO_LIB") or os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 _lib = None
 
@@ -84,6 +86,12 @@ def lib():
         L.pko_get_joints.argtypes = [C.c_void_p, U32, D, I32, C.c_size_t]
         L.pko_get_joint_colours.restype = C.c_size_t
         L.pko_get_joint_colours.argtypes = [C.c_void_p, U32, C.c_size_t]
+        L.pko_run_stage.restype = C.c_int
+        L.pko_run_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.pko_set_manifold.restype = C.c_int
+        L.pko_set_manifold.argtypes = [C.c_void_p, C.c_uint64, C.c_int, D]
+        L.pko_manifold_add.restype = C.c_int
+        L.pko_manifold_add.argtypes = [C.c_int, D, D, D]
         _lib = L
     return _lib
 
@@ -185,6 +193,16 @@ def ray_box(origin, direction, mn, mx, max_dist=float("inf")):
     return float(t[0]) if r else None
 
 
+def manifold_add(old, new):
+    """manifold::add_contact (collision_phases.h:124-130): old is (n, 13) contacts in the
+    pko_get_manifolds layout, new one contact; returns the resulting (m, 13) contacts."""
+    old = f64(np.asarray(old, float).reshape(-1, 13))
+    out = np.zeros((4, 13))
+    m = lib().pko_manifold_add(len(old), dp(old), dp(f64(np.asarray(new, float).reshape(13))), dp(out))
+    assert m >= 0
+    return out[:m].copy()
+
+
 def inverse3(m):
     out = np.zeros(9)
     lib().pko_inverse3(dp(f64(m).reshape(9)), dp(out))
@@ -238,6 +256,18 @@ class OracleWorld:
 
     def step(self, dt, nsteps=1):
         return self.L.pko_step(self.h, dt, nsteps)
+
+    STAGES = {"integrate_vel": 0, "pairs": 1, "narrow": 2, "solve": 3, "integrate_pos": 4}
+
+    def run_stage(self, name, dt=0.0):
+        """One stage of world::step (src/world.cpp:20-56)."""
+        assert self.L.pko_run_stage(self.h, self.STAGES[name], dt) == 1
+
+    def set_manifold(self, key, contacts):
+        """Replaces the cached manifold of active pair `key` by contacts ((n, 13), the
+        layout of manifolds())."""
+        c = f64(np.asarray(contacts, float).reshape(-1, 13))
+        assert self.L.pko_set_manifold(self.h, int(key), len(c), dp(c) if len(c) else None) == 1
 
     def bodies(self):
         n = self.L.pko_num_bodies(self.h)
