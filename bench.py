@@ -1,14 +1,18 @@
 """Benchmark of the collision-and-contact step on one MI355X.
 
-Workload (default): narrowphase stress — 262,144 convex hulls of 8-32 points on
-ellipsoids with semi-axes U[0.05,0.3] m, centres from Gaussian clusters of 512 bodies
-in a 200 m cube (~4 candidate pairs per body); one step = LBVH rebuild + pair query
-+ GJK/EPA + manifold update, every leaf moved, no solver or integration. FP64.
+Workload (default): 4,096 independent worlds of 16-4,096 bodies (log-uniform; 40% UV
+spheres, 40% boxes, 20% hulls of 8-16 points) over static ground boxes, stepped
+together as one batch in the mixed-precision form (FP32 broadphase leaf tests rounded
+outward, FP32 certified GJK separation, FP32 contact solve; EPA, manifolds and body
+state FP64). Beside it, in the same run: the same worlds batched in FP64, the plain
+FP64 drop-in (worlds stepped one at a time, one context each) and the oracle on the
+host's cores.
 
-Prints one JSON line. value = body-steps/s (bodies x timed steps / wall time of the
-timed steps, inputs resident in HBM); contact-pairs/s rides along in the config.
-The roofline is taken for the dominant kernel from HIP events recorded on the
-library's own stream around that kernel during the timed steps.
+Prints one JSON line. value = body-steps/s (dynamic bodies x timed steps / wall time of
+the timed steps, inputs resident in HBM); contact-pairs/s (pairs through the GJK stage)
+rides along in the config. The roofline is taken for the stage with the largest device
+time, from HIP events recorded on the library's own stream, against that stage's
+compulsory HBM bytes (stage_bytes below); every stage's figure is listed too.
 """
 from __future__ import annotations
 
@@ -41,7 +45,7 @@ WORKLOADS = {
                        "falling into a 64-box hopper, dt=1/240, 10 PGS iterations, FP64"),
     "batched": (3, "4,096 independent worlds of 16-4,096 bodies (log-uniform; 40% UV spheres "
                    "16x32, 40% boxes, 20% 8-16-point hulls) over static ground boxes, dt=1/60, "
-                   "10 PGS iterations, one batched step per launch sequence, FP64"),
+                   "10 PGS iterations, stepped together as one batch"),
     "rays": (1, "closest-hit raycasts: 4,194,304 rays (origins uniform in the scene box grown by "
                 "10 m; half aimed at a random body, half uniform directions; unbounded) into the "
                 "262,144-hull cluster scene's broadphase tree, FP64 slab tests, rays resident in HBM"),
@@ -103,16 +107,15 @@ def algorithmic_bytes(stage, s, pairs, hits_frac, pos=None):
     raise ValueError(stage)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hulls_262k_fp64_hbm_bytes.md")
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the default workload, from the committed
-    FETCH_SIZE/WRITE_SIZE counter passes (scaled by a copy kernel of known size)."""
+def measured_traffic(workload, precision, stage):
+    """HBM bytes per launch of a stage's dominant kernel, from the committed FETCH_SIZE /
+    WRITE_SIZE counter passes over this workload (profiles/<workload>_<precision>_hbm_bytes.md,
+    scaled by a copy kernel of known size); None when no such summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_hbm_bytes.md")
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == stage:
                 return float(f[2])
     except OSError:
         pass
@@ -133,44 +136,149 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
-def fp32_drift(s, steps):
-    """FP32 against FP64 solve from the same start. per_step: the largest one-step
-    position difference, max over bodies of |x32 - x64| / max(|x64|, 1 m), with the
-    FP32 world reset to the FP64 bodies after every step; trajectory: the same
-    difference without resets after step 1, 10 and `steps`."""
+def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed):
+    """Compulsory HBM bytes of each stage's launches in one step, with the step's own
+    counts (n bodies, N dynamic, M moved leaves, K candidate pairs, P pairs, H GJK hits,
+    C manifold contacts, R solver rows; s_r = 4 for FP32 rows, else 8):
+      integrate_vel     2n + 460 N + 48 M + 48 per shape (k_integrate_vel: mass, inverse
+                        mass, force and torque accumulators, velocities, world inertia and
+                        its inverse, pose, leaf box read; velocities and accumulators
+                        written, the leaf box when it moves)
+      lbvh_build        253 n (+32 n with FP32 leaf boxes): leaf boxes read, Morton keys
+                        and ids written and read, leaf records written, 2 internal nodes of
+                        32 B written and read per leaf
+      pair_update       48 M + 80 n + 36 K + 20 P: the moving leaves' boxes, the tree read
+                        once, candidates written, sorted and matched, the previous list
+                        read and the new one with its sources written
+      gjk, epa          algorithmic_bytes()
+      manifold          22 P + 208 C + 192 H: keys, sources, status, counts and events;
+                        contacts read and written; hits' contact data and poses
+      solver            496 R: contacts read, both bodies' pose, velocity and inverse
+                        inertia read for the rows, impulses and velocities written
+      solve_iterations  R (38 s_r + 96): every row read once and its bodies' velocities
+                        read and written once (rows that stay on chip across passes)
+      integrate_pos     n + 448 N (k_integrate_pos: pose, velocities and local inertia
+                        read; pose and world inertia written)"""
+    n = len(s["shape"])
+    nd = n - int(s["n_static"])
+    n_shapes = len(s["shape_offsets"]) - 1
+    sr = 4 if mixed else 8
+    M, K, P, H, C, R = (c[k] for k in ("moved", "candidates", "pairs", "gjk_hits", "contacts", "rows"))
+    out = {
+        "integrate_vel": ("k_integrate_vel", 2 * n + 460 * nd + 48 * M + 48 * n_shapes),
+        "lbvh_build": ("LBVH build (k_morton, k_rs_*, k_build_internal, k_leaf_gather, k_refit, k_span)",
+                       (253 + (32 if mixed else 0)) * n),
+        "pair_update": ("pair-set update (k_pair_query, sort, k_match, k_prev_*, k_merge)",
+                        48 * M + 80 * n + 36 * K + 20 * P),
+        "gjk": ("GJK stage (k_gjk_cull" + (", k_gjk32" if mixed else "") + ", k_gjk)",
+                algorithmic_bytes("gjk", s, pairs, hits_frac, pos) if len(pairs) else 0.0),
+        "epa": ("k_epa (+ larger-store reruns)", algorithmic_bytes("epa", s, pairs, hits_frac) if len(pairs) else 0.0),
+        "manifold": ("k_manifold", 22 * P + 208 * C + 192 * H),
+        "solver": ("contact solver stage", 496 * R),
+        "solve_iterations": ("k_solve_islands" + ("<true> (FP32 rows)" if mixed else "<false>"), R * (38 * sr + 96)),
+        "integrate_pos": ("k_integrate_pos", n + 448 * nd),
+    }
+    return out
+
+
+def roofline_of(stage_ms, steps, models):
+    """Achieved GB/s and HBM fraction of every stage, and the dominant one: the stage with
+    the largest device time, with the solver stage split into its iteration kernels
+    (solve_iterations) and the rest (rows, colouring, islands)."""
+    ms = dict(stage_ms)
+    rest = max(ms.get("solver", 0.0) - ms.get("solve_iterations", 0.0), 0.0)
+    table = {}
+    for k, (name, b) in models.items():
+        t = (rest if k == "solver" else ms.get(k, 0.0)) / steps
+        gbs = b / (t / 1e3) / 1e9 if t > 0 else 0.0
+        table[k] = {"kernel": name if k != "solver" else "contact solver stage without its iterations "
+                    "(k_row_flags, colouring, islands, k_setup, k_writeback)",
+                    "avg_launch_ms": round(t, 4), "algorithmic_bytes": float(b), "achieved_GBs": round(gbs, 2),
+                    "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    dom = max(table, key=lambda k: table[k]["avg_launch_ms"])
+    return dom, table
+
+
+def worlds_prefix(s, k):
+    """The first k worlds of a batched scene as a scene of their own (all shapes kept)."""
+    fb = s["first_body"]
+    b1 = int(fb[k])
+    out = {key: s[key][:b1].copy() for key in ("is_static", "shape", "pos", "vel", "angvel", "quat", "mass",
+                                               "inertia", "restitution", "friction")}
+    for key in ("shape_offsets", "verts", "dt", "iterations", "steps", "flags"):
+        out[key] = s[key]
+    out["n_static"] = int(np.sum(out["is_static"] != 0))
+    out["first_body"] = fb[:k + 1].copy()
+    return out
+
+
+def precision_drift(s, steps, flags_lo):
+    """A lower-precision form (flags_lo added) against the FP64 step from the same start.
+    per_step: the largest one-step position difference, max over bodies of
+    |x_lo - x64| / max(|x64|, 1 m), with the lower-precision world reset to the FP64
+    bodies after every step; trajectory: the same difference without resets after step
+    1, 10 and `steps`."""
     import physkit_amd as pk
 
     def pair():
         w64 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"])
-        w32 = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"] | pk.FLAG_FP32_SOLVE)
-        for w in (w64, w32):
+        wlo = pk.GpuWorld(iterations=s["iterations"], flags=s["flags"] | flags_lo)
+        for w in (w64, wlo):
             w.load_scene(s)
             if len(s["first_body"]) > 2:
                 w.set_worlds(s["first_body"])
-        return w64, w32
+        return w64, wlo
 
     def err(a, b):
         return float(np.max(np.abs(a - b) / np.maximum(np.abs(a), 1.0)))
 
     out = {"per_step": 0.0, "trajectory": {}}
     ids = np.arange(len(s["shape"]), dtype=np.uint32)
-    w64, w32 = pair()
+    w64, wlo = pair()
     for k in range(1, steps + 1):
         w64.step(s["dt"])
-        w32.step(s["dt"])
+        wlo.step(s["dt"])
         a = w64.bodies()
-        out["per_step"] = max(out["per_step"], err(a["pos"], w32.bodies()["pos"]))
-        w32.set_state(ids, pos=a["pos"], vel=a["vel"], angvel=a["angvel"], quat=a["quat"])
+        out["per_step"] = max(out["per_step"], err(a["pos"], wlo.bodies()["pos"]))
+        wlo.set_state(ids, pos=a["pos"], vel=a["vel"], angvel=a["angvel"], quat=a["quat"])
     w64.close()
-    w32.close()
-    w64, w32 = pair()
+    wlo.close()
+    w64, wlo = pair()
     for k in range(1, steps + 1):
         w64.step(s["dt"])
-        w32.step(s["dt"])
+        wlo.step(s["dt"])
         if k in (1, 10, steps):
-            out["trajectory"][f"step_{k}"] = err(w64.bodies()["pos"], w32.bodies()["pos"])
+            out["trajectory"][f"step_{k}"] = err(w64.bodies()["pos"], wlo.bodies()["pos"])
     w64.close()
-    w32.close()
+    wlo.close()
+    return out
+
+
+def timed_run(pk, s, flags, warmup, steps):
+    """Loads s into a context with flags, runs warmup untimed steps, then `steps` timed
+    steps (each pkg_step_timed call returns after its device work has finished, so the
+    timed region is bracketed by device synchronisation). Returns wall seconds, summed
+    stage milliseconds, pair and hit totals, the last counters, pairs and positions."""
+    g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
+    g.load_scene(s)
+    if len(s["first_body"]) > 2:
+        g.set_worlds(s["first_body"])
+    for _ in range(warmup):
+        g.step(s["dt"])
+    stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
+    pairs_total = hits_total = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms = g.step_timed(s["dt"], 1)
+        for k, v in ms.items():
+            stage[k] += v
+        c = g.counters()
+        pairs_total += c["pairs"]
+        hits_total += c["gjk_hits"]
+    wall = time.perf_counter() - t0
+    out = {"wall": wall, "stage": stage, "pairs_total": pairs_total, "hits_total": hits_total,
+           "counters": g.counters(), "pairs": g.pairs(), "pos": g.bodies()["pos"]}
+    g.close()
     return out
 
 
@@ -390,14 +498,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hulls", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="batched", choices=sorted(WORKLOADS))
     ap.add_argument("--n", type=int, default=0, help="body count override (smaller runs)")
+    ap.add_argument("--precision", default="auto", choices=("auto", "fp64", "mixed", "fp32-solve"),
+                    help="auto: the mixed-precision form for the batched workload, else FP64 (the drop-in); "
+                         "mixed: FP32 leaf tests, certified FP32 GJK separation, FP32 solve; fp32-solve: "
+                         "only the solve in FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--fp32-solve", action="store_true",
-                    help="FP32 contact solve with FP64 velocity accumulation; reports the position "
-                         "drift against the FP64 solve")
-    ap.add_argument("--no-dropin", action="store_true",
-                    help="batched workloads: skip the per-world drop-in comparison")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the comparison runs (batched FP64, per-world drop-in, precision drift)")
     ap.add_argument("--n-rays", type=int, default=RAYS, help="rays workload: rays per launch")
     ap.add_argument("--copy-check", type=int, default=1024, metavar="MIB",
                     help="after the timed steps, time a plain device copy of MIB MiB (measured "
@@ -415,76 +524,72 @@ def main():
     if args.workload == "rays":
         print(json.dumps(bench_rays(args, pk, s)), flush=True)
         return
-    flags = s["flags"] | (pk.FLAG_FP32_SOLVE if args.fp32_solve else 0)
-    g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
-    g.load_scene(s)
-    if len(s["first_body"]) > 2:
-        g.set_worlds(s["first_body"])
-    dt = s["dt"]
+    batched = len(s["first_body"]) > 2
+    prec = args.precision if args.precision != "auto" else ("mixed" if batched else "fp64")
+    extra = {"fp64": 0, "mixed": pk.FLAG_MIXED, "fp32-solve": pk.FLAG_FP32_SOLVE}[prec]
+    mixed = prec == "mixed"
     n_dyn = len(s["shape"]) - int(s["n_static"])
-    print(f"[bench] workload={args.workload} bodies={len(s['shape'])} steps={args.steps} "
+    print(f"[bench] workload={args.workload} precision={prec} bodies={len(s['shape'])} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
-    # pkg_step / pkg_step_timed return only when the step's device work is complete, so
-    # the timed region is bracketed by device synchronisation on both sides
-    for _ in range(args.warmup):
-        g.step(dt)
-    pairs_total = 0
-    hits_total = 0
-    stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ms = g.step_timed(dt, 1)
-        for k, v in ms.items():
-            stage[k] += v
-        c = g.counters()
-        pairs_total += c["pairs"]
-        hits_total += c["gjk_hits"]
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    c = g.counters()
-    pairs = g.pairs()
+    r = timed_run(pk, s, s["flags"] | extra, args.warmup, args.steps)
+    wall, stage, c, pairs = r["wall"], r["stage"], r["counters"], r["pairs"]
+    print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
-
-    step_b = step_bytes(s, c, unique_vertices=(args.workload in ("hulls", "batched")))
-
-    # dominant kernel of the narrowphase and its roofline
-    dom = max(("gjk", "epa"), key=lambda k: stage[k])
-    t_launch = stage[dom] / 1e3 / args.steps
-    alg = algorithmic_bytes(dom, s, pairs, hits_frac, g.bodies()["pos"]) if len(pairs) else 0.0
-    achieved = alg / t_launch / 1e9 if t_launch > 0 else 0.0
-    roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic("k_" + dom) if (args.workload == "hulls" and not args.n) else None,
-            "algorithmic_bytes_per_launch": alg,
-            "avg_launch_ms": round(t_launch * 1e3, 4)}
+    models = stage_bytes(s, c, pairs, hits_frac, r["pos"], s["iterations"], mixed or prec == "fp32-solve")
+    dom, table = roofline_of(stage, args.steps, models)
+    d = table[dom]
+    roof = {"bound": "hbm", "stage": dom, "kernel": d["kernel"], "achieved": d["achieved_GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+            "traffic": measured_traffic(args.workload, prec, dom) if not args.n else None,
+            "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_ms": d["avg_launch_ms"]}
+    step_b = step_bytes(s, c, unique_vertices=(args.workload in ("hulls", "batched")), scalar=4 if mixed else 8)
+    dtype = {"fp64": "f64", "mixed": "f32 leaf tests, GJK separation and solve; f64 EPA, manifolds, body state",
+             "fp32-solve": "f64 geometry, f32 solver rows"}[prec]
+    value = n_dyn * args.steps / wall
     out = {
         "metric": METRIC,
-        "value": round(n_dyn * args.steps / wall, 1),
+        "value": round(value, 1),
         "unit": "body-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 geometry, f32 solver rows" if args.fp32_solve else "f64", "data": "synthetic",
-        "config": {"workload": desc, "bodies": len(s["shape"]), "dynamic_bodies": n_dyn,
+        "dtype": dtype, "data": "synthetic",
+        "config": {"workload": desc, "precision": prec, "bodies": len(s["shape"]), "dynamic_bodies": n_dyn,
                    "pairs_per_step": c["pairs"], "gjk_hits_per_step": c["gjk_hits"],
                    "last_step_counters": {k: c[k] for k in ("contacts", "rows", "colours", "epa_overflow",
-                                                            "candidates", "moved")},
-                   "contact_pairs_per_s": round(pairs_total / wall, 1),
-                   "colliding_pairs_per_s": round(hits_total / wall, 1),
+                                                            "candidates", "moved", "gjk_listed", "gjk_fp64")},
+                   "contact_pairs_per_s": round(r["pairs_total"] / wall, 1),
+                   "colliding_pairs_per_s": round(r["hits_total"] / wall, 1),
                    "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stage.items()},
+                   "stage_roofline": table,
                    "step_algorithmic_bytes": step_b,
                    "step_hbm_frac": round(step_b / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 5),
                    "worlds": len(s["first_body"]) - 1,
-                   "parallelism": "1 GPU, " + ("one world" if len(s["first_body"]) == 2
+                   "parallelism": "1 GPU, " + ("one world" if not batched
                                                else f"{len(s['first_body']) - 1} worlds batched")},
         "roofline": roof,
     }
-    g.close()
-    if args.fp32_solve:
-        out["config"]["fp32_drift"] = fp32_drift(s, args.warmup + args.steps)
-    if len(s["first_body"]) > 2 and not args.no_dropin:
-        out["config"]["per_world_dropin"] = per_world_dropin(s, args.warmup, args.steps)
-        out["config"]["batched_over_per_world"] = round(out["value"] / out["config"]["per_world_dropin"]["value"], 2)
+    if not args.no_compare:
+        if batched:
+            # the same worlds batched in FP64, and the plain FP64 drop-in: worlds one at a time
+            if prec != "fp64":
+                r64 = timed_run(pk, s, s["flags"], args.warmup, args.steps)
+                v64 = n_dyn * args.steps / r64["wall"]
+                out["config"]["batched_fp64"] = {"value": round(v64, 1), "unit": "body-steps/s",
+                                                 "ms_per_step": round(r64["wall"] * 1e3 / args.steps, 3),
+                                                 "stage_ms_per_step": {k: round(v / args.steps, 4)
+                                                                       for k, v in r64["stage"].items()}}
+                out["config"]["batched_over_batched_fp64"] = round(value / v64, 3)
+                print(f"[bench] batched fp64: {r64['wall'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr,
+                      flush=True)
+            dropin = per_world_dropin(s, args.warmup, args.steps)
+            out["config"]["per_world_dropin"] = dropin
+            out["config"]["batched_over_per_world"] = round(value / dropin["value"], 2)
+        if prec != "fp64":
+            sub = worlds_prefix(s, 256) if batched else s
+            drift = precision_drift(sub, min(args.warmup + args.steps, 30), extra)
+            drift["sample"] = ("worlds 0-255" if batched else "the whole workload") + ", from the same start"
+            out["config"]["precision_drift"] = drift
     if args.copy_check:
         import ctypes
         from physkit_amd import build as pk_build
@@ -497,8 +602,7 @@ def main():
         out["copy_check"] = {"bytes": nbytes, "ms": round(ms.value, 4),
                              "GB/s": round(2 * nbytes / ms.value / 1e6, 1)}
     if not args.no_cpu:
-        out["cpu_baseline"] = (cpu_baseline_batched(s, args.warmup) if len(s["first_body"]) > 2
-                               else cpu_baseline(s))
+        out["cpu_baseline"] = (cpu_baseline_batched(s, args.warmup) if batched else cpu_baseline(s))
     print(json.dumps(out), flush=True)
 
 

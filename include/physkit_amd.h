@@ -53,6 +53,18 @@ typedef struct pkg_ctx pkg_ctx;
  * each body's velocity increment added to its FP64 velocity. Geometry (pairs, GJK,
  * EPA, manifolds) stays FP64 and bit-exact; results drift from the FP64 solve. */
 #define PKG_FLAG_FP32_SOLVE 16u
+/* FP32 certified separation ahead of the FP64 GJK (not in the reference): a pair whose
+ * shapes FP32 proves farther apart than GJK's margin, with a bound on the FP32
+ * rounding error, is given the reference's verdict (separated) without the FP64 GJK;
+ * every other pair runs the FP64 GJK. Verdicts are bit-exact. */
+#define PKG_FLAG_FP32_GJK 32u
+/* FP32 leaf boxes in the broadphase walk (not in the reference): leaf boxes rounded
+ * outward to FP32 are tested first and only their overlaps take the exact FP64 test,
+ * so the pair set is bit-exact. */
+#define PKG_FLAG_FP32_BROAD 64u
+/* The mixed-precision form: FP32 broadphase leaf tests, FP32 certified GJK separation,
+ * FP32 contact solve; EPA, manifolds and body state stay FP64. */
+#define PKG_FLAG_MIXED (PKG_FLAG_FP32_SOLVE | PKG_FLAG_FP32_GJK | PKG_FLAG_FP32_BROAD)
 
 /* Mirrors world_desc (include/physkit/core/world.h:23-49): gravity and solver
  * iterations; defaults there are (0,-9.81,0) and 10. */
@@ -105,6 +117,9 @@ typedef struct pkg_counters
     uint64_t colours;  /* solver colours                                            */
     uint64_t epa_overflow; /* pairs whose EPA needed the large-polytope path         */
     uint64_t candidates;   /* candidate pairs emitted by the tree query              */
+    uint64_t gjk_listed;   /* pairs the bounding-volume cull left to the GJK stage   */
+    uint64_t gjk_fp64;     /* of those, pairs the FP64 GJK ran (all of them unless
+                              PKG_FLAG_FP32_GJK certified the others apart)          */
 } pkg_counters;
 
 /* Replaces constructing a physkit::world (world.h:382-385). */
@@ -260,8 +275,10 @@ pkg_status pkg_query_aabb(pkg_ctx *ctx, uint32_t n_boxes, const double *boxes, c
 
 /* pkg_step with per-stage device time, measured with HIP events on the ctx stream.
  * Stages: 0 integrate_vel + leaf update, 1 LBVH build, 2 pair-set update, 3 GJK
- * kernel, 4 EPA kernel(s), 5 manifold update kernel, 6 contact solver,
- * 7 integrate_pos. stage_ms[i] is the sum over the nsteps steps (up to 8 entries). */
+ * stage (cull, FP32 pass, GJK kernel), 4 EPA kernel(s), 5 manifold update kernel,
+ * 6 contact solver (rows, colouring, islands, iterations, impulse write-back),
+ * 7 integrate_pos, 8 the solver's iteration kernels alone (part of stage 6).
+ * stage_ms[i] is the sum over the nsteps steps (up to 9 entries). */
 pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms, int n_stages);
 
 /* Standalone narrowphase: gjk_epa(a, b) (src/collision.cpp:512-518) over n shape

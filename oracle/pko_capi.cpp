@@ -585,6 +585,41 @@ int pko_set_manifold(void *h, uint64_t key, int n, const double *data)
     return 1;
 }
 
+// Leaf bounds of bodies ids[0..count) (6 doubles each: min xyz, max xyz), as a caller
+// restoring a saved world state sets them.
+void pko_set_fat(void *h, int count, const uint32_t *ids, const double *fat)
+{
+    auto *w = static_cast<world *>(h);
+    for (int t = 0; t < count; ++t)
+    {
+        body &b = w->bodies[ids[t]];
+        b.fat.min = ld3(fat + 6 * t);
+        b.fat.max = ld3(fat + 6 * t + 3);
+    }
+}
+
+// Replaces the active pair list (n keys, ascending) and its manifolds (counts and 4 x 13
+// doubles per pair, the pko_get_manifolds layout). Returns 0 if the keys are not
+// ascending or a count is out of range.
+int pko_set_pairs(void *h, size_t n, const uint64_t *keys, const int32_t *counts, const double *data)
+{
+    auto *w = static_cast<world *>(h);
+    for (size_t i = 0; i < n; ++i)
+        if ((i && keys[i] <= keys[i - 1]) || counts[i] < 0 || counts[i] > 4) return 0;
+    w->active.assign(keys, keys + n);
+    w->manifolds.assign(n, manifold{});
+    for (size_t i = 0; i < n; ++i)
+    {
+        manifold &m = w->manifolds[i];
+        m.a = static_cast<uint32_t>(keys[i] >> 32);
+        m.b = static_cast<uint32_t>(keys[i] & 0xFFFFFFFFu);
+        m.n = counts[i];
+        for (int c = 0; c < m.n; ++c)
+            m.c[static_cast<std::size_t>(c)] = ld_contact(data + (i * 4 + static_cast<size_t>(c)) * 13);
+    }
+    return 1;
+}
+
 // manifold::add_contact (collision_phases.h:124-130) on a manifold holding n_old
 // contacts; writes the result (up to 4 contacts) to out and returns its count.
 int pko_manifold_add(int n_old, const double *old_contacts, const double *new_contact, double *out)

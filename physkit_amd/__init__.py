@@ -22,6 +22,10 @@ PKG_OK = 0
 STATUS = {0: "PKG_OK", 1: "PKG_INVALID_ARG", 2: "PKG_OOM", 3: "PKG_OVERFLOW_PAIRS",
           4: "PKG_OVERFLOW_EPA", 5: "PKG_DEVICE_ERROR", 6: "PKG_NO_DEVICE"}
 FLAG_ALL_MOVED, FLAG_NO_SOLVER, FLAG_NO_INTEGRATE, FLAG_EPA_GUARD, FLAG_FP32_SOLVE = 1, 2, 4, 8, 16
+FLAG_FP32_GJK, FLAG_FP32_BROAD = 32, 64
+# the mixed-precision form: FP32 broadphase leaf tests, certified FP32 GJK separation,
+# FP32 contact solve (EPA, manifolds and body state stay FP64)
+FLAG_MIXED = FLAG_FP32_SOLVE | FLAG_FP32_GJK | FLAG_FP32_BROAD
 
 D = C.POINTER(C.c_double)
 U8 = C.POINTER(C.c_uint8)
@@ -71,7 +75,8 @@ class pkg_body_state(C.Structure):
 
 class pkg_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("moved", "pairs", "gjk_hits", "contacts", "rows",
-                                          "colours", "epa_overflow", "candidates")]
+                                          "colours", "epa_overflow", "candidates", "gjk_listed",
+                                          "gjk_fp64")]
 
 
 class pkg_joint_desc(C.Structure):
@@ -268,12 +273,13 @@ class GpuWorld:
         self._check(self.L.pkg_step(self.h, dt, nsteps), "step")
 
     STAGES = ("integrate_vel", "lbvh_build", "pair_update", "gjk", "epa", "manifold",
-              "solver", "integrate_pos")
+              "solver", "integrate_pos", "solve_iterations")
 
     def step_timed(self, dt, nsteps=1):
-        """Steps and returns the summed device milliseconds per stage (STAGES)."""
-        ms = np.zeros(8)
-        self._check(self.L.pkg_step_timed(self.h, dt, nsteps, _dp(ms), 8), "step_timed")
+        """Steps and returns the summed device milliseconds per stage (STAGES;
+        solve_iterations is the part of solver spent in the iteration kernels)."""
+        ms = np.zeros(len(self.STAGES))
+        self._check(self.L.pkg_step_timed(self.h, dt, nsteps, _dp(ms), len(self.STAGES)), "step_timed")
         return dict(zip(self.STAGES, ms.tolist()))
 
     def set_state(self, ids, pos=None, vel=None, angvel=None, quat=None):

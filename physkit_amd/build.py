@@ -18,8 +18,8 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libphyskit_amd.so")
 COPY_LIB = os.path.join(OUT_DIR, "libpk_copy_check.so")
 
-SOURCES = ["api.hip", "k_body.hip", "k_sort.hip", "k_broad.hip", "k_narrow.hip", "k_solver.hip", "k_joint.hip", "k_query.hip",
-           "scenes.cpp"]
+SOURCES = ["api.hip", "k_body.hip", "k_sort.hip", "k_broad.hip", "k_narrow.hip", "k_gjk32.hip", "k_solver.hip", "k_joint.hip",
+           "k_query.hip", "scenes.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",

@@ -127,6 +127,8 @@ struct pkg_ctx
     std::vector<uint32_t> shape_cost; // support candidates per shape: 12 for a tagged UV sphere, else its vertices
     dbuf<uint32_t> shape_off, shape_voff, shape_uv;
     dbuf<double> verts, shape_box, shape_rad, cull_rec;
+    dbuf<float4> verts32; // FP32 vertices for the certified-separation pass
+    dbuf<float> rad32;
 
     uint32_t n = 0;
     dbuf<double> pos, vel, ang, acc, torque, quat, mass, inv_mass, I_local, Iinv_local, Iw, Iw_inv, rest,
@@ -149,6 +151,7 @@ struct pkg_ctx
     dbuf<float4> inode_lo, inode_hi, span_box, span_tab;
     dbuf<uint4> nrec;
     dbuf<double> leaf_box;
+    dbuf<float4> leaf_f32; // with PKG_FLAG_FP32_BROAD
     dbuf<uint8_t> leaf_moved, covered;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
@@ -169,7 +172,7 @@ struct pkg_ctx
     dbuf<uint8_t> events;
 
     // narrowphase
-    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list;
+    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list, gjk_list2;
     // lanes per pair in GJK: 2 when the bodies' shapes average at most 14 support
     // candidates (boxes, certified UV spheres, small hulls), else 4
     int gjk_group = 4;
@@ -178,8 +181,6 @@ struct pkg_ctx
     dbuf<char> big_pool;
 
     // solver
-    dbuf<double> rows;
-    dbuf<uint8_t> row_valid;
     dbuf<uint32_t> colour, colour_start, order, order_alt;
     dbuf<uint32_t> par, iv, iv_alt, iflags, ipos, istarts; // contact islands
     dbuf<uint64_t> ik, ik_alt;
@@ -295,6 +296,15 @@ struct pkg_ctx
         s.uv = shape_uv.p;
         return s;
     }
+    shapes32_dev S32()
+    {
+        shapes32_dev s;
+        s.v = verts32.p;
+        s.off = shape_off.p;
+        s.rad = rad32.p;
+        s.uv = shape_uv.p;
+        return s;
+    }
     manifolds_dev MF(int k)
     {
         manifolds_dev m;
@@ -354,6 +364,7 @@ lbvh_dev pkg_ctx::tree()
     t.parent = parent.p;
     t.escape = escape.p;
     t.leaf_box = leaf_box.p;
+    t.leaf_f32 = (cfg.flags & PKG_FLAG_FP32_BROAD) ? leaf_f32.p : nullptr;
     t.leaf_body = leaf_body.p;
     t.leaf_moved = leaf_moved.p;
     t.n_chunks = refit_chunks(n);
@@ -630,7 +641,17 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     {
         gjk_list.reserve(Pn);
         cull_rec.reserve(20 * static_cast<size_t>(n));
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, cull_rec.p, counters.p, st);
+        gjk32_in f32{};
+        shapes32_dev s32 = S32();
+        if (cfg.flags & PKG_FLAG_FP32_GJK)
+        {
+            gjk_list2.reserve(Pn);
+            f32.s32 = &s32;
+            f32.src = src.p;
+            f32.prev_count = mcount[cur].p;
+            f32.list2 = gjk_list2.p;
+        }
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         record(ev, 9, st);
@@ -660,6 +681,9 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         record(ev, 5, st);
     }
     record(ev, 6, st);
+    // the solve iterations' interval (events 10-11) is empty unless a solve runs
+    record(ev, 10, st);
+    record(ev, 11, st);
 
     // ---- 4. contacts and joints (constraint.h:1052-1201)
     n_colours = 0;
@@ -669,24 +693,10 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
         uint32_t c = 0; // manifold colours (0: no contact rows)
         const double thr = 2.0 * gnorm * dt;
-        // Rows go straight into solve order (rs, built below from `ord`); the FP32 solve
-        // converts them from rows in pair order.
+        // Rows go straight into solve order (rs, built below from `ord`), as FP32 planes
+        // for the FP32 solve.
         auto write_rows = [&](const uint32_t *ord, uint32_t n_act, const rows_dev &rs, uint32_t scap)
-        {
-            if (!f32)
-            {
-                launch_contact_setup(keys[nxt].p, n_act, ord, scap, b, MF(nxt), rs, dt, thr, st);
-                return;
-            }
-            rows.reserve(static_cast<size_t>(4) * ncap * kRowF);
-            row_valid.reserve(static_cast<size_t>(4) * ncap);
-            rows_dev r{};
-            r.cap4 = static_cast<uint32_t>(std::min(rows.cap / kRowF, row_valid.cap));
-            r.d = rows.p;
-            r.valid = row_valid.p;
-            launch_contact_setup(keys[nxt].p, Pn, nullptr, 0, b, MF(nxt), r, dt, thr, st);
-            launch_rows_gather32(ord, n_act, r, rs, scap, st);
-        };
+        { launch_contact_setup(keys[nxt].p, n_act, ord, scap, b, MF(nxt), rs, dt, thr, f32, st); };
         if (Pn)
         {
             colour.reserve(ncap);
@@ -766,8 +776,10 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 rs.d = rows_s.p;
                 rs.valid = row_valid_s.p;
                 write_rows(is.ord, n_act, rs, scap);
+                record(ev, 10, st);
                 launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
                                      cfg.solver_iterations, f32, st);
+                record(ev, 11, st);
                 if (f32)
                     launch_impulse_writeback32(is.ord, n_act, MF(nxt), rs, scap, st);
                 else
@@ -808,6 +820,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             }
             const joints_dev jd = JD(jset);
             const uint32_t jc = n_joints ? static_cast<uint32_t>(jcs_h.size()) - 1 : 0;
+            record(ev, 10, st);
             for (uint32_t pass = 0; pass <= cfg.solver_iterations; ++pass)
             {
                 const bool warm = pass == 0;
@@ -820,6 +833,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                         launch_solve_colour(keys[nxt].p, ord, cs[k], cs[k + 1], b, rs, scap, warm, st);
                 }
             }
+            record(ev, 11, st);
             if (c)
             {
                 if (f32)
@@ -849,6 +863,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     last.colours = n_colours;
     last.epa_overflow = h_counters[C_EPA_OVF];
     last.candidates = candidates;
+    last.gjk_listed = Pn ? h_counters[C_GJK_LIST] : 0;
+    last.gjk_fp64 = Pn ? ((cfg.flags & PKG_FLAG_FP32_GJK) ? h_counters[C_GJK_LIST2] : h_counters[C_GJK_LIST]) : 0;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -961,11 +977,22 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
         if (pv.empty()) pv.assign(6, 0.0);
         upload(ctx->shape_voff, voff.data(), voff.size(), ctx->st);
         upload(ctx->shape_rad, rad.data(), rad.size(), ctx->st);
+        // radii for the FP32 pass, rounded up so they still bound every vertex
+        std::vector<float> rad32(rad.size());
+        for (size_t i = 0; i < rad.size(); ++i)
+        {
+            float f = static_cast<float>(rad[i]);
+            if (static_cast<double>(f) < rad[i]) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+            rad32[i] = f;
+        }
+        upload(ctx->rad32, rad32.data(), rad32.size(), ctx->st);
         upload(ctx->verts, pv.data(), pv.size(), ctx->st);
         ctx->shape_box.reserve(static_cast<size_t>(n_shapes) * 6);
         ctx->n_shapes = n_shapes;
         ctx->n_verts = nv;
         launch_shape_boxes(ctx->S(), ctx->st);
+        ctx->verts32.reserve(std::max<uint32_t>(nv, 1));
+        launch_verts32(ctx->S(), ctx->verts32.p, ctx->st);
         hip_check(hipStreamSynchronize(ctx->st), "shape upload");
     })
 }
@@ -1044,6 +1071,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->inode_hi.reserve(n);
         ctx->nrec.reserve(n);
         ctx->leaf_box.reserve(6 * static_cast<size_t>(n));
+        if (ctx->cfg.flags & PKG_FLAG_FP32_BROAD) ctx->leaf_f32.reserve(2 * static_cast<size_t>(n));
         launch_body_init(ctx->B(), ctx->S(), st);
         ctx->P[0] = ctx->P[1] = 0;
         ctx->cur = 0;
@@ -1432,17 +1460,19 @@ pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms,
     if (!ctx || nsteps < 0 || !stage_ms || n_stages < 1) return PKG_INVALID_ARG;
     if (!ctx->have_bodies) return PKG_INVALID_ARG;
     PKG_GUARD(ctx, {
-        hipEvent_t ev[10];
+        hipEvent_t ev[12];
         for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
-        // stage k spans events (from[k], to[k]); event 9 follows the GJK hit-count read
-        const int from[8] = {0, 1, 2, 3, 9, 5, 6, 7};
-        const int to[8] = {1, 2, 3, 4, 5, 6, 7, 8};
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        // stage k spans events (from[k], to[k]); event 9 follows the GJK hit-count read;
+        // stage 8 (events 10-11, inside stage 6) is the solve iterations alone
+        constexpr int kStages = 9;
+        const int from[kStages] = {0, 1, 2, 3, 9, 5, 6, 7, 10};
+        const int to[kStages] = {1, 2, 3, 4, 5, 6, 7, 8, 11};
+        double acc[kStages] = {};
         for (int i = 0; i < nsteps; ++i)
         {
             ctx->step_once(dt, ev);
             hip_check(hipEventSynchronize(ev[8]), "event sync");
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < kStages; ++k)
             {
                 float ms = 0;
                 hip_check(hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]), "elapsed");
@@ -1450,7 +1480,7 @@ pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms,
             }
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
-        for (int k = 0; k < n_stages && k < 8; ++k) stage_ms[k] = acc[k];
+        for (int k = 0; k < n_stages && k < kStages; ++k) stage_ms[k] = acc[k];
     })
 }
 

@@ -194,7 +194,17 @@ __global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, 
     uint32_t a = ids[i];
     t.leaf_body[i] = a;
     t.leaf_moved[i] = (!b.is_static[a] && (all_moved || b.moved[a])) ? 1 : 0;
-    for (int k = 0; k < 6; ++k) t.leaf_box[static_cast<size_t>(i) * 6 + k] = b.fat[static_cast<size_t>(k) * b.n + a];
+    double f[6];
+    for (int k = 0; k < 6; ++k)
+    {
+        f[k] = b.fat[static_cast<size_t>(k) * b.n + a];
+        t.leaf_box[static_cast<size_t>(i) * 6 + k] = f[k];
+    }
+    if (t.leaf_f32)
+    {
+        t.leaf_f32[2 * static_cast<size_t>(i)] = make_float4(f_down(f[0]), f_down(f[1]), f_down(f[2]), 0.0f);
+        t.leaf_f32[2 * static_cast<size_t>(i) + 1] = make_float4(f_up(f[3]), f_up(f[4]), f_up(f[5]), 0.0f);
+    }
     if (n == 1) t.parent[0] = kNone;
 }
 
@@ -583,13 +593,38 @@ __global__ void __launch_bounds__(kBlock)
         {
             uint32_t j = node - (n - 1);
             next = t.escape[node];
-            if (j != i && j >= wlo && j <= whi)
+            bool near = j != i && j >= wlo && j <= whi;
+            bool sure = false;
+            if (near && t.leaf_f32)
             {
-                const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
-                box3 lb;
-                lb.min = mk(p[0], p[1], p[2]);
-                lb.max = mk(p[3], p[4], p[5]);
-                if (box_intersects(lb, q) && (j > i || !t.leaf_moved[j]))
+                // FP32 bounds rounded outward: lo32 <= lo < succ(lo32), pred(hi32) < hi <= hi32.
+                // Apart in FP32: apart exactly. The successors of the lower bounds below the
+                // query's upper bounds (and the predecessors of the upper bounds above its
+                // lower bounds), compared in FP64: overlapping exactly. Only a leaf in
+                // between reads its FP64 box.
+                const float4 lo = t.leaf_f32[2 * static_cast<size_t>(j)], hi = t.leaf_f32[2 * static_cast<size_t>(j) + 1];
+                near = lo.x <= qhi[0] && hi.x >= qlo[0] && lo.y <= qhi[1] && hi.y >= qlo[1] && lo.z <= qhi[2] &&
+                       hi.z >= qlo[2];
+                const float inf = __builtin_inff();
+                sure = near && static_cast<double>(nextafterf(lo.x, inf)) <= q.max.x &&
+                       static_cast<double>(nextafterf(lo.y, inf)) <= q.max.y &&
+                       static_cast<double>(nextafterf(lo.z, inf)) <= q.max.z &&
+                       static_cast<double>(nextafterf(hi.x, -inf)) >= q.min.x &&
+                       static_cast<double>(nextafterf(hi.y, -inf)) >= q.min.y &&
+                       static_cast<double>(nextafterf(hi.z, -inf)) >= q.min.z;
+            }
+            if (near)
+            {
+                bool ov = sure;
+                if (!sure)
+                {
+                    const double *p = t.leaf_box + static_cast<size_t>(j) * 6;
+                    box3 lb;
+                    lb.min = mk(p[0], p[1], p[2]);
+                    lb.max = mk(p[3], p[4], p[5]);
+                    ov = box_intersects(lb, q);
+                }
+                if (ov && (j > i || !t.leaf_moved[j]))
                 {
                     if (nbuf == kEmitBuf)
                     {

@@ -247,10 +247,10 @@ __global__ void __launch_bounds__(kBlock)
 // simplex work per lane.
 template <int G, bool UV>
 __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spilling
-    k_gjk(const uint64_t *keys, const uint32_t *list, bodies_dev b, shapes_dev s, gjk_out_dev g,
+    k_gjk(const uint64_t *keys, const uint32_t *list, int list_slot, bodies_dev b, shapes_dev s, gjk_out_dev g,
           unsigned long long *counters)
 {
-    const uint32_t n_list = static_cast<uint32_t>(counters[C_GJK_LIST]);
+    const uint32_t n_list = static_cast<uint32_t>(counters[list_slot]);
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t i = t / G;
     int gl = static_cast<int>(t % G);
@@ -618,7 +618,8 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, double *cull_rec, unsigned long long *counters, hipStream_t st)
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,
+                hipStream_t st)
 {
     if (!n_pairs) return;
     // per-body records pay off with several pairs per body (hull clusters: 4; the
@@ -630,21 +631,28 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
     }
     else
         k_gjk_cull<false><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, g.status, list, counters);
+    int slot = C_GJK_LIST;
+    if (f32.s32)
+    {
+        launch_gjk32(keys, list, n_pairs, b, *f32.s32, uv, f32.src, f32.prev_count, g.status, f32.list2, counters, st);
+        list = f32.list2;
+        slot = C_GJK_LIST2;
+    }
     // sized for every pair; groups past the listed count (read on the device) exit at once
     unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
     if (group == 2)
     {
         if (uv)
-            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
+            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
         else
-            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
+            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
     }
     else
     {
         if (uv)
-            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
+            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
         else
-            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, list, b, s, g, counters);
+            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
     }
 }
 

@@ -192,12 +192,6 @@ __device__ __forceinline__ double &rf(const rows_dev &r, int f, uint32_t slot)
 { return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
 __device__ __forceinline__ v3 rf3(const rows_dev &r, int f, uint32_t slot)
 { return mk(rf(r, f, slot), rf(r, f + 1, slot), rf(r, f + 2, slot)); }
-__device__ __forceinline__ void wf3(const rows_dev &r, int f, uint32_t slot, v3 v)
-{
-    rf(r, f, slot) = v.x;
-    rf(r, f + 1, slot) = v.y;
-    rf(r, f + 2, slot) = v.z;
-}
 
 // Contact c of manifold p: world normal n and lever arms as build_contact_jacobian
 // forms them (:883-890) and whether the contact makes rows (a penetration that is not
@@ -254,9 +248,35 @@ __global__ void __launch_bounds__(kBlock)
 __device__ __forceinline__ uint32_t row_slot(const uint32_t *ord, uint32_t cap, uint32_t k, int c)
 { return ord ? static_cast<uint32_t>(c) * cap + k : 4 * k + static_cast<uint32_t>(c); }
 
+// row planes of element type T (double: the FP64 solve; float: the FP32 solve, whose
+// rows are computed in FP64 and rounded once when stored)
+template <class T> struct rows_t
+{
+    uint32_t cap4;
+    T *d;
+    uint8_t *valid;
+};
+template <class T> __host__ __device__ __forceinline__ rows_t<T> rows_as(const rows_dev &r)
+{
+    rows_t<T> o;
+    o.cap4 = r.cap4;
+    o.d = reinterpret_cast<T *>(r.d);
+    o.valid = r.valid;
+    return o;
+}
+template <class T> __device__ __forceinline__ void put(const rows_t<T> &r, int f, uint32_t slot, double v)
+{ r.d[static_cast<size_t>(f) * r.cap4 + slot] = static_cast<T>(v); }
+template <class T> __device__ __forceinline__ void put3(const rows_t<T> &r, int f, uint32_t slot, v3 v)
+{
+    put(r, f, slot, v.x);
+    put(r, f + 1, slot, v.y);
+    put(r, f + 2, slot, v.z);
+}
+
+template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, bodies_dev b, manifolds_dev m,
-            rows_dev r, double dt, double thr)
+            rows_t<T> r, double dt, double thr)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n)
@@ -313,26 +333,26 @@ __global__ void __launch_bounds__(kBlock)
                     i12 = 0.0;
                 }
                 const uint32_t slot = row_slot(ord, cap, k, c);
-                wf3(r, F_NJV, slot, nr.Jv);
-                wf3(r, F_NJWA, slot, nr.Jwa);
-                wf3(r, F_NJWB, slot, nr.Jwb);
-                rf(r, F_NMEFF, slot) = nr.Meff;
-                rf(r, F_NBIAS, slot) = nr.bias;
-                rf(r, F_NACC, slot) = d[10 * s];
-                wf3(r, F_T1JV, slot, r1.Jv);
-                wf3(r, F_T1JWA, slot, r1.Jwa);
-                wf3(r, F_T1JWB, slot, r1.Jwb);
-                rf(r, F_T1MEFF, slot) = r1.Meff;
-                rf(r, F_T1ACC, slot) = d[11 * s];
-                wf3(r, F_T2JV, slot, r2.Jv);
-                wf3(r, F_T2JWA, slot, r2.Jwa);
-                wf3(r, F_T2JWB, slot, r2.Jwb);
-                rf(r, F_T2MEFF, slot) = r2.Meff;
-                rf(r, F_T2ACC, slot) = d[12 * s];
-                rf(r, F_MU, slot) = mu;
-                rf(r, F_I11, slot) = i11;
-                rf(r, F_I12, slot) = i12;
-                rf(r, F_I22, slot) = i22;
+                put3(r, F_NJV, slot, nr.Jv);
+                put3(r, F_NJWA, slot, nr.Jwa);
+                put3(r, F_NJWB, slot, nr.Jwb);
+                put(r, F_NMEFF, slot, nr.Meff);
+                put(r, F_NBIAS, slot, nr.bias);
+                put(r, F_NACC, slot, d[10 * s]);
+                put3(r, F_T1JV, slot, r1.Jv);
+                put3(r, F_T1JWA, slot, r1.Jwa);
+                put3(r, F_T1JWB, slot, r1.Jwb);
+                put(r, F_T1MEFF, slot, r1.Meff);
+                put(r, F_T1ACC, slot, d[11 * s]);
+                put3(r, F_T2JV, slot, r2.Jv);
+                put3(r, F_T2JWA, slot, r2.Jwa);
+                put3(r, F_T2JWB, slot, r2.Jwb);
+                put(r, F_T2MEFF, slot, r2.Meff);
+                put(r, F_T2ACC, slot, d[12 * s]);
+                put(r, F_MU, slot, mu);
+                put(r, F_I11, slot, i11);
+                put(r, F_I12, slot, i12);
+                put(r, F_I22, slot, i22);
                 r.valid[slot] = 1;
             }
         }
@@ -512,9 +532,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---- FP32 solve with FP64 per-body velocity accumulation ----
-// Rows are built in FP64 (k_setup) and gathered once as FP32; the row arithmetic of
-// the solve (J.v, lambda, clamping, the 2x2 friction block) runs in FP32 and each
-// body's velocity increment is added to its FP64 velocity.
+// Rows are computed in FP64 by k_setup<float> and stored once as FP32 in solve order;
+// the row arithmetic of the solve (J.v, lambda, clamping, the 2x2 friction block) runs
+// in FP32 and each body's velocity increment is added to its FP64 velocity.
 struct f3
 {
     float x, y, z;
@@ -545,23 +565,6 @@ __device__ __forceinline__ float &rg(const rows32_dev &r, int f, uint32_t slot)
 { return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
 __device__ __forceinline__ f3 rg3(const rows32_dev &r, int f, uint32_t slot)
 { return f3m(rg(r, f, slot), rg(r, f + 1, slot), rg(r, f + 2, slot)); }
-
-__global__ void k_rows_gather32(const uint32_t *order, uint32_t n_act, rows_dev src, rows32_dev dst, uint32_t cap)
-{
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_act) return;
-    uint32_t p = order[t];
-    for (int c = 0; c < 4; ++c)
-    {
-        uint32_t s = 4 * p + static_cast<uint32_t>(c), d = static_cast<uint32_t>(c) * cap + t;
-        uint8_t v = src.valid[s];
-        dst.valid[d] = v;
-        if (!v) continue;
-        for (int f = 0; f < kRowF; ++f)
-            dst.d[static_cast<size_t>(f) * dst.cap4 + d] =
-                static_cast<float>(src.d[static_cast<size_t>(f) * src.cap4 + s]);
-    }
-}
 
 struct body32
 {
@@ -982,9 +985,13 @@ void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &
     if (n_pairs) k_row_flags<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, colour, counters);
 }
 void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
-                          manifolds_dev m, rows_dev r, double dt, double threshold, hipStream_t st)
+                          manifolds_dev m, rows_dev r, double dt, double threshold, bool f32, hipStream_t st)
 {
-    if (n) k_setup<<<grid(n), kBlock, 0, st>>>(keys, n, ord, cap, b, m, r, dt, threshold);
+    if (!n) return;
+    if (f32)
+        k_setup<float><<<grid(n), kBlock, 0, st>>>(keys, n, ord, cap, b, m, rows_as<float>(r), dt, threshold);
+    else
+        k_setup<double><<<grid(n), kBlock, 0, st>>>(keys, n, ord, cap, b, m, rows_as<double>(r), dt, threshold);
 }
 void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
                       hipStream_t st)
@@ -1091,11 +1098,6 @@ void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uin
                                                                is.cbits, b, r, cap, iters);
 }
 
-void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
-                          uint32_t cap, hipStream_t st)
-{
-    if (n_act) k_rows_gather32<<<grid(n_act), kBlock, 0, st>>>(order, n_act, src, as32(dst32), cap);
-}
 void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                            const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st)
 {

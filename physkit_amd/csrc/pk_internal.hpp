@@ -84,6 +84,7 @@ enum counter_slot
     C_NCOL = 15,
     C_NLIST = 16,
     C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
+    C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
     // addresses instead of serialising on one
@@ -138,6 +139,8 @@ struct lbvh_dev
     uint32_t *parent;    // 2n-1
     uint32_t *escape;    // 2n-1: next node after the subtree in left-first order
     double *leaf_box;    // 6n: exact FP64 leaf bounds, leaf-major
+    float4 *leaf_f32;    // 2n or null: leaf bounds rounded outward to FP32 (min, max; w unused),
+                         // tested before the exact bounds in the pair query
     uint32_t *leaf_body; // n: body id of leaf i
     uint8_t *leaf_moved; // n: leaf queries this step (dynamic and moved)
     // refit chunks of kRefitChunk leaves: per leaf, the union of its chunk's leaf boxes
@@ -203,12 +206,39 @@ struct gjk_out_dev
     uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
     uint32_t pair_cap;
 };
+// FP32 copies of the shapes for the certified-separation pass (k_gjk32.hip): vertex i
+// of shape s at v[off[s] + i] (xyz, w = 0), rounded to nearest; rad rounded up
+struct shapes32_dev
+{
+    const float4 *v;
+    const uint32_t *off;
+    const float *rad;
+    const uint32_t *uv;
+};
+void launch_verts32(const shapes_dev &s, float4 *out, hipStream_t st);
+// Pairs the cull listed (counters[C_GJK_LIST] entries of `list`, at most n_pairs) that
+// FP32 proves apart by more than GJK's margin are written separated (status 0); the
+// rest, and every pair with contacts after the previous step (prev_count[src[p]] > 0),
+// are listed in list2, count in counters[C_GJK_LIST2].
+void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, const bodies_dev &b,
+                  const shapes32_dev &s, bool uv, const uint32_t *src, const int32_t *prev_count, uint8_t *status,
+                  uint32_t *list2, unsigned long long *counters, hipStream_t st);
 // GJK over every pair with `group` (2 or 4) lanes per pair; pairs whose bounding
 // spheres are clearly apart are written separated first and the rest listed in `list`
 // (capacity n_pairs), so GJK wavefronts run only pairs that need it; cull_rec: 20
-// doubles per body of scratch for the per-body cull records
+// doubles per body of scratch for the per-body cull records. With s32 (the mixed-
+// precision form), the FP32 pass above runs between the cull and the FP64 GJK, which
+// then takes list2 (capacity n_pairs).
+struct gjk32_in
+{
+    const shapes32_dev *s32; // null: no FP32 pass
+    const uint32_t *src;
+    const int32_t *prev_count;
+    uint32_t *list2;
+};
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, double *cull_rec, unsigned long long *counters, hipStream_t st);
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,
+                hipStream_t st);
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 // overflowed hits rerun with an LDS store; its own overflows are listed in ovf2_list
@@ -251,9 +281,10 @@ constexpr uint32_t kUncoloured = 0xFFFFFFFEu; // colour of an item still to be c
 void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, manifolds_dev m, uint32_t *colour,
                       unsigned long long *counters, hipStream_t st);
 // rows of manifold ord[k], k < n, at slots c * cap + k of r (solve order); with ord
-// null, of manifold k at slots 4k + c (pair order)
+// null, of manifold k at slots 4k + c (pair order). f32: r.d holds FP32 planes (the
+// rows computed in FP64 and rounded once)
 void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
-                          manifolds_dev m, rows_dev r, double dt, double threshold, hipStream_t st);
+                          manifolds_dev m, rows_dev r, double dt, double threshold, bool f32, hipStream_t st);
 // With batched worlds, a manifold's colouring priority uses its index within its own
 // world's pair list (p - pair_base[world]), so each world colours as if stepped alone;
 // world_of and pair_base are null for a single world.
@@ -305,10 +336,8 @@ void launch_island_maxlen(const uint32_t *starts, const uint32_t *totals, uint32
 void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
                           uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
                           uint32_t iters, bool f32, hipStream_t st);
-// FP32 solve (rows gathered as FP32 planes into a rows_dev whose d holds floats;
-// body velocities accumulate in FP64)
-void launch_rows_gather32(const uint32_t *order, uint32_t n_act, const rows_dev &src, const rows_dev &dst32,
-                          uint32_t cap, hipStream_t st);
+// FP32 solve (rows_dev whose d holds FP32 planes, written by launch_contact_setup with
+// f32; body velocities accumulate in FP64)
 void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                            const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback32(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r32, uint32_t cap,

@@ -90,6 +90,9 @@ def lib():
         L.pko_run_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.pko_set_manifold.restype = C.c_int
         L.pko_set_manifold.argtypes = [C.c_void_p, C.c_uint64, C.c_int, D]
+        L.pko_set_fat.argtypes = [C.c_void_p, C.c_int, U32, D]
+        L.pko_set_pairs.restype = C.c_int
+        L.pko_set_pairs.argtypes = [C.c_void_p, C.c_size_t, U64, I32, D]
         L.pko_manifold_add.restype = C.c_int
         L.pko_manifold_add.argtypes = [C.c_int, D, D, D]
         _lib = L
@@ -262,6 +265,20 @@ class OracleWorld:
     def run_stage(self, name, dt=0.0):
         """One stage of world::step (src/world.cpp:20-56)."""
         assert self.L.pko_run_stage(self.h, self.STAGES[name], dt) == 1
+
+    def set_state(self, bodies, pairs, counts, data):
+        """Restores a saved state: every body's pos, vel, angvel, quat and fat (the dict of
+        bodies()), the sorted pair keys and their manifolds (counts, (n, 4, 13) data)."""
+        n = len(bodies["pos"])
+        ids = np.arange(n, dtype=np.uint32)
+        self.set_bodies(ids, pos=bodies["pos"], vel=bodies["vel"], angvel=bodies["angvel"], quat=bodies["quat"])
+        fat = f64(bodies["fat"])
+        self.L.pko_set_fat(self.h, n, ids.ctypes.data_as(U32), dp(fat))
+        keys = np.ascontiguousarray(pairs, dtype=np.uint64)
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        d = f64(np.asarray(data, float).reshape(-1))
+        assert self.L.pko_set_pairs(self.h, len(keys), keys.ctypes.data_as(U64), cnt.ctypes.data_as(I32),
+                                    dp(d) if len(d) else None) == 1
 
     def set_manifold(self, key, contacts):
         """Replaces the cached manifold of active pair `key` by contacts ((n, 13), the
