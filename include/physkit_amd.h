@@ -294,6 +294,24 @@ pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a,
                              const uint32_t *vend_b, const double *verts, uint32_t n_verts,
                              double *out, int32_t *hit);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc):
+ * pkg_gjk_epa_batch over arrays that are already in device memory (the ctx's device),
+ * with the same arguments, results and errors, run on the ctx's stream; returns when the
+ * results are written. It calls no hipMalloc or hipFree: its only scratch is a static
+ * device array of the library (about 0.8 MB per GPU, held until the process ends, set up
+ * by pkg_create), which limits n to PKG_NOALLOC_MAX_PAIRS (PKG_INVALID_ARG above it).
+ * Every call on one GPU shares that array, so calls that overlap on one GPU (from
+ * several ctx or threads) race. Without UV-sphere tags every support scans all
+ * vertices: identical results, slower for UV spheres. A pair with an invalid kind or
+ * vertex range gets hit = -1 and the call returns PKG_INVALID_ARG. */
+#define PKG_NOALLOC_MAX_PAIRS 65536u
+pkg_status pkg_gjk_epa_batch_device(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a,
+                                    const double *prm_a, const uint32_t *vbeg_a, const uint32_t *vend_a,
+                                    const int32_t *kind_b, const double *prm_b, const uint32_t *vbeg_b,
+                                    const uint32_t *vend_b, const double *verts, uint32_t n_verts,
+                                    double *out, int32_t *hit);
+
 /* ---- synthetic scenes (input generation for tests and benchmarks) ---- */
 typedef struct pkg_scene pkg_scene;
 typedef struct pkg_scene_info

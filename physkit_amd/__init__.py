@@ -40,7 +40,7 @@ EXPORTS = [
     "pkg_gjk_epa_batch", "pkg_scene_create", "pkg_scene_info_get", "pkg_scene_copy",
     "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds", "pkg_add_joints",
     "pkg_remove_joints", "pkg_get_joints", "pkg_raycast", "pkg_raycast_closest", "pkg_raycast_closest_device",
-    "pkg_query_aabb",
+    "pkg_query_aabb", "pkg_gjk_epa_batch_device",
 ]
 
 # joint types, numbered as constraint_type (include/physkit/collision/constraint.h:14-23)

@@ -6,6 +6,7 @@
 // pair count, GJK hit count, EPA overflow count, solver rows, colour rounds.
 #include "physkit_amd.h"
 #include "pk_access.hpp"
+#include "pk_gjk.hpp"
 #include "uv_sphere.hpp"
 
 #include <algorithm>
@@ -13,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -115,9 +117,32 @@ template <class T> std::vector<T> download(const dbuf<T> &d, size_t count, hipSt
 }
 } // namespace
 
+// Scratch of pkg_gjk_epa_batch_device, the entry point that allocates no device memory:
+// its counters, its list of solves that overflow the private EPA store and the large
+// stores that rerun them. One copy per GPU (each GPU holds it until the process ends),
+// shared by every call on that GPU: calls that overlap on one GPU race. Host code never
+// uses the name `scratch` for its memory, only the per-device address that
+// hipGetSymbolAddress returns (noalloc_ptr below).
+namespace
+{
+constexpr uint32_t kNoAllocStores = 2;
+struct noalloc_layout
+{
+    uint32_t words[64]; // counters (launch_gjk_epa_batch_dev); one 256-byte line
+    uint32_t ovf_list[PKG_NOALLOC_MAX_PAIRS];
+    pk::epa_store_big stores[kNoAllocStores];
+};
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+void *noalloc_ptr[kMaxDevices] = {};
+} // namespace
+static __device__ __align__(256) noalloc_layout scratch;
+
 struct pkg_ctx
 {
     hipStream_t st = nullptr;
+    int device = 0;
+    noalloc_layout *noalloc = nullptr; // this device's copy of `scratch`
     pkg_config cfg{};
     double gnorm = 9.81;
     std::string err;
@@ -901,8 +926,21 @@ pkg_status pkg_create(pkg_ctx **out, const pkg_config *cfg)
         c->cfg = *cfg;
         const double *g = cfg->gravity;
         c->gnorm = std::sqrt((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]); // world.h:383
+        hip_check(hipGetDevice(&c->device), "get device");
+        if (c->device < 0 || c->device >= kMaxDevices) throw pk_error(PKG_DEVICE_ERROR, "device index out of range");
+        {
+            // the no-allocation entry point's scratch: its address on this device, once
+            std::lock_guard<std::mutex> lk(noalloc_mu);
+            if (!noalloc_ptr[c->device])
+            {
+                void *p = nullptr;
+                hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch symbol address");
+                noalloc_ptr[c->device] = p;
+            }
+            c->noalloc = static_cast<noalloc_layout *>(noalloc_ptr[c->device]);
+        }
         hip_check(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking), "stream create");
-        hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), C_NCOUNTERS * sizeof(unsigned long long)),
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), (C_NCOUNTERS + 2) * sizeof(unsigned long long)),
                   "pinned alloc");
         c->counters.reserve(C_NCOUNTERS);
         if (const char *e = std::getenv("PKG_PAIR_REGION_MIN_LEAVES"))
@@ -1683,6 +1721,52 @@ pkg_status pkg_gjk_epa_batch(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a, co
         pool.release();
         for (uint32_t i = 0; i < n; ++i)
             if (h[i] == 3) throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
+    })
+}
+
+pkg_status pkg_gjk_epa_batch_device(pkg_ctx *ctx, uint32_t n, const int32_t *kind_a, const double *prm_a,
+                                    const uint32_t *vbeg_a, const uint32_t *vend_a, const int32_t *kind_b,
+                                    const double *prm_b, const uint32_t *vbeg_b, const uint32_t *vend_b,
+                                    const double *verts, uint32_t n_verts, double *out, int32_t *hit)
+{
+    if (!ctx) return PKG_INVALID_ARG;
+    if (n > PKG_NOALLOC_MAX_PAIRS)
+    {
+        ctx->err = "pkg_gjk_epa_batch_device: n = " + std::to_string(n) + " exceeds PKG_NOALLOC_MAX_PAIRS (" +
+                   std::to_string(PKG_NOALLOC_MAX_PAIRS) + "), the capacity of its static scratch";
+        return PKG_INVALID_ARG;
+    }
+    if (n && (!kind_a || !prm_a || !vbeg_a || !vend_a || !kind_b || !prm_b || !vbeg_b || !vend_b || !out || !hit ||
+              (n_verts && !verts)))
+    {
+        ctx->err = "pkg_gjk_epa_batch_device: null array";
+        return PKG_INVALID_ARG;
+    }
+    PKG_GUARD(ctx, {
+        noalloc_layout *sc = ctx->noalloc;
+        shape_pair_dev sp;
+        sp.kind_a = kind_a;
+        sp.kind_b = kind_b;
+        sp.prm_a = prm_a;
+        sp.prm_b = prm_b;
+        sp.vbeg_a = vbeg_a;
+        sp.vend_a = vend_a;
+        sp.vbeg_b = vbeg_b;
+        sp.vend_b = vend_b;
+        sp.verts = verts;
+        sp.uv_a = nullptr; // no UV-sphere tags: full support scans, the same results
+        sp.uv_b = nullptr;
+        launch_gjk_epa_batch_dev(sp, n, n_verts, out, hit, sc->words, sc->ovf_list, sc->stores, kNoAllocStores,
+                                 ctx->st);
+        hip_check(hipGetLastError(), "batch launch");
+        // the counters into the ctx's pinned words (allocated by pkg_create)
+        uint32_t *h = reinterpret_cast<uint32_t *>(ctx->h_counters + C_NCOUNTERS);
+        hip_check(hipMemcpyAsync(h, sc->words, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->st), "counter copy");
+        hip_check(hipStreamSynchronize(ctx->st), "stream sync");
+        if (h[0])
+            throw pk_error(PKG_INVALID_ARG, std::to_string(h[0]) + " pairs with a kind outside 0-2 or a vertex range "
+                                                                   "outside [0, n_verts) (their hit is -1)");
+        if (h[2]) throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
     })
 }
 

@@ -573,7 +573,7 @@ __device__ gshape pair_shape(const int32_t *kind, const double *prm, const uint3
         g.q = mkq(q[3], q[4], q[5], q[6]);
         g.v = verts + 3 * static_cast<size_t>(vb[i]);
         g.nv = ve[i] - vb[i];
-        g.uv = uv[i];
+        g.uv = uv ? uv[i] : 0u; // no tags: every support scans all vertices (same results)
     }
     return g;
 }
@@ -613,9 +613,61 @@ __global__ void k_batch_big(shape_pair_dev sp, uint32_t n, double *out, int32_t 
     for (uint32_t i = 0; i < n; ++i)
         if (hit[i] == 3) batch_one(sp, i, out, hit, pool[0]);
 }
+// ---- standalone batch over device arrays with caller-free scratch ----
+// Scratch words: [0] pairs with invalid input, [1] pairs listed for the large store,
+// [2] pairs that overflowed the large store as well.
+__global__ void k_batch_dev_clear(uint32_t *words)
+{
+    if (threadIdx.x < 4) words[threadIdx.x] = 0u;
+}
+__global__ void __launch_bounds__(kEpaBlock)
+    k_batch_dev(shape_pair_dev sp, uint32_t n, uint32_t n_verts, double *out, int32_t *hit, uint32_t *words,
+                uint32_t *ovf_list)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // the host variant checks these on the host (pkg_gjk_epa_batch)
+    for (int side = 0; side < 2; ++side)
+    {
+        const int k = side ? sp.kind_b[i] : sp.kind_a[i];
+        const uint32_t vb = side ? sp.vbeg_b[i] : sp.vbeg_a[i], ve = side ? sp.vend_b[i] : sp.vend_a[i];
+        if (k < 0 || k > 2 || (k == 2 && (ve <= vb || ve > n_verts)))
+        {
+            hit[i] = -1;
+            atomicAdd(&words[0], 1u);
+            return;
+        }
+    }
+    epa_store_fast store;
+    batch_one(sp, i, out, hit, store);
+    if (hit[i] == 3) ovf_list[atomicAdd(&words[1], 1u)] = i;
+}
+// one thread per large store, each taking every n_stores-th listed pair
+__global__ void k_batch_dev_big(shape_pair_dev sp, double *out, int32_t *hit, uint32_t *words,
+                                const uint32_t *ovf_list, epa_store_big *stores, uint32_t n_stores)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_stores) return;
+    const uint32_t m = words[1];
+    for (uint32_t k = t; k < m; k += n_stores)
+    {
+        const uint32_t i = ovf_list[k];
+        batch_one(sp, i, out, hit, stores[t]);
+        if (hit[i] == 3) atomicAdd(&words[2], 1u);
+    }
+}
 } // namespace
 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
+
+void launch_gjk_epa_batch_dev(const shape_pair_dev &sp, uint32_t n, uint32_t n_verts, double *out, int32_t *hit,
+                              uint32_t *words, uint32_t *ovf_list, void *stores, uint32_t n_stores, hipStream_t st)
+{
+    k_batch_dev_clear<<<1, 64, 0, st>>>(words);
+    if (!n) return;
+    k_batch_dev<<<grid(n, kEpaBlock), kEpaBlock, 0, st>>>(sp, n, n_verts, out, hit, words, ovf_list);
+    k_batch_dev_big<<<1, 64, 0, st>>>(sp, out, hit, words, ovf_list, static_cast<epa_store_big *>(stores), n_stores);
+}
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
                 gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,

@@ -266,6 +266,12 @@ struct shape_pair_dev
 };
 void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,
                           void *big_pool, hipStream_t st);
+// The same over device arrays with scratch the caller does not allocate: words (4 x u32:
+// invalid pairs, pairs listed for the large store, pairs that overflowed it too; cleared
+// by the first launch), ovf_list (n entries) and n_stores large EPA stores. Invalid pairs
+// get hit = -1. uv tags may be null.
+void launch_gjk_epa_batch_dev(const shape_pair_dev &sp, uint32_t n, uint32_t n_verts, double *out, int32_t *hit,
+                              uint32_t *words, uint32_t *ovf_list, void *stores, uint32_t n_stores, hipStream_t st);
 
 // ---------------- k_solver.hip ----------------
 struct rows_dev
