@@ -254,17 +254,20 @@ def precision_drift(s, steps, flags_lo):
     return out
 
 
-def timed_run(pk, s, flags, warmup, steps):
-    """Loads s into a context with flags, runs warmup untimed steps, then `steps` timed
-    steps (each pkg_step_timed call returns after its device work has finished, so the
-    timed region is bracketed by device synchronisation). Returns wall seconds, summed
-    stage milliseconds, pair and hit totals, the last counters, pairs and positions."""
+def timed_run(pk, s, flags, warmup, steps, autotune=True):
+    """Loads s into a context with flags, runs warmup untimed steps (then, with
+    autotune, pkg_autotune's 10 untimed steps picking the launch shapes), then `steps`
+    timed steps (each pkg_step_timed call returns after its device work has finished, so
+    the timed region is bracketed by device synchronisation). Returns wall seconds,
+    summed stage milliseconds, pair and hit totals, the last counters, pairs, positions
+    and the launch shapes used."""
     g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
     g.load_scene(s)
     if len(s["first_body"]) > 2:
         g.set_worlds(s["first_body"])
     for _ in range(warmup):
         g.step(s["dt"])
+    tuning = g.autotune(s["dt"], 1) if autotune else g.tuning()
     stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
     pairs_total = hits_total = 0
     t0 = time.perf_counter()
@@ -277,7 +280,7 @@ def timed_run(pk, s, flags, warmup, steps):
         hits_total += c["gjk_hits"]
     wall = time.perf_counter() - t0
     out = {"wall": wall, "stage": stage, "pairs_total": pairs_total, "hits_total": hits_total,
-           "counters": g.counters(), "pairs": g.pairs(), "pos": g.bodies()["pos"]}
+           "counters": g.counters(), "pairs": g.pairs(), "pos": g.bodies()["pos"], "tuning": tuning}
     g.close()
     return out
 
@@ -505,6 +508,8 @@ def main():
                          "mixed: FP32 leaf tests, certified FP32 GJK separation, FP32 solve; fp32-solve: "
                          "only the solve in FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of pkg_autotune's (10 untimed steps)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison runs (batched FP64, per-world drop-in, precision drift)")
     ap.add_argument("--n-rays", type=int, default=RAYS, help="rays workload: rays per launch")
@@ -531,7 +536,7 @@ def main():
     n_dyn = len(s["shape"]) - int(s["n_static"])
     print(f"[bench] workload={args.workload} precision={prec} bodies={len(s['shape'])} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
-    r = timed_run(pk, s, s["flags"] | extra, args.warmup, args.steps)
+    r = timed_run(pk, s, s["flags"] | extra, args.warmup, args.steps, autotune=not args.no_autotune)
     wall, stage, c, pairs = r["wall"], r["stage"], r["counters"], r["pairs"]
     print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
@@ -562,6 +567,7 @@ def main():
                    "colliding_pairs_per_s": round(r["hits_total"] / wall, 1),
                    "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stage.items()},
                    "stage_roofline": table,
+                   "launch_shapes": r["tuning"],
                    "step_algorithmic_bytes": step_b,
                    "step_hbm_frac": round(step_b / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 5),
                    "worlds": len(s["first_body"]) - 1,
@@ -573,7 +579,7 @@ def main():
         if batched:
             # the same worlds batched in FP64, and the plain FP64 drop-in: worlds one at a time
             if prec != "fp64":
-                r64 = timed_run(pk, s, s["flags"], args.warmup, args.steps)
+                r64 = timed_run(pk, s, s["flags"], args.warmup, args.steps, autotune=not args.no_autotune)
                 v64 = n_dyn * args.steps / r64["wall"]
                 out["config"]["batched_fp64"] = {"value": round(v64, 1), "unit": "body-steps/s",
                                                  "ms_per_step": round(r64["wall"] * 1e3 / args.steps, 3),

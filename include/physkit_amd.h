@@ -129,7 +129,8 @@ const char *pkg_last_error(const pkg_ctx *ctx);
 
 /* Convex meshes (mesh::make / box / sphere / pyramid, src/mesh.cpp:16-137):
  * n_shapes shapes, shape s owning vertices [offsets[s], offsets[s+1]) of verts (xyz).
- * The local AABB is aabb::from_points over the vertices (mesh.h:191). */
+ * The local AABB is aabb::from_points over the vertices (mesh.h:191). With bodies
+ * loaded, an upload that drops a shape a body uses is refused (PKG_INVALID_ARG). */
 pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *offsets,
                              const double *verts);
 
@@ -156,7 +157,12 @@ pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids,
                               const double *quat);
 
 /* world::step_impl(dt) (src/world.cpp:20-56), nsteps times. Synchronous: returns
- * when the device results of the last step are final. */
+ * when the device results of the last step are final.
+ * A step that fails (any status but PKG_OK: PKG_OOM, PKG_OVERFLOW_*, PKG_DEVICE_ERROR)
+ * leaves the bodies part-way through it (velocities may carry its gravity, leaf bounds
+ * may be updated and their moved flags used up). The ctx then refuses pkg_step and
+ * pkg_step_timed (PKG_DEVICE_ERROR) until pkg_upload_bodies loads bodies again;
+ * pkg_download_bodies still returns the part-way state. */
 pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps);
 
 pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out);
@@ -280,6 +286,36 @@ pkg_status pkg_query_aabb(pkg_ctx *ctx, uint32_t n_boxes, const double *boxes, c
  * 7 integrate_pos, 8 the solver's iteration kernels alone (part of stage 6).
  * stage_ms[i] is the sum over the nsteps steps (up to 9 entries). */
 pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms, int n_stages);
+
+/* ---- launch shapes (not in the reference) ----
+ * Work-list, tile and pairs-per-wavefront shapes of the narrowphase kernels. None of
+ * them changes a result (the support search returns the lowest-index maximum however
+ * its lanes are grouped), only the time a step takes. */
+typedef struct pkg_tuning
+{
+    int32_t gjk_lanes;     /* lanes per pair in the GJK kernel: 1, 2, 4 or 8 (64 / lanes pairs per
+                              wavefront); default 2 for shapes averaging at most 14 support
+                              candidates, else 4 */
+    int32_t epa_block;     /* threads per workgroup of the one-solve-per-thread EPA kernel: 64, 128
+                              or 256 (default 64) */
+    uint32_t epa_lds_max;  /* GJK-hit batches of at most this many run one EPA solve per workgroup
+                              with the polytope in LDS; larger ones one solve per thread (512) */
+    float gjk_ms[4];       /* pkg_autotune: GJK stage ms per step with 1, 2, 4, 8 lanes per pair */
+    float epa_block_ms[3]; /* EPA stage ms per step with workgroups of 64, 128, 256 threads */
+    float epa_lds_ms[3];   /* EPA stage ms per step with epa_lds_max 0, 512, 4096 */
+} pkg_tuning;
+
+/* Picks the launch shapes above by timing them on this world: it takes
+ * 10 x steps_per_candidate real steps (the world advances as under pkg_step, with the
+ * same results), the candidates of one shape taking turns step by step, and keeps the
+ * fastest of each (GJK lanes on the GJK stage, then the EPA block size with every batch
+ * per thread, then the LDS threshold). pkg_upload_bodies returns to the defaults. out
+ * (optional) receives the choice and the times measured. */
+pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps_per_candidate, pkg_tuning *out);
+pkg_status pkg_get_tuning(pkg_ctx *ctx, pkg_tuning *out);
+/* Sets the shapes directly (the *_ms fields are ignored); PKG_INVALID_ARG for values
+ * outside the lists above. */
+pkg_status pkg_set_tuning(pkg_ctx *ctx, const pkg_tuning *t);
 
 /* Standalone narrowphase: gjk_epa(a, b) (src/collision.cpp:512-518) over n shape
  * pairs. Shape i of side a: kind_a[i] (0 aabb, 1 obb, 2 mesh instance), params

@@ -440,10 +440,21 @@ public:
     // world_base::step(dt) -> world::step_impl (src/world.cpp:20-56)
     void step(double dt)
     {
+        if (M_failed)
+            throw std::runtime_error("physkit_amd: a previous step failed part-way and the world's state is lost");
         if (M_dirty) upload();
         flush_joints();
-        check(pkg_step(M_ctx, dt, 1), "pkg_step");
+        // from here the device state is the step's, whether it completes or not
         M_state_valid = false;
+        try
+        {
+            check(pkg_step(M_ctx, dt, 1), "pkg_step");
+        }
+        catch (...)
+        {
+            M_failed = true;
+            throw;
+        }
         dispatch_events();
     }
 
@@ -469,6 +480,7 @@ private:
     std::vector<uint32_t> M_free;
     std::vector<object_state> M_state;
     bool M_dirty = true, M_state_valid = false, M_uploaded = false;
+    bool M_failed = false; // a step threw: the device state is part-way through it
     size_t M_n_dev = 0; // bodies on the device (slot count at the last upload)
     // one arena per joint type: generation and liveness per slot, LIFO free slots
     std::array<std::vector<uint32_t>, 6> M_jgen, M_jfree;

@@ -40,7 +40,7 @@ EXPORTS = [
     "pkg_gjk_epa_batch", "pkg_scene_create", "pkg_scene_info_get", "pkg_scene_copy",
     "pkg_scene_destroy", "pkg_set_worlds", "pkg_scene_copy_worlds", "pkg_add_joints",
     "pkg_remove_joints", "pkg_get_joints", "pkg_raycast", "pkg_raycast_closest", "pkg_raycast_closest_device",
-    "pkg_query_aabb", "pkg_gjk_epa_batch_device",
+    "pkg_query_aabb", "pkg_gjk_epa_batch_device", "pkg_autotune", "pkg_get_tuning", "pkg_set_tuning",
 ]
 
 # joint types, numbered as constraint_type (include/physkit/collision/constraint.h:14-23)
@@ -77,6 +77,11 @@ class pkg_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("moved", "pairs", "gjk_hits", "contacts", "rows",
                                           "colours", "epa_overflow", "candidates", "gjk_listed",
                                           "gjk_fp64")]
+
+
+class pkg_tuning(C.Structure):
+    _fields_ = [("gjk_lanes", C.c_int32), ("epa_block", C.c_int32), ("epa_lds_max", C.c_uint32),
+                ("gjk_ms", C.c_float * 4), ("epa_block_ms", C.c_float * 3), ("epa_lds_ms", C.c_float * 3)]
 
 
 class pkg_joint_desc(C.Structure):
@@ -125,6 +130,9 @@ def lib():
     L.pkg_set_body_state.argtypes = [P, C.c_uint32, U32, D, D, D, D]
     L.pkg_step.argtypes = [P, C.c_double, C.c_int]
     L.pkg_step_timed.argtypes = [P, C.c_double, C.c_int, D, C.c_int]
+    L.pkg_autotune.argtypes = [P, C.c_double, C.c_int, C.POINTER(pkg_tuning)]
+    L.pkg_get_tuning.argtypes = [P, C.POINTER(pkg_tuning)]
+    L.pkg_set_tuning.argtypes = [P, C.POINTER(pkg_tuning)]
     L.pkg_download_bodies.argtypes = [P, C.POINTER(pkg_body_state)]
     L.pkg_get_pairs.argtypes = [P, U64, C.c_size_t, C.POINTER(C.c_size_t)]
     L.pkg_get_manifolds.argtypes = [P, I32, D, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -281,6 +289,25 @@ class GpuWorld:
         ms = np.zeros(len(self.STAGES))
         self._check(self.L.pkg_step_timed(self.h, dt, nsteps, _dp(ms), len(self.STAGES)), "step_timed")
         return dict(zip(self.STAGES, ms.tolist()))
+
+    def autotune(self, dt, steps_per_candidate=1):
+        """pkg_autotune: times the launch-shape candidates over real steps (the world
+        advances 10 x steps_per_candidate steps) and keeps the fastest; returns the
+        choice and the measured stage milliseconds."""
+        t = pkg_tuning()
+        self._check(self.L.pkg_autotune(self.h, dt, steps_per_candidate, C.byref(t)), "autotune")
+        return {"gjk_lanes": t.gjk_lanes, "epa_block": t.epa_block, "epa_lds_max": t.epa_lds_max,
+                "gjk_ms": [round(x, 4) for x in t.gjk_ms], "epa_block_ms": [round(x, 4) for x in t.epa_block_ms],
+                "epa_lds_ms": [round(x, 4) for x in t.epa_lds_ms]}
+
+    def tuning(self):
+        t = pkg_tuning()
+        self._check(self.L.pkg_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return {"gjk_lanes": t.gjk_lanes, "epa_block": t.epa_block, "epa_lds_max": t.epa_lds_max}
+
+    def set_tuning(self, gjk_lanes, epa_block, epa_lds_max):
+        t = pkg_tuning(gjk_lanes=gjk_lanes, epa_block=epa_block, epa_lds_max=epa_lds_max)
+        self._check(self.L.pkg_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def set_state(self, ids, pos=None, vel=None, angvel=None, quat=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)

@@ -190,17 +190,23 @@ struct pkg_ctx
     dbuf<uint64_t> keys[2];
     dbuf<uint32_t> src;
     dbuf<int32_t> mcount[2];
+    dbuf<uint32_t> mslot[2];
     dbuf<double> mdata[2];
-    uint32_t mcap[2] = {0, 0};
+    uint32_t mcap[2] = {0, 0};   // pairs the count and slot arrays hold
+    uint32_t mdcap[2] = {0, 0};  // manifold data slots
+    uint32_t mused[2] = {0, 0};  // slots in use after the step that wrote the set
+    uint32_t last_hits = 0;      // GJK hits of the last step (sizes the hit lists)
     uint32_t P[2] = {0, 0};
     int cur = 0;
     dbuf<uint8_t> events;
 
     // narrowphase
-    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list, gjk_list2;
-    // lanes per pair in GJK: 2 when the bodies' shapes average at most 14 support
-    // candidates (boxes, certified UV spheres, small hulls), else 4
-    int gjk_group = 4;
+    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list, gjk_list2, hit_of;
+    // launch shapes: GJK lanes per pair defaults to 2 when the bodies' shapes average at
+    // most 14 support candidates (boxes, certified UV spheres, small hulls), else 4, until
+    // pkg_autotune or pkg_set_tuning picks them
+    tuning tune;
+    bool tuned = false;
     dbuf<double> simplex, info;
     dbuf<uint8_t> status;
     dbuf<char> big_pool;
@@ -246,6 +252,22 @@ struct pkg_ctx
 
     pkg_counters last{};
     bool have_bodies = false;
+    // a step failed part-way: body state holds part of that step, so stepping is refused
+    // until pkg_upload_bodies loads bodies again
+    bool poisoned = false;
+    std::vector<uint32_t> h_shape; // shape of each uploaded body (checks shape re-uploads)
+    void update_gjk_group()
+    {
+        double cost = 0.0;
+        uint32_t live = 0;
+        for (size_t i = 0; i < h_shape.size(); ++i)
+            if (h_static[i] != 2)
+            {
+                cost += shape_cost[h_shape[i]];
+                ++live;
+            }
+        if (!tuned) tune.gjk_group = (live && cost <= 14.0 * live) ? 2 : 4;
+    }
 
     lbvh_dev tree();
     lbvh_dev build_tree(const bodies_dev &b, bool all_moved);
@@ -335,6 +357,8 @@ struct pkg_ctx
         manifolds_dev m;
         m.cap = mcap[k];
         m.count = mcount[k].p;
+        m.slot = mslot[k].p;
+        m.dcap = mdcap[k];
         m.data = mdata[k].p;
         return m;
     }
@@ -639,29 +663,37 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     record(ev, 3, st);
 
     // ---- 3. narrowphase (collision_phases.h:244-320)
+    // per-pair arrays hold every pair; per-hit arrays (hit list, simplex, contact data)
+    // and manifold data only the GJK hits and the pairs in contact
     if (mcap[nxt] < Pn)
     {
         uint32_t c = std::max<uint32_t>(Pn + Pn / 4, 1024);
         mcount[nxt].release();
-        mdata[nxt].release();
+        mslot[nxt].release();
         mcount[nxt].reserve(c);
-        mdata[nxt].reserve(static_cast<size_t>(c) * 4 * kMF);
+        mslot[nxt].reserve(c);
         mcap[nxt] = c;
     }
     uint32_t ncap = std::max<uint32_t>(mcap[nxt], 1);
-    hit_list.reserve(ncap);
-    ovf_list.reserve(ncap);
-    simplex.reserve(static_cast<size_t>(ncap) * kSimplexWords);
-    info.reserve(static_cast<size_t>(ncap) * 10);
     status.reserve(ncap);
     events.reserve(ncap);
+    hit_of.reserve(ncap);
     gjk_out_dev g;
-    g.hit_list = hit_list.p;
-    g.simplex = simplex.p;
-    g.cap = static_cast<uint32_t>(std::min(hit_list.cap, simplex.cap / kSimplexWords));
-    g.info = info.p;
     g.status = status.p;
-    g.pair_cap = static_cast<uint32_t>(std::min(info.cap / 10, status.cap));
+    g.hit_of = hit_of.p;
+    auto size_hits = [&](uint32_t want)
+    {
+        want = std::max<uint32_t>(want, 1024);
+        hit_list.reserve(want);
+        ovf_list.reserve(want);
+        simplex.reserve(static_cast<size_t>(want) * kSimplexWords);
+        info.reserve(static_cast<size_t>(want) * 10);
+        g.hit_list = hit_list.p;
+        g.simplex = simplex.p;
+        g.info = info.p;
+        g.cap = static_cast<uint32_t>(std::min({hit_list.cap, ovf_list.cap, simplex.cap / kSimplexWords, info.cap / 10}));
+    };
+    size_hits(std::min<uint32_t>(Pn, last_hits + last_hits / 2));
     if (Pn)
     {
         gjk_list.reserve(Pn);
@@ -676,26 +708,57 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             f32.prev_count = mcount[cur].p;
             f32.list2 = gjk_list2.p;
         }
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
+        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, tune.gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
+        if (hits > g.cap)
+        {
+            // more hits than the hit lists hold: grow them and run the (idempotent) GJK
+            // stage again
+            size_hits(hits + hits / 4);
+            for (int c : {C_GJK_LIST, C_GJK_LIST2, C_HITS}) reset_counter(c);
+            launch_gjk(keys[nxt].p, Pn, b, s, has_uv, tune.gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p,
+                       st);
+            record(ev, 4, st);
+            hits = static_cast<uint32_t>(read_counter(C_HITS));
+            if (hits > g.cap) throw pk_error(PKG_DEVICE_ERROR, "GJK hit count changed between identical runs");
+        }
+        last_hits = hits;
         record(ev, 9, st);
-        launch_epa(keys[nxt].p, hits, b, s, has_uv, g, ovf_list.p, counters.p, st);
+        launch_epa(keys[nxt].p, hits, b, s, has_uv, tune, g, ovf_list.p, counters.p, st);
         uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
         if (novf) // rerun with larger stores; capacity never changes a result
         {
-            ovf2_list.reserve(novf);
-            launch_epa_lds(keys[nxt].p, ovf_list.p, novf, b, s, g, ovf2_list.p, counters.p, st);
-            const uint32_t n2 = static_cast<uint32_t>(read_counter(C_EPA_OVF2));
+            // a small batch already ran with the LDS store: its overflows go straight to
+            // the global-memory store
+            const uint32_t *big_in = ovf_list.p;
+            uint32_t n2 = novf;
+            if (!epa_first_tier_is_lds(hits, tune))
+            {
+                ovf2_list.reserve(novf);
+                launch_epa_lds(keys[nxt].p, ovf_list.p, novf, b, s, g, ovf2_list.p, counters.p, st);
+                n2 = static_cast<uint32_t>(read_counter(C_EPA_OVF2));
+                big_in = ovf2_list.p;
+            }
             if (n2)
             {
                 big_pool.reserve(static_cast<size_t>(n2) * epa_big_bytes());
-                launch_epa_big(keys[nxt].p, ovf2_list.p, n2, b, s, g, big_pool.p, counters.p, st);
+                launch_epa_big(keys[nxt].p, big_in, n2, b, s, g, big_pool.p, counters.p, st);
                 if (read_counter(C_EPA_BIG_FAIL))
                     throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
             }
         }
         record(ev, 5, st);
+        // a manifold is non-empty only if its pair had contacts or has a hit now: the
+        // data store needs at most hits + the previous step's slots
+        const uint32_t dneed = hits + mused[cur];
+        if (mdcap[nxt] < dneed)
+        {
+            const uint32_t c = std::max<uint32_t>(dneed + dneed / 4, 1024);
+            mdata[nxt].release();
+            mdata[nxt].reserve(static_cast<size_t>(c) * 4 * kMF);
+            mdcap[nxt] = c;
+        }
         launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p,
                                (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, counters.p, st);
     }
@@ -716,7 +779,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     if (solve && (Pn || n_joints))
     {
         const bool f32 = (cfg.flags & PKG_FLAG_FP32_SOLVE) != 0;
-        uint32_t c = 0; // manifold colours (0: no contact rows)
+        uint32_t c = 0;  // manifold colours (0: no contact rows)
+        uint32_t nl = 0; // manifolds with rows, listed in clist
         const double thr = 2.0 * gnorm * dt;
         // Rows go straight into solve order (rs, built below from `ord`), as FP32 planes
         // for the FP32 solve.
@@ -734,7 +798,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             read_counters(C_ROWS, C_NLIST);
             if (h_counters[C_ROWS] > 0)
             {
-                const uint32_t nl = static_cast<uint32_t>(h_counters[C_NLIST]);
+                nl = static_cast<uint32_t>(h_counters[C_NLIST]);
                 const uint32_t *wo = nullptr, *pb = nullptr;
                 if (n_worlds > 1)
                 {
@@ -771,22 +835,23 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         if (c && !n_joints && c <= kIslandMaxColours)
         {
             // one workgroup per chunk of whole contact islands when every island is small enough
+            // over the listed row manifolds only
             par.reserve(std::max<uint32_t>(n, 1));
-            ik.reserve(Pn);
-            ik_alt.reserve(Pn);
-            iv.reserve(Pn);
-            iv_alt.reserve(Pn);
-            iflags.reserve(Pn);
-            ipos.reserve(Pn);
-            istarts.reserve(Pn);
-            bsums.reserve(scan_block_sums(Pn));
+            ik.reserve(nl);
+            ik_alt.reserve(nl);
+            iv.reserve(nl);
+            iv_alt.reserve(nl);
+            iflags.reserve(nl);
+            ipos.reserve(nl);
+            istarts.reserve(nl);
+            bsums.reserve(scan_block_sums(nl));
             total.reserve(2);
             reset_counter(C_ISL_MAX);
-            island_sort is = launch_island_sort(keys[nxt].p, colour.p, Pn, c, b, par.p, ik.p, iv.p, ik_alt.p,
-                                                iv_alt.p, counters.p + C_ISL_MAX, sc, st);
-            launch_island_segments(is, Pn, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
+            island_sort is = launch_island_sort(keys[nxt].p, colour.p, clist.p, nl, c, b, par.p, ik.p, iv.p,
+                                                ik_alt.p, iv_alt.p, counters.p + C_ISL_MAX, sc, st);
+            launch_island_segments(is, nl, iflags.p, ipos.p, bsums.p, total.p, istarts.p, total.p + 1, st);
             // island count, row manifold count and the longest island in one wait
-            launch_island_maxlen(istarts.p, total.p, Pn, counters.p + C_ISL_MAX, st);
+            launch_island_maxlen(istarts.p, total.p, nl, counters.p + C_ISL_MAX, st);
             uint32_t h2[2] = {0, 0};
             hip_check(hipMemcpyAsync(h2, total.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "island count");
             read_counters(C_ISL_MAX, C_ISL_MAX);
@@ -888,6 +953,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     last.colours = n_colours;
     last.epa_overflow = h_counters[C_EPA_OVF];
     last.candidates = candidates;
+    mused[cur] = Pn ? static_cast<uint32_t>(h_counters[C_MSLOTS]) : 0u;
     last.gjk_listed = Pn ? h_counters[C_GJK_LIST] : 0;
     last.gjk_fp64 = Pn ? ((cfg.flags & PKG_FLAG_FP32_GJK) ? h_counters[C_GJK_LIST2] : h_counters[C_GJK_LIST]) : 0;
 }
@@ -981,6 +1047,14 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
             ctx->err = "every shape needs at least one vertex (mesh::make asserts non-empty, mesh.cpp:19)";
             return PKG_INVALID_ARG;
         }
+    if (ctx->have_bodies)
+        for (size_t i = 0; i < ctx->h_shape.size(); ++i)
+            if (ctx->h_shape[i] >= n_shapes)
+            {
+                ctx->err = "body " + std::to_string(i) + " uses shape " + std::to_string(ctx->h_shape[i]) +
+                           ", which this upload drops; upload the bodies again after the shapes";
+                return PKG_INVALID_ARG;
+            }
     PKG_GUARD(ctx, {
         uint32_t nv = offsets[n_shapes];
         std::vector<uint32_t> uv(std::max<uint32_t>(n_shapes, 1), 0u);
@@ -1032,6 +1106,7 @@ pkg_status pkg_upload_shapes(pkg_ctx *ctx, uint32_t n_shapes, const uint32_t *of
         ctx->verts32.reserve(std::max<uint32_t>(nv, 1));
         launch_verts32(ctx->S(), ctx->verts32.p, ctx->st);
         hip_check(hipStreamSynchronize(ctx->st), "shape upload");
+        if (ctx->have_bodies) ctx->update_gjk_group(); // the loaded bodies' shapes changed
     })
 }
 
@@ -1055,15 +1130,9 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         }
     PKG_GUARD(ctx, {
         ctx->h_static.assign(d->is_static, d->is_static + n);
-        double cost = 0.0;
-        uint32_t live = 0;
-        for (uint32_t i = 0; i < n; ++i)
-            if (d->is_static[i] != 2)
-            {
-                cost += ctx->shape_cost[d->shape[i]];
-                ++live;
-            }
-        ctx->gjk_group = (live && cost <= 14.0 * live) ? 2 : 4;
+        ctx->h_shape.assign(d->shape, d->shape + n);
+        ctx->tuned = false; // a new body set: back to the default shapes until tuned again
+        ctx->update_gjk_group();
         ctx->jcoloured = false;
         hipStream_t st = ctx->st;
         ctx->n = n;
@@ -1112,9 +1181,11 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         if (ctx->cfg.flags & PKG_FLAG_FP32_BROAD) ctx->leaf_f32.reserve(2 * static_cast<size_t>(n));
         launch_body_init(ctx->B(), ctx->S(), st);
         ctx->P[0] = ctx->P[1] = 0;
+        ctx->mused[0] = ctx->mused[1] = 0;
         ctx->cur = 0;
         ctx->n_worlds = 1;
         ctx->have_bodies = true;
+        ctx->poisoned = false;
         ctx->tree_ok = false;
         hip_check(hipStreamSynchronize(st), "body upload");
     })
@@ -1150,6 +1221,7 @@ pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first
         ctx->n_worlds = n_worlds;
         ctx->tree_ok = false;
         ctx->P[0] = ctx->P[1] = 0; // pairs restart under the new partition
+        ctx->mused[0] = ctx->mused[1] = 0;
         ctx->cur = 0;
     })
 }
@@ -1483,28 +1555,52 @@ pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids,
     })
 }
 
+namespace
+{
+// stage k of pkg_step_timed spans events (kFrom[k], kTo[k]) recorded by step_once; event 9
+// follows the GJK hit-count read; stage 8 (events 10-11, inside stage 6) is the solve
+// iterations alone
+constexpr int kStages = 9;
+constexpr int kFrom[kStages] = {0, 1, 2, 3, 9, 5, 6, 7, 10};
+constexpr int kTo[kStages] = {1, 2, 3, 4, 5, 6, 7, 8, 11};
+
+// a failed step leaves body state part-way through it (velocities may carry its gravity,
+// leaf bounds may be updated and their moved flags used up): the ctx refuses further
+// steps until bodies are uploaded again
+pkg_status refuse_if_poisoned(pkg_ctx *ctx)
+{
+    if (!ctx->poisoned) return PKG_OK;
+    ctx->err = "a previous step failed part-way; upload the bodies again (pkg_upload_bodies) before stepping";
+    return PKG_DEVICE_ERROR;
+}
+} // namespace
+
 pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps)
 {
     if (!ctx || nsteps < 0 || !(dt >= 0.0)) return PKG_INVALID_ARG;
     if (!ctx->have_bodies) return PKG_INVALID_ARG;
-    PKG_GUARD(ctx, {
-        for (int i = 0; i < nsteps; ++i) ctx->step_once(dt, nullptr);
-        hip_check(hipGetLastError(), "step");
-    })
+    if (pkg_status p = refuse_if_poisoned(ctx)) return p;
+    const pkg_status r = [&]() -> pkg_status
+    {
+        PKG_GUARD(ctx, {
+            for (int i = 0; i < nsteps; ++i) ctx->step_once(dt, nullptr);
+            hip_check(hipGetLastError(), "step");
+        })
+    }();
+    if (r != PKG_OK) ctx->poisoned = true;
+    return r;
 }
 
 pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms, int n_stages)
 {
     if (!ctx || nsteps < 0 || !stage_ms || n_stages < 1) return PKG_INVALID_ARG;
     if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    if (pkg_status p = refuse_if_poisoned(ctx)) return p;
+    const pkg_status r = [&]() -> pkg_status
+    {
     PKG_GUARD(ctx, {
         hipEvent_t ev[12];
         for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
-        // stage k spans events (from[k], to[k]); event 9 follows the GJK hit-count read;
-        // stage 8 (events 10-11, inside stage 6) is the solve iterations alone
-        constexpr int kStages = 9;
-        const int from[kStages] = {0, 1, 2, 3, 9, 5, 6, 7, 10};
-        const int to[kStages] = {1, 2, 3, 4, 5, 6, 7, 8, 11};
         double acc[kStages] = {};
         for (int i = 0; i < nsteps; ++i)
         {
@@ -1513,13 +1609,112 @@ pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms,
             for (int k = 0; k < kStages; ++k)
             {
                 float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]), "elapsed");
+                hip_check(hipEventElapsedTime(&ms, ev[kFrom[k]], ev[kTo[k]]), "elapsed");
                 acc[k] += ms;
             }
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
         for (int k = 0; k < n_stages && k < kStages; ++k) stage_ms[k] = acc[k];
     })
+    }();
+    if (r != PKG_OK) ctx->poisoned = true;
+    return r;
+}
+
+namespace
+{
+void tuning_out(const tuning &t, pkg_tuning *o)
+{
+    o->gjk_lanes = t.gjk_group;
+    o->epa_block = t.epa_block;
+    o->epa_lds_max = t.epa_lds_max;
+}
+} // namespace
+
+pkg_status pkg_get_tuning(pkg_ctx *ctx, pkg_tuning *out)
+{
+    if (!ctx || !out) return PKG_INVALID_ARG;
+    *out = pkg_tuning{};
+    tuning_out(ctx->tune, out);
+    return PKG_OK;
+}
+
+pkg_status pkg_set_tuning(pkg_ctx *ctx, const pkg_tuning *t)
+{
+    if (!ctx || !t) return PKG_INVALID_ARG;
+    const int g = t->gjk_lanes, e = t->epa_block;
+    if (!(g == 1 || g == 2 || g == 4 || g == 8) || !(e == 64 || e == 128 || e == 256))
+    {
+        ctx->err = "pkg_set_tuning: gjk_lanes must be 1, 2, 4 or 8 and epa_block 64, 128 or 256";
+        return PKG_INVALID_ARG;
+    }
+    ctx->tune.gjk_group = g;
+    ctx->tune.epa_block = e;
+    ctx->tune.epa_lds_max = t->epa_lds_max;
+    ctx->tuned = true;
+    return PKG_OK;
+}
+
+pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps_per_candidate, pkg_tuning *out)
+{
+    if (!ctx || !(dt >= 0.0) || steps_per_candidate < 1) return PKG_INVALID_ARG;
+    if (!ctx->have_bodies) return PKG_INVALID_ARG;
+    if (pkg_status p = refuse_if_poisoned(ctx)) return p;
+    pkg_tuning res{};
+    const pkg_status r = [&]() -> pkg_status
+    {
+        PKG_GUARD(ctx, {
+            hipEvent_t ev[12];
+            for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
+            // one real step with the stage's time read from the events
+            auto timed = [&](int stage)
+            {
+                ctx->step_once(dt, ev);
+                hip_check(hipEventSynchronize(ev[8]), "event sync");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, ev[kFrom[stage]], ev[kTo[stage]]), "elapsed");
+                return static_cast<double>(ms);
+            };
+            // candidates take turns, step by step, so the scene's drift weighs on them alike
+            auto pick = [&](auto set, int n, int stage, float *ms_out)
+            {
+                double acc[4] = {0, 0, 0, 0};
+                for (int k = 0; k < steps_per_candidate; ++k)
+                    for (int c = 0; c < n; ++c)
+                    {
+                        set(c);
+                        acc[c] += timed(stage);
+                    }
+                int best = 0;
+                for (int c = 0; c < n; ++c)
+                {
+                    ms_out[c] = static_cast<float>(acc[c] / steps_per_candidate);
+                    if (acc[c] < acc[best]) best = c;
+                }
+                set(best);
+            };
+            const int lanes[4] = {1, 2, 4, 8};
+            const int blocks[3] = {64, 128, 256};
+            const uint32_t lds_max[3] = {0u, 512u, 4096u};
+            tuning &tu = ctx->tune;
+            pick([&](int c) { tu.gjk_group = lanes[c]; }, 4, 3, res.gjk_ms);
+            const uint32_t keep_lds = tu.epa_lds_max;
+            tu.epa_lds_max = 0; // every batch through k_epa while its block size is chosen
+            pick([&](int c) { tu.epa_block = blocks[c]; }, 3, 4, res.epa_block_ms);
+            tu.epa_lds_max = keep_lds;
+            pick([&](int c) { tu.epa_lds_max = lds_max[c]; }, 3, 4, res.epa_lds_ms);
+            ctx->tuned = true;
+            for (auto &e : ev) (void)hipEventDestroy(e);
+        })
+    }();
+    if (r != PKG_OK)
+    {
+        ctx->poisoned = true;
+        return r;
+    }
+    tuning_out(ctx->tune, &res);
+    if (out) *out = res;
+    return PKG_OK;
 }
 
 pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out)
@@ -1572,7 +1767,14 @@ pkg_status pkg_get_manifolds(pkg_ctx *ctx, int32_t *counts, double *data, size_t
         if (m && (counts || data))
         {
             std::vector<int32_t> cnt = download(ctx->mcount[c], P, ctx->st);
-            std::vector<double> raw = download(ctx->mdata[c], static_cast<size_t>(ctx->mcap[c]) * 4 * kMF, ctx->st);
+            std::vector<uint32_t> sl;
+            std::vector<double> raw;
+            const size_t dcap = ctx->mdcap[c];
+            if (data && dcap)
+            {
+                sl = download(ctx->mslot[c], P, ctx->st);
+                raw = download(ctx->mdata[c], dcap * 4 * kMF, ctx->st);
+            }
             for (size_t p = 0; p < m; ++p)
             {
                 if (counts) counts[p] = cnt[p];
@@ -1580,7 +1782,7 @@ pkg_status pkg_get_manifolds(pkg_ctx *ctx, int32_t *counts, double *data, size_t
                     for (int k = 0; k < 4; ++k)
                         for (int f = 0; f < kMF; ++f)
                             data[(p * 4 + k) * kMF + f] =
-                                k < cnt[p] ? raw[static_cast<size_t>(k * kMF + f) * ctx->mcap[c] + p] : 0.0;
+                                k < cnt[p] ? raw[static_cast<size_t>(k * kMF + f) * dcap + sl[p]] : 0.0;
             }
         }
     })

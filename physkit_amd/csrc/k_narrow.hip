@@ -10,16 +10,14 @@
 #include "pk_access.hpp"
 #include "pk_gjk.hpp"
 
+#include <type_traits>
+
 namespace pk
 {
 namespace
 {
 constexpr int kBlock = 256;
 constexpr int kEpaBlock = 64;
-// up to this many hits, each solve runs in its own workgroup with the LDS store: on a
-// near-idle device the longest solve's chain of LDS round trips ends sooner than one
-// through private memory
-constexpr uint32_t kEpaSmall = 512;
 unsigned grid(size_t n, int b = kBlock) { return static_cast<unsigned>((n + b - 1) / b); }
 
 // UV: the shapes may include tagged UV spheres (a kernel built without it carries no
@@ -271,8 +269,14 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
     unsigned long long slot = block_append(&counters[C_HITS], lead);
     if (lead)
     {
-        g.hit_list[slot] = p;
-        save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
+        g.hit_of[p] = static_cast<uint32_t>(slot);
+        // past the hit list's capacity the hit is only counted: the host sees the count,
+        // grows the list and runs the GJK stage again
+        if (slot < g.cap)
+        {
+            g.hit_list[slot] = p;
+            save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
+        }
     }
 }
 
@@ -283,8 +287,8 @@ __device__ __forceinline__ void write_info(double *info, uint32_t cap, uint32_t 
 
 // one thread per hit, 192-face store in private memory; a solve that needs more is
 // listed for the 4096-face kernel below
-template <bool UV>
-__global__ void __launch_bounds__(kEpaBlock)
+template <bool UV, int BS>
+__global__ void __launch_bounds__(BS)
     k_epa(const uint64_t *keys, uint32_t n_hits, bodies_dev b, shapes_dev s, gjk_out_dev g,
           uint32_t *ovf_list, unsigned long long *counters)
 {
@@ -301,7 +305,7 @@ __global__ void __launch_bounds__(kEpaBlock)
         epa_store_fast store;
         double out[10];
         int r = epa_solve<UV>(A, B, x, store, out);
-        if (r == 1) write_info(g.info, g.pair_cap, p, out);
+        if (r == 1) write_info(g.info, g.cap, h, out);
         g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
         ovf = r == 3;
     }
@@ -334,7 +338,7 @@ __global__ void __launch_bounds__(64)
         ovf2_list[atomicAdd(&counters[out_slot], 1ull)] = h;
         return;
     }
-    if (r == 1) write_info(g.info, g.pair_cap, p, out);
+    if (r == 1) write_info(g.info, g.cap, h, out);
     g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : 2);
 }
 
@@ -353,7 +357,7 @@ __global__ void k_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32
     load_simplex(g.simplex, g.cap, h, x);
     double out[10];
     int r = epa_solve<UV>(A, B, x, pool[t], out);
-    if (r == 1) write_info(g.info, g.pair_cap, p, out);
+    if (r == 1) write_info(g.info, g.cap, h, out);
     g.status[p] = static_cast<uint8_t>(r == 1 ? 1 : (r == 0 ? 2 : 3));
     if (r == 3) atomicAdd(&counters[C_EPA_BIG_FAIL], 1ull);
 }
@@ -365,10 +369,11 @@ struct contact
     double depth, ln, lt1, lt2;
 };
 
-__device__ __forceinline__ contact ld_contact(const manifolds_dev &m, uint32_t p, int c)
+// contact c of the manifold in data slot sl
+__device__ __forceinline__ contact ld_contact(const manifolds_dev &m, uint32_t sl, int c)
 {
-    const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-    size_t s = m.cap;
+    const double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + sl;
+    size_t s = m.dcap;
     contact k;
     k.n = mk(d[0], d[s], d[2 * s]);
     k.la = mk(d[3 * s], d[4 * s], d[5 * s]);
@@ -379,10 +384,10 @@ __device__ __forceinline__ contact ld_contact(const manifolds_dev &m, uint32_t p
     k.lt2 = d[12 * s];
     return k;
 }
-__device__ __forceinline__ void st_contact(const manifolds_dev &m, uint32_t p, int c, const contact &k)
+__device__ __forceinline__ void st_contact(const manifolds_dev &m, uint32_t sl, int c, const contact &k)
 {
-    double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-    size_t s = m.cap;
+    double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + sl;
+    size_t s = m.dcap;
     d[0] = k.n.x;
     d[s] = k.n.y;
     d[2 * s] = k.n.z;
@@ -464,11 +469,14 @@ __global__ void __launch_bounds__(kBlock)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t count = 0;
+    mani nm;
+    nm.n = 0;
     // a pair with no old contacts and no new one stays empty: no body state is read
     const bool live = p < n_pairs && (g.status[p] == 1 || (src[p] != kNone && prev.count[src[p]] > 0));
     if (p < n_pairs && !live)
     {
         next.count[p] = 0;
+        next.slot[p] = kNone;
         events[p] = 0;
     }
     if (live)
@@ -484,8 +492,8 @@ __global__ void __launch_bounds__(kBlock)
         contact nc{};
         if (has_new)
         {
-            const double *inf = g.info + p;
-            size_t s = g.pair_cap;
+            const double *inf = g.info + g.hit_of[p];
+            size_t s = g.cap;
             nc.n = mk(inf[0], inf[s], inf[2 * s]);
             v3 wa = mk(inf[3 * s], inf[4 * s], inf[5 * s]);
             v3 wb = mk(inf[6 * s], inf[7 * s], inf[8 * s]);
@@ -504,11 +512,10 @@ __global__ void __launch_bounds__(kBlock)
         }
         uint32_t sp = src[p];
         int oldn = sp != kNone ? prev.count[sp] : 0;
-        mani nm;
-        nm.n = 0;
+        const uint32_t osl = oldn > 0 ? prev.slot[sp] : kNone;
         for (int i = 0; i < oldn; ++i)
         {
-            contact old = ld_contact(prev, sp, i);
+            contact old = ld_contact(prev, osl, i);
             if (has_new)
             {
                 if (sqn(nc.la - old.la) < distance2_eps || sqn(nc.lb - old.lb) < distance2_eps)
@@ -534,11 +541,18 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         if (has_new) add_contact(nm, nc);
-        for (int i = 0; i < nm.n; ++i) st_contact(next, p, i, nm.c[i]);
         next.count[p] = nm.n;
         bool was = oldn > 0, is = nm.n > 0;
         events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
         count = static_cast<uint32_t>(nm.n);
+    }
+    // a data slot for each non-empty manifold (the host sized the store for every pair
+    // that had contacts or a GJK hit, so the slots fit)
+    const unsigned long long sl = block_append(&counters[C_MSLOTS], count > 0);
+    if (live)
+    {
+        next.slot[p] = count > 0 ? static_cast<uint32_t>(sl) : kNone;
+        for (int i = 0; i < nm.n; ++i) st_contact(next, static_cast<uint32_t>(sl), i, nm.c[i]);
     }
     block_add(&counters[C_CONTACTS], count);
 }
@@ -659,6 +673,7 @@ __global__ void k_batch_dev_big(shape_pair_dev sp, double *out, int32_t *hit, ui
 } // namespace
 
 size_t epa_big_bytes() { return sizeof(epa_store_big); }
+bool epa_first_tier_is_lds(uint32_t n_hits, const tuning &tu) { return n_hits <= tu.epa_lds_max; }
 
 void launch_gjk_epa_batch_dev(const shape_pair_dev &sp, uint32_t n, uint32_t n_verts, double *out, int32_t *hit,
                               uint32_t *words, uint32_t *ovf_list, void *stores, uint32_t n_stores, hipStream_t st)
@@ -692,36 +707,51 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
     }
     // sized for every pair; groups past the listed count (read on the device) exit at once
     unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
-    if (group == 2)
+    auto go = [&](auto lanes)
     {
+        constexpr int G = decltype(lanes)::value;
         if (uv)
-            k_gjk<2, true><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
+            k_gjk<G, true><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
         else
-            k_gjk<2, false><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
-    }
-    else
+            k_gjk<G, false><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
+    };
+    switch (group)
     {
-        if (uv)
-            k_gjk<4, true><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
-        else
-            k_gjk<4, false><<<gr, kBlock, 0, st>>>(keys, list, slot, b, s, g, counters);
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    default: go(std::integral_constant<int, 4>{}); break;
     }
 }
 
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
-                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
+                const tuning &tu, gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_hits) return;
-    if (n_hits <= kEpaSmall && uv)
-        k_epa_lds<epa_store_lds, true><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters,
-                                                             C_EPA_OVF);
-    else if (n_hits <= kEpaSmall)
-        k_epa_lds<epa_store_lds, false><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters,
-                                                              C_EPA_OVF);
-    else if (uv)
-        k_epa<true><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
-    else
-        k_epa<false><<<grid(n_hits, kEpaBlock), kEpaBlock, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+    if (epa_first_tier_is_lds(n_hits, tu))
+    {
+        if (uv)
+            k_epa_lds<epa_store_lds, true><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list, counters,
+                                                                 C_EPA_OVF);
+        else
+            k_epa_lds<epa_store_lds, false><<<n_hits, 64, 0, st>>>(keys, nullptr, n_hits, b, s, g, ovf_list,
+                                                                  counters, C_EPA_OVF);
+        return;
+    }
+    auto go = [&](auto block)
+    {
+        constexpr int BS = decltype(block)::value;
+        if (uv)
+            k_epa<true, BS><<<grid(n_hits, BS), BS, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+        else
+            k_epa<false, BS><<<grid(n_hits, BS), BS, 0, st>>>(keys, n_hits, b, s, g, ovf_list, counters);
+    };
+    switch (tu.epa_block)
+    {
+    case 128: go(std::integral_constant<int, 128>{}); break;
+    case 256: go(std::integral_constant<int, 256>{}); break;
+    default: go(std::integral_constant<int, 64>{}); break;
+    }
 }
 
 void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, const bodies_dev &b,

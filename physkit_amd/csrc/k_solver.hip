@@ -10,6 +10,7 @@
 #include "pk_access.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace pk
 {
@@ -186,7 +187,8 @@ enum
     F_NJV = 0, F_NJWA = 3, F_NJWB = 6, F_NMEFF = 9, F_NBIAS = 10, F_NACC = 11,
     F_T1JV = 12, F_T1JWA = 15, F_T1JWB = 18, F_T1MEFF = 21, F_T1ACC = 22,
     F_T2JV = 23, F_T2JWA = 26, F_T2JWB = 29, F_T2MEFF = 32, F_T2ACC = 33,
-    F_MU = 34, F_I11 = 35, F_I12 = 36, F_I22 = 37
+    F_MU = 34, F_I11 = 35, F_I12 = 36, F_I22 = 37,
+    F_RA = 38, F_RB = 41 // lever arms: the island solve rebuilds the Jacobians from them
 };
 __device__ __forceinline__ double &rf(const rows_dev &r, int f, uint32_t slot)
 { return r.d[static_cast<size_t>(f) * r.cap4 + slot]; }
@@ -196,11 +198,11 @@ __device__ __forceinline__ v3 rf3(const rows_dev &r, int f, uint32_t slot)
 // Contact c of manifold p: world normal n and lever arms as build_contact_jacobian
 // forms them (:883-890) and whether the contact makes rows (a penetration that is not
 // <= 0, so a NaN one does too, as in the reference's `if (penetration <= 0) continue`).
-__device__ __forceinline__ bool contact_frame(const manifolds_dev &m, uint32_t p, int c, v3 pa, v3 pb, qt qa, qt qb,
-                                              v3 &n, v3 &ra, v3 &rb, double &pen)
+__device__ __forceinline__ bool contact_frame(const manifolds_dev &m, uint32_t sl, int c, v3 pa, v3 pb, qt qa,
+                                              qt qb, v3 &n, v3 &ra, v3 &rb, double &pen)
 {
-    const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-    const size_t s = m.cap;
+    const double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + sl;
+    const size_t s = m.dcap;
     const v3 nw = normalized(mk(d[0], d[s], d[2 * s]));
     const v3 la = mk(d[3 * s], d[4 * s], d[5 * s]);
     const v3 lb = mk(d[6 * s], d[7 * s], d[8 * s]);
@@ -230,11 +232,12 @@ __global__ void __launch_bounds__(kBlock)
             const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
             const v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
             const qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+            const uint32_t sl = m.slot[p];
             for (int c = 0; c < cnt; ++c)
             {
                 v3 n, ra, rb;
                 double pen;
-                if (contact_frame(m, p, c, pa, pb, qa, qb, n, ra, rb, pen)) ++nrows;
+                if (contact_frame(m, sl, c, pa, pb, qa, qb, n, ra, rb, pen)) ++nrows;
             }
         }
         colour[p] = nrows ? kUncoloured : kNone;
@@ -295,14 +298,15 @@ __global__ void __launch_bounds__(kBlock)
             qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
             double e = smax(b.restitution[ia], b.restitution[ib]);
             double mu = sqrt(b.friction[ia] * b.friction[ib]);
+            const uint32_t sl = m.slot[p];
             for (int c = 0; c < cnt; ++c)
             {
-                const double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-                const size_t s = m.cap;
+                const double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + sl;
+                const size_t s = m.dcap;
                 // build_contact_jacobian, :874-953
                 v3 n, ra, rb;
                 double pen;
-                if (!contact_frame(m, p, c, pa, pb, qa, qb, n, ra, rb, pen)) continue;
+                if (!contact_frame(m, sl, c, pa, pb, qa, qb, n, ra, rb, pen)) continue;
                 rowv nr = make_row(A, B, n, ra, rb);
                 v3 vca = A.vel + cross(A.ang, ra);
                 v3 vcb = B.vel + cross(B.ang, rb);
@@ -353,6 +357,8 @@ __global__ void __launch_bounds__(kBlock)
                 put(r, F_I11, slot, i11);
                 put(r, F_I12, slot, i12);
                 put(r, F_I22, slot, i22);
+                put3(r, F_RA, slot, ra);
+                put3(r, F_RB, slot, rb);
                 r.valid[slot] = 1;
             }
         }
@@ -675,8 +681,8 @@ __global__ void k_writeback32(const uint32_t *order, uint32_t n_act, manifolds_d
     {
         uint32_t slot = static_cast<uint32_t>(c) * cap + t;
         if (!r.valid[slot]) continue;
-        double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-        size_t s = m.cap;
+        double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + m.slot[p];
+        size_t s = m.dcap;
         d[10 * s] = static_cast<double>(rg(r, F_NACC, slot));
         d[11 * s] = static_cast<double>(rg(r, F_T1ACC, slot));
         d[12 * s] = static_cast<double>(rg(r, F_T2ACC, slot));
@@ -692,8 +698,8 @@ __global__ void k_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev
     {
         uint32_t slot = static_cast<uint32_t>(c) * cap + t;
         if (!r.valid[slot]) continue;
-        double *d = m.data + static_cast<size_t>(c * kMF) * m.cap + p;
-        size_t s = m.cap;
+        double *d = m.data + static_cast<size_t>(c * kMF) * m.dcap + m.slot[p];
+        size_t s = m.dcap;
         d[10 * s] = rf(r, F_NACC, slot);
         d[11 * s] = rf(r, F_T1ACC, slot);
         d[12 * s] = rf(r, F_T2ACC, slot);
@@ -725,11 +731,12 @@ __global__ void k_uf_init(uint32_t *par, uint32_t n)
     if (i < n) par[i] = i;
 }
 
-__global__ void k_uf_union(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                           bodies_dev b, uint32_t *par)
+// the row manifolds are list[0..nl)
+__global__ void k_uf_union(const uint64_t *keys, const uint32_t *list, uint32_t nl, bodies_dev b, uint32_t *par)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs || colour[p] >= n_colours) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const uint32_t p = list[i];
     uint64_t key = keys[p];
     uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
     if (b.is_static[a] || b.is_static[c]) return;
@@ -753,34 +760,29 @@ __global__ void k_uf_flatten(uint32_t *par, uint32_t n)
 
 // a union that gave up (never seen; bounded so no thread can spin forever) leaves two
 // bodies of one manifold in different islands: flagged, and the caller falls back
-__global__ void k_uf_verify(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                            bodies_dev b, const uint32_t *par, unsigned long long *fail)
+__global__ void k_uf_verify(const uint64_t *keys, const uint32_t *list, uint32_t nl, bodies_dev b, const uint32_t *par,
+                            unsigned long long *fail)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs || colour[p] >= n_colours) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const uint32_t p = list[i];
     uint64_t key = keys[p];
     uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
     if (!b.is_static[a] && !b.is_static[c] && par[a] != par[c]) atomicMax(fail, ~0ull);
 }
 
-// sort key of a row manifold: (island root << cbits) | colour; others sort last
-__global__ void k_island_keys(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                              bodies_dev b, const uint32_t *par, int cbits, uint64_t last, uint64_t *out,
-                              uint32_t *vals)
+// sort key of row manifold list[i]: (island root << cbits) | colour
+__global__ void k_island_keys(const uint64_t *keys, const uint32_t *colour, const uint32_t *list, uint32_t nl,
+                              bodies_dev b, const uint32_t *par, int cbits, uint64_t *out, uint32_t *vals)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    uint32_t c = colour[p];
-    uint64_t k = last;
-    if (c < n_colours)
-    {
-        uint64_t key = keys[p];
-        uint32_t a = static_cast<uint32_t>(key >> 32);
-        if (b.is_static[a]) a = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        k = (static_cast<uint64_t>(par[a]) << cbits) | c;
-    }
-    out[p] = k;
-    vals[p] = p;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const uint32_t p = list[i];
+    const uint64_t key = keys[p];
+    uint32_t a = static_cast<uint32_t>(key >> 32);
+    if (b.is_static[a]) a = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+    out[i] = (static_cast<uint64_t>(par[a]) << cbits) | colour[p];
+    vals[i] = p;
 }
 
 __global__ void k_island_flags(const uint64_t *sk, uint32_t n, uint64_t last, int cbits, uint32_t *flags)
@@ -868,6 +870,210 @@ __device__ __forceinline__ uint16_t slot_of(const uint32_t *ids, uint32_t id)
     return static_cast<uint16_t>(lo);
 }
 
+// ---- register-resident island solve ----
+// A chunk of at most kIslandThreads manifolds gives each thread one manifold for the
+// whole solve: its contacts' normals, lever arms, effective mass, bias, friction block
+// and impulses, and both bodies' inverse mass and world inverse inertia, stay in
+// registers across every pass and colour; only the bodies' velocities move, through
+// LDS. The Jacobians are rebuilt from the normal and lever arms with the calls
+// k_setup made (make_row, basis), so the FP64 solve is the one solve_one runs, bit for
+// bit. The FP32 solve rebuilds them in FP32 from the rounded normal and lever arms.
+template <class T> struct ctc_t
+{
+    T n[3], ra[3], rb[3];
+    T meff, bias, mu, i11, i12, i22, an, a1, a2;
+};
+template <class T> __device__ __forceinline__ void ctc_load(ctc_t<T> &k, const rows_t<T> &r, uint32_t slot)
+{
+    for (int c = 0; c < 3; ++c)
+    {
+        k.n[c] = r.d[static_cast<size_t>(F_NJV + c) * r.cap4 + slot];
+        k.ra[c] = r.d[static_cast<size_t>(F_RA + c) * r.cap4 + slot];
+        k.rb[c] = r.d[static_cast<size_t>(F_RB + c) * r.cap4 + slot];
+    }
+    auto g = [&](int f) { return r.d[static_cast<size_t>(f) * r.cap4 + slot]; };
+    k.meff = g(F_NMEFF);
+    k.bias = g(F_NBIAS);
+    k.mu = g(F_MU);
+    k.i11 = g(F_I11);
+    k.i12 = g(F_I12);
+    k.i22 = g(F_I22);
+    k.an = g(F_NACC);
+    k.a1 = g(F_T1ACC);
+    k.a2 = g(F_T2ACC);
+}
+template <class T> __device__ __forceinline__ void ctc_store_acc(const ctc_t<T> &k, const rows_t<T> &r, uint32_t slot)
+{
+    r.d[static_cast<size_t>(F_NACC) * r.cap4 + slot] = k.an;
+    r.d[static_cast<size_t>(F_T1ACC) * r.cap4 + slot] = k.a1;
+    r.d[static_cast<size_t>(F_T2ACC) * r.cap4 + slot] = k.a2;
+}
+// FP64: the rows of solve_one rebuilt (Jacobians from make_row's and basis' operations)
+__device__ __forceinline__ void ctc_rows(const ctc_t<double> &k, rowv &nr, rowv &r1, rowv &r2)
+{
+    const v3 n = mk(k.n[0], k.n[1], k.n[2]), ra = mk(k.ra[0], k.ra[1], k.ra[2]), rb = mk(k.rb[0], k.rb[1], k.rb[2]);
+    v3 t1, t2;
+    basis(n, t1, t2);
+    nr.Jv = n;
+    nr.Jwa = cross(ra, n);
+    nr.Jwb = -cross(rb, n);
+    nr.Meff = k.meff;
+    nr.bias = k.bias;
+    nr.acc = k.an;
+    r1.Jv = t1;
+    r1.Jwa = cross(ra, t1);
+    r1.Jwb = -cross(rb, t1);
+    r1.Meff = 0.0;
+    r1.bias = 0.0;
+    r1.acc = k.a1;
+    r2.Jv = t2;
+    r2.Jwa = cross(ra, t2);
+    r2.Jwb = -cross(rb, t2);
+    r2.Meff = 0.0;
+    r2.bias = 0.0;
+    r2.acc = k.a2;
+}
+__device__ __forceinline__ void ctc_solve(ctc_t<double> &k, bool warm, body_state &A, body_state &B)
+{
+    rowv nr, t1, t2;
+    ctc_rows(k, nr, t1, t2);
+    if (warm)
+    {
+        row_warm(nr, A, B);
+        row_warm(t1, A, B);
+        row_warm(t2, A, B);
+        return;
+    }
+    row_apply(nr, 0.0, DBL_MAX, A, B);
+    friction_block(t1, t2, k.mu, nr.acc, k.i11, k.i12, k.i22, A, B);
+    k.an = nr.acc;
+    k.a1 = t1.acc;
+    k.a2 = t2.acc;
+}
+// FP32: solve_one32's arithmetic with the Jacobians rebuilt in FP32
+__device__ __forceinline__ f3 fcr(f3 a, f3 b) { return f3m(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__device__ __forceinline__ void fbasis(f3 n, f3 &t1, f3 &t2)
+{
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    t1 = f3m(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    t2 = f3m(b, sign + n.y * n.y * a, -n.y);
+}
+__device__ __forceinline__ void ctc_solve(ctc_t<float> &k, bool warm, body32 &A, body32 &B)
+{
+    const f3 n = f3m(k.n[0], k.n[1], k.n[2]), ra = f3m(k.ra[0], k.ra[1], k.ra[2]), rb = f3m(k.rb[0], k.rb[1], k.rb[2]);
+    f3 t1, t2;
+    fbasis(n, t1, t2);
+    const f3 nJa = fcr(ra, n), nJb = scl(fcr(rb, n), -1.0f);
+    const f3 aJa = fcr(ra, t1), aJb = scl(fcr(rb, t1), -1.0f);
+    const f3 bJa = fcr(ra, t2), bJb = scl(fcr(rb, t2), -1.0f);
+    if (warm)
+    {
+        if (k.an != 0.0f) push32(A, B, scl(n, k.an), scl(nJa, k.an), scl(nJb, k.an));
+        if (k.a1 != 0.0f) push32(A, B, scl(t1, k.a1), scl(aJa, k.a1), scl(aJb, k.a1));
+        if (k.a2 != 0.0f) push32(A, B, scl(t2, k.a2), scl(bJa, k.a2), scl(bJb, k.a2));
+        return;
+    }
+    const float jv = jv32(n, nJa, nJb, A, B);
+    const float lambda = -(jv + k.bias) * k.meff;
+    float nv = k.an + lambda;
+    nv = nv < 0.0f ? 0.0f : nv;
+    const float ap = nv - k.an;
+    k.an = nv;
+    push32(A, B, scl(n, ap), scl(nJa, ap), scl(nJb, ap));
+    const float maxf = k.mu * k.an;
+    const float b1 = jv32(t1, aJa, aJb, A, B), b2 = jv32(t2, bJa, bJb, A, B);
+    float n1 = k.a1 - (k.i11 * b1 + k.i12 * b2), n2 = k.a2 - (k.i12 * b1 + k.i22 * b2);
+    const float mag = sqrtf(n1 * n1 + n2 * n2);
+    if (mag > maxf)
+    {
+        n1 = n1 * (maxf / mag);
+        n2 = n2 * (maxf / mag);
+    }
+    const float a1 = n1 - k.a1, a2 = n2 - k.a2;
+    k.a1 = n1;
+    k.a2 = n2;
+    push32(A, B, addf(scl(t1, a1), scl(t2, a2)), addf(scl(aJa, a1), scl(bJa, a2)), addf(scl(aJb, a1), scl(bJb, a2)));
+}
+__device__ __forceinline__ void body_consts(const bodies_dev &b, uint32_t i, body_state &s) { s = ld_body(b, i); }
+__device__ __forceinline__ void body_consts(const bodies_dev &b, uint32_t i, body32 &s) { s = ld_body32(b, i); }
+
+// the whole solve of one chunk, one manifold per thread (m <= kIslandThreads)
+template <bool F32>
+__device__ __forceinline__ void solve_chunk_reg(const uint64_t *keys, const uint32_t *order, const uint64_t *sk,
+                                                uint32_t s0, uint32_t m, uint64_t cmask, uint32_t n_colours,
+                                                const bodies_dev &b, const rows_dev &rd, uint32_t cap, uint32_t iters,
+                                                const vel_lds &L, const uint16_t (*sl)[kLdsManifolds])
+{
+    using T = typename std::conditional<F32, float, double>::type;
+    using B = typename std::conditional<F32, body32, body_state>::type;
+    const rows_t<T> r = rows_as<T>(rd);
+    const uint32_t i = threadIdx.x;
+    const bool own = i < m;
+    const uint32_t t = s0 + i;
+    uint32_t col = kNone, ia = 0, ib = 0;
+    uint16_t sa = kNoSlot, sb = kNoSlot;
+    B A, Bb;
+    ctc_t<T> k[4];
+    bool valid[4] = {false, false, false, false};
+    v3 sva = mk(0.0, 0.0, 0.0), swa = sva, svb = sva, swb = sva; // velocities of static bodies
+    if (own)
+    {
+        col = static_cast<uint32_t>(sk[t] & cmask);
+        const uint64_t key = keys[order[t]];
+        ia = static_cast<uint32_t>(key >> 32);
+        ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        sa = sl[0][i];
+        sb = sl[1][i];
+        body_consts(b, ia, A);
+        body_consts(b, ib, Bb);
+        if (sa == kNoSlot) vel_global().load(b, ia, kNoSlot, sva, swa);
+        if (sb == kNoSlot) vel_global().load(b, ib, kNoSlot, svb, swb);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const uint32_t slot = static_cast<uint32_t>(c) * cap + t;
+            valid[c] = r.valid[slot] != 0;
+            if (valid[c]) ctc_load(k[c], r, slot);
+        }
+    }
+    for (uint32_t pass = 0; pass <= iters; ++pass)
+    {
+        const bool warm = pass == 0;
+        for (uint32_t c = 0; c < n_colours; ++c)
+        {
+            if (own && col == c)
+            {
+                if (sa == kNoSlot)
+                {
+                    A.vel = sva;
+                    A.ang = swa;
+                }
+                else
+                    L.load(b, ia, sa, A.vel, A.ang);
+                if (sb == kNoSlot)
+                {
+                    Bb.vel = svb;
+                    Bb.ang = swb;
+                }
+                else
+                    L.load(b, ib, sb, Bb.vel, Bb.ang);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (valid[q]) ctc_solve(k[q], warm, A, Bb);
+                if (!A.stat) L.store(b, ia, sa, A.vel, A.ang);
+                if (!Bb.stat) L.store(b, ib, sb, Bb.vel, Bb.ang);
+            }
+            __syncthreads();
+        }
+    }
+    if (own)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (valid[q]) ctc_store_acc(k[q], r, static_cast<uint32_t>(q) * cap + t);
+}
+
 template <bool F32>
 __global__ void __launch_bounds__(kIslandThreads)
     k_solve_islands(const uint64_t *keys, const uint32_t *order, const uint64_t *sk, const uint32_t *starts,
@@ -942,6 +1148,9 @@ __global__ void __launch_bounds__(kIslandThreads)
         __syncthreads();
     }
     vel_lds L{lv};
+    if (use_lds && m <= kIslandThreads)
+        solve_chunk_reg<F32>(keys, order, sk, s0, m, cmask, n_colours, b, r, cap, iters, L, sl);
+    else
     for (uint32_t pass = 0; pass <= iters; ++pass)
     {
         const int warm = pass == 0 ? 1 : 0;
@@ -1045,16 +1254,17 @@ void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_d
     if (n_act) k_writeback<<<grid(n_act), kBlock, 0, st>>>(order, n_act, m, r, cap);
 }
 
-island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                               const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv, uint64_t *ik_alt,
-                               uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc, hipStream_t st)
+island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, const uint32_t *list, uint32_t nl,
+                               uint32_t n_colours, const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv,
+                               uint64_t *ik_alt, uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc,
+                               hipStream_t st)
 {
     island_sort is{};
     const uint32_t n = b.n;
     k_uf_init<<<grid(n), kBlock, 0, st>>>(par, n);
-    k_uf_union<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par);
+    if (nl) k_uf_union<<<grid(nl), kBlock, 0, st>>>(keys, list, nl, b, par);
     k_uf_flatten<<<grid(n), kBlock, 0, st>>>(par, n);
-    k_uf_verify<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par, fail);
+    if (nl) k_uf_verify<<<grid(nl), kBlock, 0, st>>>(keys, list, nl, b, par, fail);
     int cbits = 1;
     while (cbits < 31 && (1u << cbits) <= n_colours) ++cbits;
     int ibits = 1;
@@ -1062,11 +1272,11 @@ island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uin
     const int bits = cbits + ibits;
     is.cbits = cbits;
     is.last = (1ull << bits) - 1;
-    k_island_keys<<<grid(n_pairs), kBlock, 0, st>>>(keys, colour, n_pairs, n_colours, b, par, cbits, is.last, ik, iv);
+    if (nl) k_island_keys<<<grid(nl), kBlock, 0, st>>>(keys, colour, list, nl, b, par, cbits, ik, iv);
     int shifts[8];
     int ns = 0;
     for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
-    bool alt = radix_sort_u64(ik, iv, ik_alt, iv_alt, n_pairs, shifts, ns, sc, st);
+    bool alt = nl > 1 && radix_sort_u64(ik, iv, ik_alt, iv_alt, nl, shifts, ns, sc, st);
     is.sk = alt ? ik_alt : ik;
     is.ord = alt ? iv_alt : iv;
     return is;

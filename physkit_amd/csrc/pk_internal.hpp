@@ -52,12 +52,16 @@ struct shapes_dev
     const uint32_t *uv;  // n: UV-sphere tag ((stacks << 16) | sectors) or 0
 };
 
-// manifold cache: up to 4 contacts x 13 fields per pair, planar with stride `cap`
-// field f of contact c of pair p at data[(c * 13 + f) * cap + p]
+// manifold cache: a contact count and a data slot per pair (kNone when empty); up to
+// 4 contacts x 13 fields per slot, planar with stride dcap: field f of contact c of the
+// manifold in slot s at data[(c * 13 + f) * dcap + s]. Only pairs with contacts hold a
+// slot, so the data scales with the contacts, not with the broadphase pairs.
 struct manifolds_dev
 {
-    uint32_t cap;
+    uint32_t cap;  // pairs
     int32_t *count;
+    uint32_t *slot;
+    uint32_t dcap; // slots
     double *data;
 };
 constexpr int kMF = 13; // normal3 local_a3 local_b3 depth ln lt1 lt2
@@ -85,6 +89,7 @@ enum counter_slot
     C_NLIST = 16,
     C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
     C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
+    C_MSLOTS = 19,    // manifold data slots taken by the manifold update
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
     // addresses instead of serialising on one
@@ -199,12 +204,13 @@ void launch_merge(const uint64_t *xk, const uint32_t *xs, uint32_t nx, const uin
 // ---------------- k_narrow.hip ----------------
 struct gjk_out_dev
 {
-    uint32_t *hit_list;   // compacted pair indices whose GJK found an intersection
+    uint32_t *hit_list;   // compacted pair indices whose GJK found an intersection (cap entries;
+                          // hits past cap are counted, not written)
     double *simplex;      // per hit-list entry: n, then 4 x 9 doubles (p, pa, pb), planar stride cap
     uint32_t cap;
-    double *info;         // per pair: normal3 wa3 wb3 depth, planar stride pair_cap
+    double *info;         // per hit-list entry: normal3 wa3 wb3 depth, planar stride cap
+    uint32_t *hit_of;     // per pair: its hit-list entry (read only where status is 1)
     uint8_t *status;      // per pair: 0 separated, 1 contact, 2 degenerate, 3 overflow
-    uint32_t pair_cap;
 };
 // FP32 copies of the shapes for the certified-separation pass (k_gjk32.hip): vertex i
 // of shape s at v[off[s] + i] (xyz, w = 0), rounded to nearest; rad rounded up
@@ -239,14 +245,25 @@ struct gjk32_in
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
                 gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,
                 hipStream_t st);
+// Launch shapes the autotuner picks (pkg_autotune); none changes a result.
+struct tuning
+{
+    int gjk_group = 4;           // lanes per pair in k_gjk (1, 2, 4, 8): 64 / gjk_group pairs per wavefront
+    int epa_block = 64;          // threads per workgroup of k_epa (64, 128, 256)
+    uint32_t epa_lds_max = 512;  // batches of at most this many GJK hits run one EPA solve per workgroup
+                                 // with the LDS store (k_epa_lds); larger ones one per thread
+};
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
-                gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
+                const tuning &tu, gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
 // overflowed hits rerun with an LDS store; its own overflows are listed in ovf2_list
 // (count at counters[C_EPA_OVF2]) for launch_epa_big
 void launch_epa_lds(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf, const bodies_dev &b,
                     const shapes_dev &s, const gjk_out_dev &g, uint32_t *ovf2_list, unsigned long long *counters,
                     hipStream_t st);
 size_t epa_big_bytes();
+// launch_epa runs batches this small with the LDS store already (its overflows skip
+// launch_epa_lds)
+bool epa_first_tier_is_lds(uint32_t n_hits, const tuning &tu);
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
                     const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,
                     unsigned long long *counters, hipStream_t st);
@@ -280,7 +297,7 @@ struct rows_dev
     double *d;       // kRowF planes of cap4
     uint8_t *valid;  // cap4
 };
-constexpr int kRowF = 38;
+constexpr int kRowF = 44; // 38 row words + the contact's lever arms (6)
 constexpr uint32_t kUncoloured = 0xFFFFFFFEu; // colour of an item still to be coloured
 // colour[p] = kUncoloured for manifolds that make solver rows (else kNone); their row
 // count at counters[C_ROWS]
@@ -331,9 +348,12 @@ struct island_sort
 constexpr uint32_t kIslandMaxManifolds = 4096; // larger islands use the per-colour launches
 constexpr uint32_t kIslandMaxColours = 64;  // above: the per-colour launches
 // `fail` becomes all-ones if the islands could not be built (the caller then falls back)
-island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours,
-                               const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv, uint64_t *ik_alt,
-                               uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc, hipStream_t st);
+// over the nl row manifolds list[0..nl) (the colouring's list); positions of the sorted
+// order index that list's manifolds, so n_pairs below is nl
+island_sort launch_island_sort(const uint64_t *keys, const uint32_t *colour, const uint32_t *list, uint32_t nl,
+                               uint32_t n_colours, const bodies_dev &b, uint32_t *par, uint64_t *ik, uint32_t *iv,
+                               uint64_t *ik_alt, uint32_t *iv_alt, unsigned long long *fail, sort_scratch &sc,
+                               hipStream_t st);
 // island starts (positions where the island changes) and the row-manifold count
 void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *flags, uint32_t *pos, uint32_t *bsums,
                             uint32_t *total, uint32_t *starts, uint32_t *n_act, hipStream_t st);

@@ -98,6 +98,28 @@ int main()
         threw = true;
     }
     REQUIRE(threw);
+
+    // a step that fails part-way: the facade throws and refuses to step again
+    threw = false;
+    try
+    {
+        w.step(1234.5);
+    }
+    catch (const std::runtime_error &)
+    {
+        threw = true;
+    }
+    REQUIRE(threw);
+    threw = false;
+    try
+    {
+        w.step(0.1);
+    }
+    catch (const std::runtime_error &)
+    {
+        threw = true;
+    }
+    REQUIRE(threw);
     std::printf("facade logic ok\n");
     return 0;
 }

@@ -60,6 +60,8 @@ pkg_status pkg_set_body_state(pkg_ctx *, uint32_t, const uint32_t *, const doubl
 pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps)
 {
     if (!ctx) return PKG_INVALID_ARG;
+    // dt = 1234.5 stands for a step that fails part-way (the library's PKG_OOM case)
+    const bool fail = dt == 1234.5;
     for (int s = 0; s < nsteps; ++s)
         for (uint32_t i = 0; i < ctx->n; ++i)
         {
@@ -70,7 +72,7 @@ pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps)
                 ctx->pos[3 * i + k] += ctx->vel[3 * i + k] * dt;
             }
         }
-    return PKG_OK;
+    return fail ? PKG_OOM : PKG_OK;
 }
 pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out)
 {

@@ -33,3 +33,24 @@ def test_reference_scene_bitexact(pk, oracle, path):
         contacts += g.counters()["contacts"]
     assert contacts > 0
     g.close()
+
+
+@pytest.mark.parametrize("path", SCENES[:4], ids=[os.path.basename(p) for p in SCENES[:4]])
+def test_cpp_scene_reader_through_the_facade(pk, path):
+    """The same files read by the C++ reader (include/physkit_amd/scene_json.hpp) into
+    the C++ facade world (tests/cpp/scene_run.cpp) and stepped 90 steps reach the poses
+    the Python path reaches, bit for bit."""
+    import subprocess
+    from physkit_amd import scene_json
+    exe = os.path.join(HERE, "cpp", "_build", "scene_run")
+    r = subprocess.run([exe, path, "90", repr(1 / 60)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().split("\n")])
+    s = scene_json.load(path)
+    g = pk.GpuWorld(gravity=s["gravity"], iterations=s["iterations"])
+    g.load_scene(s)
+    g.step(1 / 60, 90)
+    b = g.bodies()
+    np.testing.assert_array_equal(got[:, :3], b["pos"])
+    np.testing.assert_array_equal(got[:, 3:], b["quat"])
+    g.close()

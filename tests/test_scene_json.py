@@ -78,3 +78,61 @@ def test_bad_scenes_are_rejected():
     s = scene_json.parse({"objects": [{"pos": [0, 1, 0], "type": "stat", "mesh": {"name": "box", "half_extents": [1, 2, 3]}},
                                       {"pos": [0, 3, 0], "mass": 2.0, "mesh": {"name": "box", "half_extents": [1, 2, 3]}}]})
     assert list(s["shape"]) == [0, 0] and len(s["shape_offsets"]) == 2  # one shared shape
+
+
+@pytest.fixture(scope="module")
+def scene_dump(tmp_path_factory):
+    """tests/cpp/scene_dump.cpp: the C++ reader (include/physkit_amd/scene_json.hpp)."""
+    import subprocess
+    root = os.path.dirname(HERE)
+    exe = str(tmp_path_factory.mktemp("cpp") / "scene_dump")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
+                    os.path.join(HERE, "cpp", "scene_dump.cpp"), "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("path", SCENES, ids=[os.path.basename(p) for p in SCENES])
+def test_cpp_reader_matches(scene_dump, path):
+    """The C++ reader builds, field for field and vertex for vertex, what the Python
+    reader builds from the same file (both follow graphics.cpp:20-135)."""
+    import subprocess
+    from physkit_amd import scene_json
+    s = scene_json.load(path)
+    out = subprocess.run([scene_dump, path], capture_output=True, text=True, check=True).stdout.split("\n")
+    w = out[0].split()
+    assert w[0] == "world"
+    assert [float(x) for x in w[1:4]] == list(s["gravity"]) and int(w[4]) == s["iterations"]
+    n = int(w[5])
+    assert n == len(s["shape"])
+    for i in range(n):
+        f = out[1 + i].split()
+        assert f[0] == "object"
+        vals = [float(x) for x in f[2:21]]
+        assert int(f[1]) == int(s["is_static"][i])
+        # a static body's mass is infinite whatever the file says (object.h:81-85): not compared
+        mass = 0.0 if s["is_static"][i] else s["mass"][i]
+        exp = (list(s["pos"][i]) + list(s["vel"][i]) + list(s["quat"][i]) + list(s["angvel"][i]) +
+               [s["inertia"][i, 0], s["inertia"][i, 4], s["inertia"][i, 8], mass,
+                s["restitution"][i], s["friction"][i]])
+        assert vals == exp, i
+        nv = int(f[21])
+        v = np.array([float(x) for x in f[22:22 + 3 * nv]]).reshape(nv, 3)
+        sh = s["shape"][i]
+        np.testing.assert_array_equal(v, s["verts"][s["shape_offsets"][sh]:s["shape_offsets"][sh + 1]])
+
+
+def test_cpp_reader_rejects_what_the_schema_does_not_allow(scene_dump, tmp_path):
+    import subprocess
+    cases = {
+        "no_pos": '{"objects": [{"mass": 1}]}',
+        "bad_type": '{"objects": [{"pos": [0,0,0], "type": "kinematic", "mass": 1}]}',
+        "no_mass": '{"objects": [{"pos": [0,0,0]}]}',
+        "bad_mesh": '{"objects": [{"pos": [0,0,0], "mass": 1, "mesh": {"name": "torus"}}]}',
+        "thin_sphere": '{"objects": [{"pos": [0,0,0], "mass": 1, "mesh": {"name": "sphere", "radius": 1, "stacks": 1}}]}',
+        "broken": '{"objects": [',
+    }
+    for name, text in cases.items():
+        p = tmp_path / f"{name}.json"
+        p.write_text(text)
+        out = subprocess.run([scene_dump, str(p)], capture_output=True, text=True, check=True).stdout
+        assert out.startswith("error "), (name, out)
