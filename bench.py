@@ -51,6 +51,8 @@ WORKLOADS = {
                 "262,144-hull cluster scene's broadphase tree, FP64 slab tests, rays resident in HBM"),
 }
 RAYS = 1 << 22
+# LBVH refits between full builds on the measured worlds (pkg_tuning.tree_refits)
+TREE_REFITS = 7
 
 
 def support_vertices(s):
@@ -254,20 +256,43 @@ def precision_drift(s, steps, flags_lo):
     return out
 
 
-def timed_run(pk, s, flags, warmup, steps, autotune=True):
-    """Loads s into a context with flags, runs warmup untimed steps (then, with
-    autotune, pkg_autotune's 10 untimed steps picking the launch shapes), then `steps`
-    timed steps (each pkg_step_timed call returns after its device work has finished, so
-    the timed region is bracketed by device synchronisation). Returns wall seconds,
-    summed stage milliseconds, pair and hit totals, the last counters, pairs, positions
-    and the launch shapes used."""
+def tuned_shapes(pk, s, flags, warmup):
+    """Launch shapes from pkg_autotune (3 steps, every candidate timed on each step's
+    own inputs), run on a separate copy of the world after the same warmup steps, so
+    the measured world's timed steps are the same steps with or without tuning."""
     g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
     g.load_scene(s)
     if len(s["first_body"]) > 2:
         g.set_worlds(s["first_body"])
+    g.step(s["dt"], warmup)
+    t = g.autotune(s["dt"], 3)
+    g.close()
+    return t
+
+
+def timed_run(pk, s, flags, warmup, steps, autotune=True, fixed=None, tree_refits=0):
+    """Loads s into a context with flags, runs warmup untimed steps, then `steps` timed
+    steps (each pkg_step_timed call returns after its device work has finished, so the
+    timed region is bracketed by device synchronisation), with the launch shapes
+    pkg_autotune picked on a copy of the world (tuned_shapes), the `fixed` ones
+    (gjk_lanes, epa_block, epa_lds_max) or the defaults, and tree_refits LBVH refits
+    between full builds. Returns wall seconds, summed
+    stage milliseconds, pair and hit totals, the last counters, pairs, positions and the
+    launch shapes used."""
+    if fixed:
+        tuning = dict(zip(("gjk_lanes", "epa_block", "epa_lds_max"), fixed))
+    else:
+        tuning = tuned_shapes(pk, s, flags, warmup) if autotune else None
+    g = pk.GpuWorld(iterations=s["iterations"], flags=flags)
+    g.load_scene(s)
+    if len(s["first_body"]) > 2:
+        g.set_worlds(s["first_body"])
+    if tuning is None:
+        tuning = g.tuning()
+    g.set_tuning(tuning["gjk_lanes"], tuning["epa_block"], tuning["epa_lds_max"], tree_refits)
+    tuning = dict(tuning, tree_refits=tree_refits)
     for _ in range(warmup):
         g.step(s["dt"])
-    tuning = g.autotune(s["dt"], 1) if autotune else g.tuning()
     stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
     pairs_total = hits_total = 0
     t0 = time.perf_counter()
@@ -509,7 +534,15 @@ def main():
                          "only the solve in FP32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch shapes instead of pkg_autotune's (10 untimed steps)")
+                    help="keep the default launch shapes instead of those pkg_autotune picks on a copy "
+                         "of the world")
+    ap.add_argument("--tuning", default=None, metavar="LANES,BLOCK,LDS_MAX",
+                    help="fixed launch shapes (GJK lanes per pair, EPA workgroup size, LDS-store "
+                         "threshold) instead of pkg_autotune's, e.g. to profile the shapes a bench "
+                         "line reports")
+    ap.add_argument("--tree-refits", type=int, default=TREE_REFITS, metavar="N",
+                    help="steps between full LBVH builds that refit the last hierarchy instead "
+                         f"(pkg_tuning.tree_refits; default {TREE_REFITS})")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison runs (batched FP64, per-world drop-in, precision drift)")
     ap.add_argument("--n-rays", type=int, default=RAYS, help="rays workload: rays per launch")
@@ -536,7 +569,11 @@ def main():
     n_dyn = len(s["shape"]) - int(s["n_static"])
     print(f"[bench] workload={args.workload} precision={prec} bodies={len(s['shape'])} steps={args.steps} "
           f"warmup={args.warmup}", file=sys.stderr, flush=True)
-    r = timed_run(pk, s, s["flags"] | extra, args.warmup, args.steps, autotune=not args.no_autotune)
+    fixed = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
+    if fixed is not None and len(fixed) != 3:
+        ap.error("--tuning takes three integers: LANES,BLOCK,LDS_MAX")
+    r = timed_run(pk, s, s["flags"] | extra, args.warmup, args.steps, autotune=not args.no_autotune,
+                  fixed=fixed, tree_refits=args.tree_refits)
     wall, stage, c, pairs = r["wall"], r["stage"], r["counters"], r["pairs"]
     print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
@@ -579,7 +616,8 @@ def main():
         if batched:
             # the same worlds batched in FP64, and the plain FP64 drop-in: worlds one at a time
             if prec != "fp64":
-                r64 = timed_run(pk, s, s["flags"], args.warmup, args.steps, autotune=not args.no_autotune)
+                r64 = timed_run(pk, s, s["flags"], args.warmup, args.steps, autotune=not args.no_autotune,
+                                fixed=fixed, tree_refits=args.tree_refits)
                 v64 = n_dyn * args.steps / r64["wall"]
                 out["config"]["batched_fp64"] = {"value": round(v64, 1), "unit": "body-steps/s",
                                                  "ms_per_step": round(r64["wall"] * 1e3 / args.steps, 3),

@@ -300,18 +300,24 @@ typedef struct pkg_tuning
                               or 256 (default 64) */
     uint32_t epa_lds_max;  /* GJK-hit batches of at most this many run one EPA solve per workgroup
                               with the polytope in LDS; larger ones one solve per thread (512) */
+    uint32_t tree_refits;  /* steps between full LBVH builds that refit the last build's hierarchy
+                              to the new leaf boxes instead (0-64; default 7; 0 builds every
+                              step). Pairs and query answers do not depend on the tree's shape,
+                              so this changes only the walks' cost. Not chosen by pkg_autotune */
     float gjk_ms[4];       /* pkg_autotune: GJK stage ms per step with 1, 2, 4, 8 lanes per pair */
     float epa_block_ms[3]; /* EPA stage ms per step with workgroups of 64, 128, 256 threads */
     float epa_lds_ms[3];   /* EPA stage ms per step with epa_lds_max 0, 512, 4096 */
 } pkg_tuning;
 
-/* Picks the launch shapes above by timing them on this world: it takes
- * 10 x steps_per_candidate real steps (the world advances as under pkg_step, with the
- * same results), the candidates of one shape taking turns step by step, and keeps the
- * fastest of each (GJK lanes on the GJK stage, then the EPA block size with every batch
- * per thread, then the LDS threshold). pkg_upload_bodies returns to the defaults. out
- * (optional) receives the choice and the times measured. */
-pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps_per_candidate, pkg_tuning *out);
+/* Picks the launch shapes above by timing them on this world. It takes `steps` real
+ * steps (the world advances as under pkg_step, with the same results); in each, after
+ * the GJK and EPA stages have run, it runs them again with every candidate shape on
+ * that step's own pairs and hits (both stages are idempotent) and times each run. It
+ * keeps the fastest of each: GJK lanes on the GJK stage, the EPA block size with every
+ * batch per thread, then the LDS threshold with the step's fastest block size.
+ * pkg_upload_bodies returns to the defaults. out (optional) receives the choice and the
+ * mean times measured. */
+pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps, pkg_tuning *out);
 pkg_status pkg_get_tuning(pkg_ctx *ctx, pkg_tuning *out);
 /* Sets the shapes directly (the *_ms fields are ignored); PKG_INVALID_ARG for values
  * outside the lists above. */

@@ -81,6 +81,7 @@ class pkg_counters(C.Structure):
 
 class pkg_tuning(C.Structure):
     _fields_ = [("gjk_lanes", C.c_int32), ("epa_block", C.c_int32), ("epa_lds_max", C.c_uint32),
+                ("tree_refits", C.c_uint32),
                 ("gjk_ms", C.c_float * 4), ("epa_block_ms", C.c_float * 3), ("epa_lds_ms", C.c_float * 3)]
 
 
@@ -290,23 +291,27 @@ class GpuWorld:
         self._check(self.L.pkg_step_timed(self.h, dt, nsteps, _dp(ms), len(self.STAGES)), "step_timed")
         return dict(zip(self.STAGES, ms.tolist()))
 
-    def autotune(self, dt, steps_per_candidate=1):
-        """pkg_autotune: times the launch-shape candidates over real steps (the world
-        advances 10 x steps_per_candidate steps) and keeps the fastest; returns the
-        choice and the measured stage milliseconds."""
+    def autotune(self, dt, steps=1):
+        """pkg_autotune: steps the world `steps` times, timing every launch-shape
+        candidate of the GJK and EPA stages on each step's own inputs, and keeps the
+        fastest; returns the choice and the measured stage milliseconds."""
         t = pkg_tuning()
-        self._check(self.L.pkg_autotune(self.h, dt, steps_per_candidate, C.byref(t)), "autotune")
+        self._check(self.L.pkg_autotune(self.h, dt, steps, C.byref(t)), "autotune")
         return {"gjk_lanes": t.gjk_lanes, "epa_block": t.epa_block, "epa_lds_max": t.epa_lds_max,
-                "gjk_ms": [round(x, 4) for x in t.gjk_ms], "epa_block_ms": [round(x, 4) for x in t.epa_block_ms],
+                "tree_refits": t.tree_refits, "gjk_ms": [round(x, 4) for x in t.gjk_ms], "epa_block_ms": [round(x, 4) for x in t.epa_block_ms],
                 "epa_lds_ms": [round(x, 4) for x in t.epa_lds_ms]}
 
     def tuning(self):
         t = pkg_tuning()
         self._check(self.L.pkg_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return {"gjk_lanes": t.gjk_lanes, "epa_block": t.epa_block, "epa_lds_max": t.epa_lds_max}
+        return {"gjk_lanes": t.gjk_lanes, "epa_block": t.epa_block, "epa_lds_max": t.epa_lds_max,
+                "tree_refits": t.tree_refits}
 
-    def set_tuning(self, gjk_lanes, epa_block, epa_lds_max):
-        t = pkg_tuning(gjk_lanes=gjk_lanes, epa_block=epa_block, epa_lds_max=epa_lds_max)
+    def set_tuning(self, gjk_lanes, epa_block, epa_lds_max, tree_refits=None):
+        """pkg_set_tuning; tree_refits None keeps the current value."""
+        if tree_refits is None:
+            tree_refits = self.tuning()["tree_refits"]
+        t = pkg_tuning(gjk_lanes=gjk_lanes, epa_block=epa_block, epa_lds_max=epa_lds_max, tree_refits=tree_refits)
         self._check(self.L.pkg_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def set_state(self, ids, pos=None, vel=None, angvel=None, quat=None):

@@ -47,8 +47,23 @@ template <class T> struct dbuf
     void reserve(size_t n) // contents are not kept
     {
         if (n <= cap) return;
+        const bool regrow = cap > 0;
         release();
         size_t want = std::max<size_t>(n, 1);
+        // A buffer that grows again (its count creeping up step by step) takes a quarter
+        // more, so the step does not pay a hipFree (which waits for the device) and a
+        // hipMalloc each time the count reaches a new high; exactly n if that does not fit.
+        if (regrow)
+        {
+            const size_t roomy = want + want / 4;
+            if (hipMalloc(reinterpret_cast<void **>(&p), roomy * sizeof(T)) == hipSuccess)
+            {
+                cap = roomy;
+                return;
+            }
+            p = nullptr;
+            (void)hipGetLastError();
+        }
         hip_check(hipMalloc(reinterpret_cast<void **>(&p), want * sizeof(T)), "hipMalloc");
         cap = want;
     }
@@ -57,6 +72,11 @@ template <class T> struct dbuf
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+    }
+    void swap(dbuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
     }
 };
 
@@ -177,7 +197,7 @@ struct pkg_ctx
     dbuf<uint4> nrec;
     dbuf<double> leaf_box;
     dbuf<float4> leaf_f32; // with PKG_FLAG_FP32_BROAD
-    dbuf<uint8_t> leaf_moved, covered;
+    dbuf<uint8_t> leaf_moved;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
     // batched worlds
@@ -207,6 +227,14 @@ struct pkg_ctx
     // pkg_autotune or pkg_set_tuning picks them
     tuning tune;
     bool tuned = false;
+    // pkg_autotune: while set, each step also times every GJK and EPA launch shape on
+    // that step's own inputs (the stages are idempotent, so the step's results do not
+    // change), adding the milliseconds to these sums
+    struct shape_trials
+    {
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        double gjk[4] = {0, 0, 0, 0}, epa_block[3] = {0, 0, 0}, epa_lds[3] = {0, 0, 0};
+    } *trials = nullptr;
     dbuf<double> simplex, info;
     dbuf<uint8_t> status;
     dbuf<char> big_pool;
@@ -242,6 +270,8 @@ struct pkg_ctx
 
     // world queries
     bool tree_ok = false; // the LBVH holds the current tree bounds (built by the last step or query)
+    bool topo_ok = false; // the LBVH's leaf order and hierarchy belong to the current bodies and worlds
+    uint32_t topo_age = 0; // refits since the last full build
     bool rn_ok = false;   // q_rn holds the walk topology of that tree
     dbuf<uint4> q_rn;
     dbuf<uint32_t> q_cnt, q_off, q_hid, q_hq, q_perm, q_perm_alt, q_bsums, q_total, q_u32;
@@ -431,11 +461,21 @@ lbvh_dev pkg_ctx::tree()
 lbvh_dev pkg_ctx::build_tree(const bodies_dev &b, bool all_moved)
 {
     lbvh_dev t = tree();
-    launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
-    if (t.morton != morton.p)
+    if (topo_ok && topo_age < tune.tree_refits)
     {
-        std::swap(morton.p, morton_alt.p);
-        std::swap(ids.p, ids_alt.p);
+        launch_lbvh_refit(t, b, all_moved, st);
+        ++topo_age;
+    }
+    else
+    {
+        launch_lbvh_build(t, b, all_moved, sc, counters.p, st);
+        if (t.morton != morton.p)
+        {
+            std::swap(morton.p, morton_alt.p);
+            std::swap(ids.p, ids_alt.p);
+        }
+        topo_ok = true;
+        topo_age = 0;
     }
     tree_ok = true;
     rn_ok = false;
@@ -590,7 +630,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         rcap = static_cast<uint32_t>(std::min<size_t>(qout.cap / nreg, 0xFFFFFFF0u / nreg));
         launch_fill_u64(counters.p + C_QCUR, 0ull, static_cast<size_t>(nreg) * kCursorStride, st);
         launch_pair_query(t, b, qout.p, rcap, nreg, counters.p, st);
-        read_counters(C_TREE_FAIL, C_QCUR + static_cast<int>((nreg - 1) * kCursorStride));
+        read_counters(C_MOVED, C_QCUR + static_cast<int>((nreg - 1) * kCursorStride));
         if (h_counters[C_TREE_FAIL]) throw pk_error(PKG_DEVICE_ERROR, "broadphase tree walk exceeded its node count");
         unsigned long long cnt = 0, most = 0;
         for (uint32_t r = 0; r < nreg; ++r)
@@ -634,32 +674,46 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     }
     // carry-over: a candidate found in the previous sorted list keeps its manifold;
     // previous pairs no candidate covers stay while their leaves overlap
-    srcb.reserve(std::max<uint32_t>(M, 1));
-    covered.reserve(static_cast<size_t>(Pprev) + 4);
-    if (Pprev) launch_fill_u32(reinterpret_cast<uint32_t *>(covered.p), 0u, (Pprev + 3) / 4, st);
-    launch_pair_match(ck, M, keys[cur].p, Pprev, srcb.p, covered.p, st);
     uint32_t K = 0;
-    if (!all_moved && Pprev)
+    if (!all_moved && Pprev && h_counters[C_MOVED] == 0)
     {
-        flag.reserve(Pprev);
-        fpos.reserve(Pprev);
-        bsums.reserve(scan_block_sums(Pprev));
-        total.reserve(1);
-        launch_prev_keep(keys[cur].p, Pprev, covered.p, b, flag.p, st);
-        exclusive_scan_u32(flag.p, fpos.p, Pprev, bsums.p, total.p, st);
-        uint32_t h_total = 0;
-        hip_check(hipMemcpyAsync(&h_total, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total copy");
-        hip_check(hipStreamSynchronize(st), "stream sync");
-        K = h_total;
-        akeys.reserve(std::max<uint32_t>(K, 1));
-        asrc.reserve(std::max<uint32_t>(K, 1));
-        launch_prev_compact(keys[cur].p, Pprev, flag.p, fpos.p, akeys.p, asrc.p, st);
+        // No leaf moved: there are no candidates and every previous pair keeps both fat
+        // boxes, so the carry-over would keep the previous list whole and in order. It
+        // is taken over as it stands, each pair's manifold coming from its own index.
+        if (M != 0) throw pk_error(PKG_DEVICE_ERROR, "broadphase found candidates with no leaf moved");
+        K = Pprev;
+        keys[nxt].swap(keys[cur]);
+        src.reserve(Pprev);
+        launch_iota_u32(src.p, Pprev, st);
+    }
+    else
+    {
+        srcb.reserve(std::max<uint32_t>(M, 1));
+        launch_pair_match(ck, M, keys[cur].p, Pprev, srcb.p, st);
+        if (!all_moved && Pprev)
+        {
+            flag.reserve(Pprev);
+            fpos.reserve(Pprev);
+            bsums.reserve(scan_block_sums(Pprev));
+            total.reserve(1);
+            launch_prev_keep(keys[cur].p, Pprev, b, flag.p, st);
+            exclusive_scan_u32(flag.p, fpos.p, Pprev, bsums.p, total.p, st);
+            uint32_t h_total = 0;
+            hip_check(hipMemcpyAsync(&h_total, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "total copy");
+            hip_check(hipStreamSynchronize(st), "stream sync");
+            K = h_total;
+            akeys.reserve(std::max<uint32_t>(K, 1));
+            asrc.reserve(std::max<uint32_t>(K, 1));
+            launch_prev_compact(keys[cur].p, Pprev, flag.p, fpos.p, akeys.p, asrc.p, st);
+        }
+        if (static_cast<unsigned long long>(M) + K > 0xFFFFFFF0ull)
+            throw pk_error(PKG_OVERFLOW_PAIRS, "pair count exceeds 32-bit indexing");
+        keys[nxt].reserve(std::max<size_t>(static_cast<size_t>(M) + K, 1));
+        src.reserve(std::max<size_t>(static_cast<size_t>(M) + K, 1));
+        launch_merge(ck, srcb.p, M, akeys.p, K, keys[nxt].p, src.p, st);
+        launch_merge(akeys.p, asrc.p, K, ck, M, keys[nxt].p, src.p, st);
     }
     const uint32_t Pn = M + K;
-    keys[nxt].reserve(std::max<uint32_t>(Pn, 1));
-    src.reserve(std::max<uint32_t>(Pn, 1));
-    launch_merge(ck, srcb.p, M, akeys.p, K, keys[nxt].p, src.p, st);
-    launch_merge(akeys.p, asrc.p, K, ck, M, keys[nxt].p, src.p, st);
     record(ev, 3, st);
 
     // ---- 3. narrowphase (collision_phases.h:244-320)
@@ -708,32 +762,41 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             f32.prev_count = mcount[cur].p;
             f32.list2 = gjk_list2.p;
         }
-        launch_gjk(keys[nxt].p, Pn, b, s, has_uv, tune.gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
+        // the GJK stage; it is idempotent once its three counters are cleared (they start
+        // the step at zero)
+        auto gjk_stage = [&](int group, bool again)
+        {
+            if (again)
+                for (int c : {C_GJK_LIST, C_GJK_LIST2, C_HITS}) reset_counter(c);
+            launch_gjk(keys[nxt].p, Pn, b, s, has_uv, group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
+        };
+        gjk_stage(tune.gjk_group, false);
         record(ev, 4, st);
         uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
         if (hits > g.cap)
         {
-            // more hits than the hit lists hold: grow them and run the (idempotent) GJK
-            // stage again
+            // more hits than the hit lists hold: grow them and run the stage again
             size_hits(hits + hits / 4);
-            for (int c : {C_GJK_LIST, C_GJK_LIST2, C_HITS}) reset_counter(c);
-            launch_gjk(keys[nxt].p, Pn, b, s, has_uv, tune.gjk_group, g, gjk_list.p, cull_rec.p, f32, counters.p,
-                       st);
+            gjk_stage(tune.gjk_group, true);
             record(ev, 4, st);
             hits = static_cast<uint32_t>(read_counter(C_HITS));
             if (hits > g.cap) throw pk_error(PKG_DEVICE_ERROR, "GJK hit count changed between identical runs");
         }
         last_hits = hits;
-        record(ev, 9, st);
-        launch_epa(keys[nxt].p, hits, b, s, has_uv, tune, g, ovf_list.p, counters.p, st);
-        uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
-        if (novf) // rerun with larger stores; capacity never changes a result
+        // the EPA stage, with overflow reruns in larger stores (capacity never changes a
+        // result); it writes every hit's status and contact, so it too can run again
+        auto epa_stage = [&](const tuning &tu, bool again)
         {
+            if (again)
+                for (int c : {C_EPA_OVF, C_EPA_OVF2, C_EPA_BIG_FAIL}) reset_counter(c);
+            launch_epa(keys[nxt].p, hits, b, s, has_uv, tu, g, ovf_list.p, counters.p, st);
+            const uint32_t novf = static_cast<uint32_t>(read_counter(C_EPA_OVF));
+            if (!novf) return;
             // a small batch already ran with the LDS store: its overflows go straight to
             // the global-memory store
             const uint32_t *big_in = ovf_list.p;
             uint32_t n2 = novf;
-            if (!epa_first_tier_is_lds(hits, tune))
+            if (!epa_first_tier_is_lds(hits, tu))
             {
                 ovf2_list.reserve(novf);
                 launch_epa_lds(keys[nxt].p, ovf_list.p, novf, b, s, g, ovf2_list.p, counters.p, st);
@@ -747,8 +810,44 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 if (read_counter(C_EPA_BIG_FAIL))
                     throw pk_error(PKG_OVERFLOW_EPA, "EPA polytope exceeded 4096 faces");
             }
-        }
+        };
+        record(ev, 9, st);
+        epa_stage(tune, false);
         record(ev, 5, st);
+        if (trials)
+        {
+            auto timed = [&](auto &&run)
+            {
+                hip_check(hipEventRecord(trials->t0, st), "event record");
+                run();
+                hip_check(hipEventRecord(trials->t1, st), "event record");
+                hip_check(hipEventSynchronize(trials->t1), "event sync");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, trials->t0, trials->t1), "elapsed");
+                return static_cast<double>(ms);
+            };
+            static const int lanes[4] = {1, 2, 4, 8}, blocks[3] = {64, 128, 256};
+            static const uint32_t lds_max[3] = {0u, 512u, 4096u};
+            for (int c = 0; c < 4; ++c) trials->gjk[c] += timed([&] { gjk_stage(lanes[c], true); });
+            if (read_counter(C_HITS) != hits) throw pk_error(PKG_DEVICE_ERROR, "GJK hit count changed between runs");
+            tuning tu = tune;
+            tu.epa_lds_max = 0; // every batch through k_epa while its block size is timed
+            int best = 0;
+            double ms[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                tu.epa_block = blocks[c];
+                ms[c] = timed([&] { epa_stage(tu, true); });
+                trials->epa_block[c] += ms[c];
+                if (ms[c] < ms[best]) best = c;
+            }
+            tu.epa_block = blocks[best];
+            for (int c = 0; c < 3; ++c)
+            {
+                tu.epa_lds_max = lds_max[c];
+                trials->epa_lds[c] += timed([&] { epa_stage(tu, true); });
+            }
+        }
         // a manifold is non-empty only if its pair had contacts or has a hit now: the
         // data store needs at most hits + the previous step's slots
         const uint32_t dneed = hits + mused[cur];
@@ -1187,6 +1286,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->have_bodies = true;
         ctx->poisoned = false;
         ctx->tree_ok = false;
+        ctx->topo_ok = false;
         hip_check(hipStreamSynchronize(st), "body upload");
     })
 }
@@ -1220,6 +1320,7 @@ pkg_status pkg_set_worlds(pkg_ctx *ctx, uint32_t n_worlds, const uint32_t *first
         hip_check(hipStreamSynchronize(st), "world upload");
         ctx->n_worlds = n_worlds;
         ctx->tree_ok = false;
+        ctx->topo_ok = false;
         ctx->P[0] = ctx->P[1] = 0; // pairs restart under the new partition
         ctx->mused[0] = ctx->mused[1] = 0;
         ctx->cur = 0;
@@ -1628,6 +1729,7 @@ void tuning_out(const tuning &t, pkg_tuning *o)
     o->gjk_lanes = t.gjk_group;
     o->epa_block = t.epa_block;
     o->epa_lds_max = t.epa_lds_max;
+    o->tree_refits = t.tree_refits;
 }
 } // namespace
 
@@ -1648,63 +1750,64 @@ pkg_status pkg_set_tuning(pkg_ctx *ctx, const pkg_tuning *t)
         ctx->err = "pkg_set_tuning: gjk_lanes must be 1, 2, 4 or 8 and epa_block 64, 128 or 256";
         return PKG_INVALID_ARG;
     }
+    if (t->tree_refits > 64)
+    {
+        ctx->err = "pkg_set_tuning: tree_refits must be at most 64";
+        return PKG_INVALID_ARG;
+    }
     ctx->tune.gjk_group = g;
     ctx->tune.epa_block = e;
     ctx->tune.epa_lds_max = t->epa_lds_max;
+    ctx->tune.tree_refits = t->tree_refits;
     ctx->tuned = true;
     return PKG_OK;
 }
 
-pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps_per_candidate, pkg_tuning *out)
+pkg_status pkg_autotune(pkg_ctx *ctx, double dt, int steps, pkg_tuning *out)
 {
-    if (!ctx || !(dt >= 0.0) || steps_per_candidate < 1) return PKG_INVALID_ARG;
+    if (!ctx || !(dt >= 0.0) || steps < 1) return PKG_INVALID_ARG;
     if (!ctx->have_bodies) return PKG_INVALID_ARG;
     if (pkg_status p = refuse_if_poisoned(ctx)) return p;
     pkg_tuning res{};
     const pkg_status r = [&]() -> pkg_status
     {
         PKG_GUARD(ctx, {
-            hipEvent_t ev[12];
-            for (auto &e : ev) hip_check(hipEventCreate(&e), "event create");
-            // one real step with the stage's time read from the events
-            auto timed = [&](int stage)
+            pkg_ctx::shape_trials tr;
+            hip_check(hipEventCreate(&tr.t0), "event create");
+            hip_check(hipEventCreate(&tr.t1), "event create");
+            struct release
             {
-                ctx->step_once(dt, ev);
-                hip_check(hipEventSynchronize(ev[8]), "event sync");
-                float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, ev[kFrom[stage]], ev[kTo[stage]]), "elapsed");
-                return static_cast<double>(ms);
-            };
-            // candidates take turns, step by step, so the scene's drift weighs on them alike
-            auto pick = [&](auto set, int n, int stage, float *ms_out)
-            {
-                double acc[4] = {0, 0, 0, 0};
-                for (int k = 0; k < steps_per_candidate; ++k)
-                    for (int c = 0; c < n; ++c)
-                    {
-                        set(c);
-                        acc[c] += timed(stage);
-                    }
-                int best = 0;
-                for (int c = 0; c < n; ++c)
+                pkg_ctx *c;
+                pkg_ctx::shape_trials &t;
+                ~release()
                 {
-                    ms_out[c] = static_cast<float>(acc[c] / steps_per_candidate);
-                    if (acc[c] < acc[best]) best = c;
+                    c->trials = nullptr;
+                    (void)hipEventDestroy(t.t0);
+                    (void)hipEventDestroy(t.t1);
                 }
-                set(best);
+            } rel{ctx, tr};
+            ctx->trials = &tr;
+            for (int k = 0; k < steps; ++k) ctx->step_once(dt, nullptr);
+            auto argmin = [](const double *v, int n)
+            {
+                int b = 0;
+                for (int c = 1; c < n; ++c)
+                    if (v[c] < v[b]) b = c;
+                return b;
             };
-            const int lanes[4] = {1, 2, 4, 8};
-            const int blocks[3] = {64, 128, 256};
-            const uint32_t lds_max[3] = {0u, 512u, 4096u};
+            static const int lanes[4] = {1, 2, 4, 8}, blocks[3] = {64, 128, 256};
+            static const uint32_t lds_max[3] = {0u, 512u, 4096u};
             tuning &tu = ctx->tune;
-            pick([&](int c) { tu.gjk_group = lanes[c]; }, 4, 3, res.gjk_ms);
-            const uint32_t keep_lds = tu.epa_lds_max;
-            tu.epa_lds_max = 0; // every batch through k_epa while its block size is chosen
-            pick([&](int c) { tu.epa_block = blocks[c]; }, 3, 4, res.epa_block_ms);
-            tu.epa_lds_max = keep_lds;
-            pick([&](int c) { tu.epa_lds_max = lds_max[c]; }, 3, 4, res.epa_lds_ms);
+            tu.gjk_group = lanes[argmin(tr.gjk, 4)];
+            tu.epa_block = blocks[argmin(tr.epa_block, 3)];
+            tu.epa_lds_max = lds_max[argmin(tr.epa_lds, 3)];
+            for (int c = 0; c < 4; ++c) res.gjk_ms[c] = static_cast<float>(tr.gjk[c] / steps);
+            for (int c = 0; c < 3; ++c)
+            {
+                res.epa_block_ms[c] = static_cast<float>(tr.epa_block[c] / steps);
+                res.epa_lds_ms[c] = static_cast<float>(tr.epa_lds[c] / steps);
+            }
             ctx->tuned = true;
-            for (auto &e : ev) (void)hipEventDestroy(e);
         })
     }();
     if (r != PKG_OK)

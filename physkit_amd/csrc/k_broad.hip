@@ -675,38 +675,29 @@ __device__ __forceinline__ uint32_t lower_bound(const uint64_t *a, uint32_t n, u
 }
 
 // candidate e found in the previous (sorted) pair list: its manifold carries over
-__global__ void k_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,
-                        uint8_t *covered)
+__global__ void k_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src)
 {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
     uint64_t k = cand[e];
     uint32_t j = lower_bound(prev, n_prev, k);
-    uint32_t s = kNone;
-    if (j < n_prev && prev[j] == k)
-    {
-        s = j;
-        covered[j] = 1;
-    }
-    src[e] = s;
+    src[e] = (j < n_prev && prev[j] == k) ? j : kNone;
 }
 
-// previous pairs that no candidate covers stay when their leaves still overlap
-// (collision_phases.h:383-403)
 // A previous pair no candidate covers stays while its fat boxes overlap
 // (collision_phases.h:383-403). The candidates are every overlapping pair with a moved
 // leaf, so an uncovered pair with a moved body no longer overlaps; with neither body
 // moved, both fat boxes are the ones under which the pair was active last step (a fat
 // box changes only when its body moves, and an upload clears the pair list), so it
-// still overlaps. The overlap test thus reduces to the two moved flags.
-__global__ void k_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered, bodies_dev b,
-                            uint32_t *flag)
+// still overlaps. And a pair with neither body moved is never a candidate (each
+// candidate has a moved leaf), so the test reduces to the two moved flags.
+__global__ void k_prev_keep(const uint64_t *prev, uint32_t n_prev, bodies_dev b, uint32_t *flag)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_prev) return;
     uint64_t key = prev[p];
     uint32_t a = static_cast<uint32_t>(key >> 32), c = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-    flag[p] = (!covered[p] && !b.moved[a] && !b.moved[c]) ? 1u : 0u;
+    flag[p] = (!b.moved[a] && !b.moved[c]) ? 1u : 0u;
 }
 
 __global__ void k_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag, const uint32_t *pos,
@@ -766,6 +757,25 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
     }
 }
 
+// The last build's leaf order and hierarchy with this step's leaf boxes and moved flags:
+// the pair set and query answers do not depend on the tree's shape, only the walks' cost.
+void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, hipStream_t st)
+{
+    const uint32_t n = b.n;
+    t.n = n;
+    if (n == 0) return;
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    if (n >= 2)
+    {
+        k_refit<<<t.n_chunks, kRefitThreads, 0, st>>>(t);
+        if (t.n_chunks > 1)
+        {
+            if (t.span_levels > 1) k_span_table<<<1, kRefitThreads, 0, st>>>(t);
+            k_span<<<grid(n - 1), kBlock, 0, st>>>(t);
+        }
+    }
+}
+
 __global__ void k_pair_concat(const uint64_t *regions, uint32_t rcap, region_counts rc, uint64_t *dst)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -790,15 +800,14 @@ void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_key
 }
 
 void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,
-                       uint8_t *covered, hipStream_t st)
+                       hipStream_t st)
 {
-    if (m) k_match<<<grid(m), kBlock, 0, st>>>(cand, m, prev, n_prev, src, covered);
+    if (m) k_match<<<grid(m), kBlock, 0, st>>>(cand, m, prev, n_prev, src);
 }
 
-void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered, const bodies_dev &b,
-                      uint32_t *flag, hipStream_t st)
+void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const bodies_dev &b, uint32_t *flag, hipStream_t st)
 {
-    if (n_prev) k_prev_keep<<<grid(n_prev), kBlock, 0, st>>>(prev, n_prev, covered, b, flag);
+    if (n_prev) k_prev_keep<<<grid(n_prev), kBlock, 0, st>>>(prev, n_prev, b, flag);
 }
 
 void launch_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag, const uint32_t *pos,

@@ -107,6 +107,7 @@ void launch_integrate_vel(const bodies_dev &b, const shapes_dev &s, double gx, d
 void launch_integrate_pos(const bodies_dev &b, double dt, hipStream_t st);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+void launch_iota_u32(uint32_t *p, size_t n, hipStream_t st); // p[i] = i
 
 // ---------------- k_sort.hip ----------------
 // LSD radix sort of (key, value) pairs on 8-bit digits at the given bit shifts
@@ -176,6 +177,9 @@ inline uint32_t refit_levels(uint32_t n)
 }
 void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_scratch &sc,
                        unsigned long long *counters, hipStream_t st);
+// the last build's hierarchy refitted to the current leaf boxes (no Morton codes, sort
+// or topology); the same ids, node count and world layout as that build
+void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, hipStream_t st);
 // candidate keys (pairs with at least one moved leaf), appended per region (below);
 // a malformed tree is reported in counters[C_TREE_FAIL]
 // workgroup w writes its candidates to region r = w % n_regions: out_keys[r * rcap, ...)
@@ -189,12 +193,12 @@ struct region_counts
 };
 void launch_pair_concat(const uint64_t *regions, uint32_t rcap, const region_counts &rc, uint64_t *dst,
                         hipStream_t st);
-// src[e] = index of candidate e in the sorted previous list (kNone if new); marks it covered
+// src[e] = index of candidate e in the sorted previous list (kNone if new)
 void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev,
-                       uint32_t *src, uint8_t *covered, hipStream_t st);
-// flag previous pairs no candidate covers whose leaves still overlap, then compact them
-void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const uint8_t *covered,
-                      const bodies_dev &b, uint32_t *flag, hipStream_t st);
+                       uint32_t *src, hipStream_t st);
+// flag the previous pairs that stay without being candidates (neither body moved), then
+// compact them
+void launch_prev_keep(const uint64_t *prev, uint32_t n_prev, const bodies_dev &b, uint32_t *flag, hipStream_t st);
 void launch_prev_compact(const uint64_t *prev, uint32_t n_prev, const uint32_t *flag,
                          const uint32_t *pos, uint64_t *akeys, uint32_t *asrc, hipStream_t st);
 // writes list x (sorted) into its slots of the merge of x and y (sorted, disjoint)
@@ -252,6 +256,7 @@ struct tuning
     int epa_block = 64;          // threads per workgroup of k_epa (64, 128, 256)
     uint32_t epa_lds_max = 512;  // batches of at most this many GJK hits run one EPA solve per workgroup
                                  // with the LDS store (k_epa_lds); larger ones one per thread
+    uint32_t tree_refits = 7;    // steps that refit the last LBVH hierarchy between full builds
 };
 void launch_epa(const uint64_t *keys, uint32_t n_hits, const bodies_dev &b, const shapes_dev &s, bool uv,
                 const tuning &tu, gjk_out_dev &g, uint32_t *ovf_list, unsigned long long *counters, hipStream_t st);
