@@ -138,7 +138,7 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
-def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed):
+def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0):
     """Compulsory HBM bytes of each stage's launches in one step, with the step's own
     counts (n bodies, N dynamic, M moved leaves, K candidate pairs, P pairs, H GJK hits,
     C manifold contacts, R solver rows; s_r = 4 for FP32 rows, else 8):
@@ -146,9 +146,10 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed):
                         mass, force and torque accumulators, velocities, world inertia and
                         its inverse, pose, leaf box read; velocities and accumulators
                         written, the leaf box when it moves)
-      lbvh_build        253 n (+32 n with FP32 leaf boxes): leaf boxes read, Morton keys
-                        and ids written and read, leaf records written, 2 internal nodes of
-                        32 B written and read per leaf
+      lbvh_build        253 n (+32 n with FP32 leaf boxes) on a full build: leaf boxes
+                        read, Morton keys and ids written and read, leaf records written,
+                        2 internal nodes of 32 B written and read per leaf; 229 n on a refit
+                        (no keys); averaged over one build and tree_refits refits
       pair_update       48 M + 80 n + 36 K + 20 P: the moving leaves' boxes, the tree read
                         once, candidates written, sorted and matched, the previous list
                         read and the new one with its sources written
@@ -168,8 +169,8 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed):
     M, K, P, H, C, R = (c[k] for k in ("moved", "candidates", "pairs", "gjk_hits", "contacts", "rows"))
     out = {
         "integrate_vel": ("k_integrate_vel", 2 * n + 460 * nd + 48 * M + 48 * n_shapes),
-        "lbvh_build": ("LBVH build (k_morton, k_rs_*, k_build_internal, k_leaf_gather, k_refit, k_span)",
-                       (253 + (32 if mixed else 0)) * n),
+        "lbvh_build": ("LBVH build or refit (k_morton, k_rs_*, k_build_internal, k_leaf_gather, k_refit, k_span)",
+                       ((253 + 229 * tree_refits) / (1 + tree_refits) + (32 if mixed else 0)) * n),
         "pair_update": ("pair-set update (k_pair_query, sort, k_match, k_prev_*, k_merge)",
                         48 * M + 80 * n + 36 * K + 20 * P),
         "gjk": ("GJK stage (k_gjk_cull" + (", k_gjk32" if mixed else "") + ", k_gjk)",
@@ -577,7 +578,8 @@ def main():
     wall, stage, c, pairs = r["wall"], r["stage"], r["counters"], r["pairs"]
     print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
-    models = stage_bytes(s, c, pairs, hits_frac, r["pos"], s["iterations"], mixed or prec == "fp32-solve")
+    models = stage_bytes(s, c, pairs, hits_frac, r["pos"], s["iterations"], mixed or prec == "fp32-solve",
+                         args.tree_refits)
     dom, table = roofline_of(stage, args.steps, models)
     d = table[dom]
     roof = {"bound": "hbm", "stage": dom, "kernel": d["kernel"], "achieved": d["achieved_GBs"],
