@@ -856,9 +856,9 @@ __device__ __forceinline__ uint32_t lower_u32(const uint32_t *a, uint32_t n, uin
 // first position in the sorted list, and the velocities are written back at the end.
 constexpr uint32_t kLdsManifolds = 512;
 constexpr uint32_t kLdsIds = 2 * kLdsManifolds;
-__device__ __forceinline__ uint16_t slot_of(const uint32_t *ids, uint32_t id)
+__device__ __forceinline__ uint16_t slot_of(const uint32_t *ids, uint32_t n_ids, uint32_t id)
 {
-    uint32_t lo = 0, hi = kLdsIds;
+    uint32_t lo = 0, hi = n_ids;
     while (lo < hi)
     {
         uint32_t mid = (lo + hi) >> 1;
@@ -1091,9 +1091,13 @@ __global__ void __launch_bounds__(kIslandThreads)
     const uint64_t cmask = (1ull << cbits) - 1;
     const uint32_t m = e0 - s0;
     const bool use_lds = m <= kLdsManifolds;
+    // the sorted id list spans the smallest power of two holding both bodies of every
+    // manifold (sentinels sort last)
+    uint32_t n_ids = 2;
+    while (n_ids < 2 * m) n_ids <<= 1;
     if (use_lds)
     {
-        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        for (uint32_t i = threadIdx.x; i < n_ids; i += kIslandThreads)
         {
             uint32_t id = 0xFFFFFFFFu;
             if (i < 2 * m)
@@ -1105,10 +1109,10 @@ __global__ void __launch_bounds__(kIslandThreads)
             ids[i] = id;
         }
         __syncthreads();
-        for (uint32_t size = 2; size <= kLdsIds; size <<= 1) // bitonic sort
+        for (uint32_t size = 2; size <= n_ids; size <<= 1) // bitonic sort
             for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
             {
-                for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+                for (uint32_t i = threadIdx.x; i < n_ids; i += kIslandThreads)
                 {
                     uint32_t j = i ^ stride;
                     if (j > i)
@@ -1124,7 +1128,7 @@ __global__ void __launch_bounds__(kIslandThreads)
                 }
                 __syncthreads();
             }
-        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        for (uint32_t i = threadIdx.x; i < n_ids; i += kIslandThreads)
         {
             uint32_t id = ids[i];
             if (id != 0xFFFFFFFFu && (i == 0 || ids[i - 1] != id))
@@ -1142,8 +1146,8 @@ __global__ void __launch_bounds__(kIslandThreads)
         {
             uint64_t key = keys[order[s0 + i]];
             uint32_t x = static_cast<uint32_t>(key >> 32), y = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-            sl[0][i] = b.is_static[x] ? kNoSlot : slot_of(ids, x);
-            sl[1][i] = b.is_static[y] ? kNoSlot : slot_of(ids, y);
+            sl[0][i] = b.is_static[x] ? kNoSlot : slot_of(ids, n_ids, x);
+            sl[1][i] = b.is_static[y] ? kNoSlot : slot_of(ids, n_ids, y);
         }
         __syncthreads();
     }
@@ -1176,7 +1180,7 @@ __global__ void __launch_bounds__(kIslandThreads)
         }
     }
     if (use_lds)
-        for (uint32_t i = threadIdx.x; i < kLdsIds; i += kIslandThreads)
+        for (uint32_t i = threadIdx.x; i < n_ids; i += kIslandThreads)
         {
             uint32_t id = ids[i];
             if (id != 0xFFFFFFFFu && (i == 0 || ids[i - 1] != id))
