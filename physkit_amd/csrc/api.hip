@@ -967,7 +967,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
                 write_rows(is.ord, n_act, rs, scap);
                 record(ev, 10, st);
                 launch_solve_islands(keys[nxt].p, is, istarts.p, n_isl, n_act, c, b, rs, scap,
-                                     cfg.solver_iterations, f32, st);
+                                     cfg.solver_iterations, static_cast<uint32_t>(h_counters[C_ISL_MAX]), f32, st);
                 record(ev, 11, st);
                 if (f32)
                     launch_impulse_writeback32(is.ord, n_act, MF(nxt), rs, scap, st);

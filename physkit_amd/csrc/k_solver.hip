@@ -831,13 +831,13 @@ __global__ void k_count_below(const uint64_t *sk, uint32_t n, uint64_t last, uin
 }
 
 // One workgroup per chunk of whole islands: chunk k holds the islands whose first
-// position lies in [k C, (k+1) C) of the island-sorted order. Warm start then `iters`
+// position lies in [k W, (k+1) W) of the island-sorted order. Warm start then `iters`
 // passes; each pass steps through the colours with a workgroup barrier between them,
 // every thread solving its positions of the current colour. Manifolds of one colour
 // share no dynamic body, and the islands of a chunk share none at all, so this is the
-// colour-ordered solve restricted to the chunk.
-constexpr int kIslandThreads = 256;
-constexpr uint32_t kChunkManifolds = 256;
+// colour-ordered solve restricted to the chunk. A chunk holds at most W + L - 1
+// manifolds for islands of at most L, so W = CAP + 1 - L keeps every chunk within the
+// workgroup's CAP threads (one manifold each, register-resident) and its LDS.
 __device__ __forceinline__ uint32_t lower_u32(const uint32_t *a, uint32_t n, uint32_t key)
 {
     uint32_t lo = 0, hi = n;
@@ -851,11 +851,9 @@ __device__ __forceinline__ uint32_t lower_u32(const uint32_t *a, uint32_t n, uin
     }
     return lo;
 }
-// Chunks of at most kLdsManifolds manifolds keep the velocities of their dynamic bodies
-// in LDS for all passes: body ids are sorted in LDS (bitonic), a body's slot is its
-// first position in the sorted list, and the velocities are written back at the end.
-constexpr uint32_t kLdsManifolds = 512;
-constexpr uint32_t kLdsIds = 2 * kLdsManifolds;
+// Chunks of at most CAP manifolds keep the velocities of their dynamic bodies in LDS
+// for all passes: body ids are sorted in LDS (bitonic), a body's slot is its first
+// position in the sorted list, and the velocities are written back at the end.
 __device__ __forceinline__ uint16_t slot_of(const uint32_t *ids, uint32_t n_ids, uint32_t id)
 {
     uint32_t lo = 0, hi = n_ids;
@@ -999,12 +997,12 @@ __device__ __forceinline__ void ctc_solve(ctc_t<float> &k, bool warm, body32 &A,
 __device__ __forceinline__ void body_consts(const bodies_dev &b, uint32_t i, body_state &s) { s = ld_body(b, i); }
 __device__ __forceinline__ void body_consts(const bodies_dev &b, uint32_t i, body32 &s) { s = ld_body32(b, i); }
 
-// the whole solve of one chunk, one manifold per thread (m <= kIslandThreads)
-template <bool F32>
+// the whole solve of one chunk, one manifold per thread (m <= CAP)
+template <bool F32, int CAP>
 __device__ __forceinline__ void solve_chunk_reg(const uint64_t *keys, const uint32_t *order, const uint64_t *sk,
                                                 uint32_t s0, uint32_t m, uint64_t cmask, uint32_t n_colours,
                                                 const bodies_dev &b, const rows_dev &rd, uint32_t cap, uint32_t iters,
-                                                const vel_lds &L, const uint16_t (*sl)[kLdsManifolds])
+                                                const vel_lds &L, const uint16_t (*sl)[CAP])
 {
     using T = typename std::conditional<F32, float, double>::type;
     using B = typename std::conditional<F32, body32, body_state>::type;
@@ -1074,23 +1072,31 @@ __device__ __forceinline__ void solve_chunk_reg(const uint64_t *keys, const uint
             if (valid[q]) ctc_store_acc(k[q], r, static_cast<uint32_t>(q) * cap + t);
 }
 
-template <bool F32>
-__global__ void __launch_bounds__(kIslandThreads)
+// kIslandThreads threads. A chunk of at most kIslandThreads manifolds is solved one
+// manifold per thread in registers, one of at most CAP manifolds with the velocities in
+// LDS and each manifold's rows read per pass, a larger one (an island of more than CAP
+// manifolds) from global memory. CAP 256 (every island at most 256 manifolds, windows
+// sized to match) halves the LDS, and its FP32 form is held to two waves per SIMD, so
+// two workgroups share a CU; CAP 512 keeps every register for its long islands.
+constexpr int kIslandThreads = 256;
+template <bool F32, int CAP>
+__global__ void __launch_bounds__(kIslandThreads, (F32 && CAP == 256) ? 2 : 1)
     k_solve_islands(const uint64_t *keys, const uint32_t *order, const uint64_t *sk, const uint32_t *starts,
                     uint32_t n_isl, uint32_t n_act, uint32_t n_colours, int cbits, bodies_dev b, rows_dev r,
-                    uint32_t cap, uint32_t iters)
+                    uint32_t cap, uint32_t iters, uint32_t window)
 {
+    constexpr uint32_t kLdsIds = 2 * CAP;
     __shared__ uint32_t ids[kLdsIds];
     __shared__ double lv[kLdsIds][6];
-    __shared__ uint16_t sl[2][kLdsManifolds];
+    __shared__ uint16_t sl[2][CAP];
     const uint32_t k = blockIdx.x;
-    const uint32_t lo = lower_u32(starts, n_isl, k * kChunkManifolds);
-    const uint32_t hi = lower_u32(starts, n_isl, (k + 1) * kChunkManifolds);
+    const uint32_t lo = lower_u32(starts, n_isl, k * window);
+    const uint32_t hi = lower_u32(starts, n_isl, (k + 1) * window);
     if (lo >= hi) return; // no island starts in this chunk
     const uint32_t s0 = starts[lo], e0 = hi < n_isl ? starts[hi] : n_act;
     const uint64_t cmask = (1ull << cbits) - 1;
     const uint32_t m = e0 - s0;
-    const bool use_lds = m <= kLdsManifolds;
+    const bool use_lds = m <= static_cast<uint32_t>(CAP);
     // the sorted id list spans the smallest power of two holding both bodies of every
     // manifold (sentinels sort last)
     uint32_t n_ids = 2;
@@ -1152,8 +1158,8 @@ __global__ void __launch_bounds__(kIslandThreads)
         __syncthreads();
     }
     vel_lds L{lv};
-    if (use_lds && m <= kIslandThreads)
-        solve_chunk_reg<F32>(keys, order, sk, s0, m, cmask, n_colours, b, r, cap, iters, L, sl);
+    if (use_lds && m <= static_cast<uint32_t>(kIslandThreads))
+        solve_chunk_reg<F32, CAP>(keys, order, sk, s0, m, cmask, n_colours, b, r, cap, iters, L, sl);
     else
     for (uint32_t pass = 0; pass <= iters; ++pass)
     {
@@ -1300,18 +1306,27 @@ void launch_island_maxlen(const uint32_t *starts, const uint32_t *totals, uint32
 }
 void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
                           uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
-                          uint32_t iters, bool f32, hipStream_t st)
+                          uint32_t iters, uint32_t longest, bool f32, hipStream_t st)
 {
     if (!n_isl) return;
-    const unsigned nb = static_cast<unsigned>((n_act + kChunkManifolds - 1) / kChunkManifolds);
-    if (f32)
-        k_solve_islands<true><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act, n_colours,
-                                                              is.cbits, b, r, cap, iters);
+    // the smaller workgroup (and LDS) when every island fits it, so more chunks are
+    // resident at once; windows shrink with the longest island so every chunk fits
+    auto go = [&](auto capacity, uint32_t window)
+    {
+        constexpr int CAP = decltype(capacity)::value;
+        const unsigned nb = static_cast<unsigned>((n_act + window - 1) / window);
+        if (f32)
+            k_solve_islands<true, CAP><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act,
+                                                                       n_colours, is.cbits, b, r, cap, iters, window);
+        else
+            k_solve_islands<false, CAP><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act,
+                                                                        n_colours, is.cbits, b, r, cap, iters, window);
+    };
+    if (longest <= static_cast<uint32_t>(kIslandThreads))
+        go(std::integral_constant<int, kIslandThreads>{}, kIslandThreads + 1 - std::max<uint32_t>(longest, 1));
     else
-        k_solve_islands<false><<<nb, kIslandThreads, 0, st>>>(keys, is.ord, is.sk, starts, n_isl, n_act, n_colours,
-                                                               is.cbits, b, r, cap, iters);
+        go(std::integral_constant<int, 2 * kIslandThreads>{}, kIslandThreads);
 }
-
 void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                            const bodies_dev &b, rows_dev r32, uint32_t cap, bool warm, hipStream_t st)
 {

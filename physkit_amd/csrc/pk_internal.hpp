@@ -364,9 +364,10 @@ void launch_island_segments(const island_sort &is, uint32_t n_pairs, uint32_t *f
                             uint32_t *total, uint32_t *starts, uint32_t *n_act, hipStream_t st);
 void launch_island_maxlen(const uint32_t *starts, const uint32_t *totals, uint32_t max_islands,
                           unsigned long long *mx, hipStream_t st);
+// longest: manifolds of the longest island (chunks are sized so each fits one workgroup)
 void launch_solve_islands(const uint64_t *keys, const island_sort &is, const uint32_t *starts, uint32_t n_isl,
                           uint32_t n_act, uint32_t n_colours, const bodies_dev &b, const rows_dev &r, uint32_t cap,
-                          uint32_t iters, bool f32, hipStream_t st);
+                          uint32_t iters, uint32_t longest, bool f32, hipStream_t st);
 // FP32 solve (rows_dev whose d holds FP32 planes, written by launch_contact_setup with
 // f32; body velocities accumulate in FP64)
 void launch_solve_colour32(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
