@@ -212,6 +212,7 @@ struct pkg_ctx
     dbuf<int32_t> mcount[2];
     dbuf<uint32_t> mslot[2];
     dbuf<double> mdata[2];
+    dbuf<uint32_t> mowner[2]; // the pair of each data slot
     uint32_t mcap[2] = {0, 0};   // pairs the count and slot arrays hold
     uint32_t mdcap[2] = {0, 0};  // manifold data slots
     uint32_t mused[2] = {0, 0};  // slots in use after the step that wrote the set
@@ -390,6 +391,7 @@ struct pkg_ctx
         m.slot = mslot[k].p;
         m.dcap = mdcap[k];
         m.data = mdata[k].p;
+        m.owner = mowner[k].p;
         return m;
     }
 
@@ -856,6 +858,8 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             const uint32_t c = std::max<uint32_t>(dneed + dneed / 4, 1024);
             mdata[nxt].release();
             mdata[nxt].reserve(static_cast<size_t>(c) * 4 * kMF);
+            mowner[nxt].release();
+            mowner[nxt].reserve(c);
             mdcap[nxt] = c;
         }
         launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p,
@@ -889,11 +893,10 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         {
             colour.reserve(ncap);
             claim.reserve(std::max<uint32_t>(n, 1));
-            launch_row_flags(keys[nxt].p, Pn, b, MF(nxt), colour.p, counters.p, st);
-            // the manifolds with rows are listed before the host looks at the row count,
-            // so one copy returns both (C_NCOL and C_NLIST are zero from the step's start)
-            clist.reserve(Pn);
-            launch_colour_list(colour.p, Pn, clist.p, counters.p, st);
+            clist.reserve(std::max<uint32_t>(mdcap[nxt], 1));
+            // the manifolds with rows are listed as they are flagged, so one copy returns
+            // the row and list counts (C_NCOL and C_NLIST are zero from the step's start)
+            launch_row_flags(keys[nxt].p, Pn, b, MF(nxt), colour.p, clist.p, counters.p, st);
             read_counters(C_ROWS, C_NLIST);
             if (h_counters[C_ROWS] > 0)
             {

@@ -552,6 +552,7 @@ __global__ void __launch_bounds__(kBlock)
     if (live)
     {
         next.slot[p] = count > 0 ? static_cast<uint32_t>(sl) : kNone;
+        if (count > 0) next.owner[sl] = p;
         for (int i = 0; i < nm.n; ++i) st_contact(next, static_cast<uint32_t>(sl), i, nm.c[i]);
     }
     block_add(&counters[C_CONTACTS], count);

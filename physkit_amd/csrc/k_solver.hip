@@ -218,31 +218,31 @@ __device__ __forceinline__ bool contact_frame(const manifolds_dev &m, uint32_t s
 // and the row count; the rows themselves are written by k_setup once the solve order
 // is known
 __global__ void __launch_bounds__(kBlock)
-    k_row_flags(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, manifolds_dev m, uint32_t *colour,
+    k_row_flags(const uint64_t *keys, bodies_dev b, manifolds_dev m, uint32_t *colour, uint32_t *list,
                 unsigned long long *counters)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nrows = 0;
-    if (p < n_pairs)
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_slots = static_cast<uint32_t>(counters[C_MSLOTS]);
+    uint32_t nrows = 0, p = 0;
+    if (s < n_slots && s < m.dcap)
     {
+        p = m.owner[s];
         const int cnt = m.count[p];
-        if (cnt > 0)
+        const uint64_t key = keys[p];
+        const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+        const v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
+        const qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+        for (int c = 0; c < cnt; ++c)
         {
-            const uint64_t key = keys[p];
-            const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-            const v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
-            const qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
-            const uint32_t sl = m.slot[p];
-            for (int c = 0; c < cnt; ++c)
-            {
-                v3 n, ra, rb;
-                double pen;
-                if (contact_frame(m, sl, c, pa, pb, qa, qb, n, ra, rb, pen)) ++nrows;
-            }
+            v3 n, ra, rb;
+            double pen;
+            if (contact_frame(m, s, c, pa, pb, qa, qb, n, ra, rb, pen)) ++nrows;
         }
-        colour[p] = nrows ? kUncoloured : kNone;
+        if (nrows) colour[p] = kUncoloured;
     }
     block_add(&counters[C_ROWS], nrows);
+    const unsigned long long at = block_append(&counters[C_NLIST], nrows > 0);
+    if (nrows) list[at] = p;
 }
 
 // Rows of manifold ord[k] in solve order: contact c at slot c * cap + k of every row
@@ -403,14 +403,6 @@ __global__ void k_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, 
 }
 
 // the row manifolds (colour kUncoloured after contact setup), listed for the rounds
-__global__ void k_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters)
-{
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    bool rows = p < n_pairs && colour[p] == kUncoloured;
-    unsigned long long slot = block_append(&counters[C_NLIST], rows);
-    if (rows) list[slot] = p;
-}
-
 // A manifold holding the claim at each of its dynamic bodies takes the smallest colour
 // that no coloured manifold at those bodies holds (`used`: colours 0-63 taken at each
 // body), or 64 + the round number when all 64 are taken. Manifolds coloured in one
@@ -1199,9 +1191,11 @@ __global__ void __launch_bounds__(kIslandThreads, (F32 && CAP == 256) ? 2 : 1)
 } // namespace
 
 void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, manifolds_dev m, uint32_t *colour,
-                      unsigned long long *counters, hipStream_t st)
+                      uint32_t *list, unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs) k_row_flags<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, m, colour, counters);
+    if (!n_pairs) return;
+    launch_fill_u32(colour, kNone, n_pairs, st);
+    if (m.dcap) k_row_flags<<<grid(m.dcap), kBlock, 0, st>>>(keys, b, m, colour, list, counters);
 }
 void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
                           manifolds_dev m, rows_dev r, double dt, double threshold, bool f32, hipStream_t st)
@@ -1216,11 +1210,6 @@ void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *wo
                       hipStream_t st)
 {
     if (n_pairs) k_pair_base<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, world_of, pair_base);
-}
-void launch_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters,
-                        hipStream_t st)
-{
-    if (n_pairs) k_colour_list<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, list, counters);
 }
 void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
                          const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,

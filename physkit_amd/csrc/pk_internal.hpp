@@ -63,6 +63,7 @@ struct manifolds_dev
     uint32_t *slot;
     uint32_t dcap; // slots
     double *data;
+    uint32_t *owner; // dcap: the pair whose contacts a slot holds (written by the manifold update)
 };
 constexpr int kMF = 13; // normal3 local_a3 local_b3 depth ln lt1 lt2
 constexpr int kSimplexWords = 17; // GJK simplex hand-off per hit: count + 4 x (p.xyz, packed features)
@@ -304,10 +305,13 @@ struct rows_dev
 };
 constexpr int kRowF = 44; // 38 row words + the contact's lever arms (6)
 constexpr uint32_t kUncoloured = 0xFFFFFFFEu; // colour of an item still to be coloured
-// colour[p] = kUncoloured for manifolds that make solver rows (else kNone); their row
-// count at counters[C_ROWS]
+// colour[p] = kUncoloured for manifolds that make solver rows (else kNone), their row
+// count at counters[C_ROWS], and those manifolds listed in list (count in C_NLIST).
+// Only pairs holding contacts can make rows: the pass runs over the data slots the
+// manifold update filled (counters[C_MSLOTS] of at most m.dcap) after every colour has
+// been set to kNone.
 void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, manifolds_dev m, uint32_t *colour,
-                      unsigned long long *counters, hipStream_t st);
+                      uint32_t *list, unsigned long long *counters, hipStream_t st);
 // rows of manifold ord[k], k < n, at slots c * cap + k of r (solve order); with ord
 // null, of manifold k at slots 4k + c (pair order). f32: r.d holds FP32 planes (the
 // rows computed in FP64 and rounded once)
@@ -322,8 +326,6 @@ void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *wo
 // once before the first round; `used` (colours 0-63 taken at each body) too; each round
 // the claim holders take the smallest colour free at their dynamic bodies, and C_NCOL
 // ends as the colour count.
-void launch_colour_list(const uint32_t *colour, uint32_t n_pairs, uint32_t *list, unsigned long long *counters,
-                        hipStream_t st); // the row manifolds; count in C_NLIST
 void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
                          const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                          unsigned long long *claim, uint32_t round, hipStream_t st);
