@@ -753,7 +753,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     if (Pn)
     {
         gjk_list.reserve(Pn);
-        cull_rec.reserve(20 * static_cast<size_t>(n));
+        cull_rec.reserve(22 * static_cast<size_t>(n)); // FP64 records, then FP32 spheres (mixed form)
         gjk32_in f32{};
         shapes32_dev s32 = S32();
         if (cfg.flags & PKG_FLAG_FP32_GJK)

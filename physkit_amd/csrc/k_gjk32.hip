@@ -70,25 +70,30 @@ __global__ void __launch_bounds__(kBlock)
             const int32_t *prev_count, uint8_t *status, uint32_t *list2, unsigned long long *counters)
 {
     const uint32_t n_list = static_cast<uint32_t>(counters[C_GJK_LIST]);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool pass = false;
-    uint32_t p = 0;
-    if (i < n_list)
+    const uint32_t threads = gridDim.x * blockDim.x;
+    // the same number of rounds on every lane (each round's append is workgroup-wide)
+    for (uint32_t base = 0; base < n_list; base += threads)
     {
-        p = list[i];
-        // a pair in contact after the previous step is most likely still touching: it goes
-        // to the FP64 GJK without an attempt
-        const uint32_t sp = src ? src[p] : kNone;
-        const bool touched = sp != kNone && prev_count[sp] > 0;
-        const uint64_t key = keys[p];
-        if (!touched && certify_apart<UV>(b, s, static_cast<uint32_t>(key >> 32),
-                                          static_cast<uint32_t>(key & 0xFFFFFFFFu)))
-            status[p] = 0;
-        else
-            pass = true;
+        const uint32_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+        bool pass = false;
+        uint32_t p = 0;
+        if (i < n_list)
+        {
+            p = list[i];
+            // a pair in contact after the previous step is most likely still touching: it
+            // goes to the FP64 GJK without an attempt
+            const uint32_t sp = src ? src[p] : kNone;
+            const bool touched = sp != kNone && prev_count[sp] > 0;
+            const uint64_t key = keys[p];
+            if (!touched && certify_apart<UV>(b, s, static_cast<uint32_t>(key >> 32),
+                                              static_cast<uint32_t>(key & 0xFFFFFFFFu)))
+                status[p] = 0;
+            else
+                pass = true;
+        }
+        const unsigned long long slot = block_append(&counters[C_GJK_LIST2], pass);
+        if (pass) list2[slot] = p;
     }
-    const unsigned long long slot = block_append(&counters[C_GJK_LIST2], pass);
-    if (pass) list2[slot] = p;
 }
 
 __global__ void k_verts32(const uint32_t *off, const uint32_t *voff, const double *verts, uint32_t n_shapes,
@@ -115,7 +120,7 @@ void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, 
                   uint32_t *list2, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    const unsigned gr = static_cast<unsigned>((n_pairs + kBlock - 1) / kBlock);
+    const unsigned gr = std::min<unsigned>(static_cast<unsigned>((n_pairs + kBlock - 1) / kBlock), kListGrid);
     if (uv)
         k_gjk32<true><<<gr, kBlock, 0, st>>>(keys, list, b, s, src, prev_count, status, list2, counters);
     else

@@ -171,10 +171,34 @@ __device__ __forceinline__ bool bodies_apart(const cull_body &A, const cull_body
     return obbs_apart(A, B, m);
 }
 
-__global__ void __launch_bounds__(kBlock) k_cull_prep(bodies_dev b, shapes_dev s, double *rec)
+// FP32 sphere records for the mixed form: (position rounded to nearest, radius rounded
+// up), 16 bytes per body; infinite radius for a body the FP64 test never culls
+__global__ void __launch_bounds__(kBlock) k_cull_prep(bodies_dev b, shapes_dev s, double *rec, float4 *rec32)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < b.n) put_cull(rec + static_cast<size_t>(kCullRec) * i, cull_of(b, s, i));
+    if (i >= b.n) return;
+    const cull_body o = cull_of(b, s, i);
+    put_cull(rec + static_cast<size_t>(kCullRec) * i, o);
+    if (rec32)
+    {
+        const bool unit = fabs(o.qn - 1.0) <= 1e-9;
+        float r = static_cast<float>(o.r);
+        if (static_cast<double>(r) < o.r) r = nextafterf(r, __builtin_inff());
+        rec32[i] = make_float4(static_cast<float>(o.pos.x), static_cast<float>(o.pos.y), static_cast<float>(o.pos.z),
+                               unit ? r : __builtin_inff());
+    }
+}
+// The FP32 spheres are certainly apart in the FP64 test's sense: with FP32 rounding of
+// the positions (relative 2^-24 each) and of the distance (a few 2^-24), the FP64
+// distance is at least d (1 - 1e-5) - 4e-7 (|pA|_1 + |pB|_1); with radii rounded up, the
+// FP64 threshold r + 1e-6 r + 1e-6 is at most (rA + rB)(1 + 1e-5) + 2e-6. Infinite or
+// NaN values never pass.
+__device__ __forceinline__ bool spheres32_apart(float4 a, float4 b)
+{
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float mag = ((fabsf(a.x) + fabsf(a.y)) + (fabsf(a.z) + fabsf(b.x))) + (fabsf(b.y) + fabsf(b.z));
+    return d * (1.0f - 1e-5f) - 4e-7f * mag > (a.w + b.w) * (1.0f + 1e-5f) + 2e-6f;
 }
 
 // GJK simplex hand-off, planar: slot 0 = point count, then per point p.xyz and the
@@ -219,8 +243,8 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
 // without GJK; the others are listed for k_gjk
 template <bool REC>
 __global__ void __launch_bounds__(kBlock)
-    k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec, uint8_t *status,
-               uint32_t *list, unsigned long long *counters)
+    k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec,
+               const float4 *rec32, uint8_t *status, uint32_t *list, unsigned long long *counters)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool keep = false;
@@ -229,7 +253,8 @@ __global__ void __launch_bounds__(kBlock)
         const uint64_t key = keys[p];
         const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
         if (REC)
-            keep = !records_apart(rec + static_cast<size_t>(kCullRec) * ia, rec + static_cast<size_t>(kCullRec) * ib);
+            keep = !((rec32 && spheres32_apart(rec32[ia], rec32[ib])) ||
+                     records_apart(rec + static_cast<size_t>(kCullRec) * ia, rec + static_cast<size_t>(kCullRec) * ib));
         else
             keep = !bodies_apart(cull_of(b, s, ia), cull_of(b, s, ib));
         if (!keep) status[p] = 0;
@@ -249,33 +274,39 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
           unsigned long long *counters)
 {
     const uint32_t n_list = static_cast<uint32_t>(counters[list_slot]);
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t i = t / G;
-    int gl = static_cast<int>(t % G);
-    bool hit = false;
-    simplex x;
-    x.n = 0;
-    uint32_t p = 0;
-    if (i < n_list)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gl = static_cast<int>(t % G);
+    // groups step through the list together: the same number of rounds on every lane,
+    // so each round's workgroup-wide append sees all of them
+    const uint32_t per_round = gridDim.x * blockDim.x / G;
+    for (uint32_t base = 0; base < n_list; base += per_round)
     {
-        p = list[i];
-        uint64_t key = keys[p];
-        gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
-        gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
-        hit = gjk<G, UV>(A, B, x, gl);
-        if (gl == 0) g.status[p] = 0;
-    }
-    bool lead = hit && gl == 0;
-    unsigned long long slot = block_append(&counters[C_HITS], lead);
-    if (lead)
-    {
-        g.hit_of[p] = static_cast<uint32_t>(slot);
-        // past the hit list's capacity the hit is only counted: the host sees the count,
-        // grows the list and runs the GJK stage again
-        if (slot < g.cap)
+        const uint32_t i = base + t / G;
+        bool hit = false;
+        simplex x;
+        x.n = 0;
+        uint32_t p = 0;
+        if (i < n_list)
         {
-            g.hit_list[slot] = p;
-            save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
+            p = list[i];
+            uint64_t key = keys[p];
+            gshape A = body_shape<UV>(b, s, static_cast<uint32_t>(key >> 32));
+            gshape B = body_shape<UV>(b, s, static_cast<uint32_t>(key & 0xFFFFFFFFu));
+            hit = gjk<G, UV>(A, B, x, gl);
+            if (gl == 0) g.status[p] = 0;
+        }
+        bool lead = hit && gl == 0;
+        unsigned long long slot = block_append(&counters[C_HITS], lead);
+        if (lead)
+        {
+            g.hit_of[p] = static_cast<uint32_t>(slot);
+            // past the hit list's capacity the hit is only counted: the host sees the
+            // count, grows the list and runs the GJK stage again
+            if (slot < g.cap)
+            {
+                g.hit_list[slot] = p;
+                save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
+            }
         }
     }
 }
@@ -694,11 +725,15 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
     // batched worlds, 0.6, read the planar arrays faster)
     if (static_cast<size_t>(n_pairs) >= 2 * static_cast<size_t>(b.n))
     {
-        k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec);
-        k_gjk_cull<true><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, g.status, list, counters);
+        // the mixed form adds the FP32 sphere records after the FP64 ones (16-byte aligned)
+        float4 *rec32 = f32.s32 ? reinterpret_cast<float4 *>(cull_rec + static_cast<size_t>(kCullRec) * b.n) : nullptr;
+        k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec, rec32);
+        k_gjk_cull<true><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, rec32, g.status, list,
+                                                           counters);
     }
     else
-        k_gjk_cull<false><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, g.status, list, counters);
+        k_gjk_cull<false><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, nullptr, g.status, list,
+                                                            counters);
     int slot = C_GJK_LIST;
     if (f32.s32)
     {
@@ -706,8 +741,9 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
         list = f32.list2;
         slot = C_GJK_LIST2;
     }
-    // sized for every pair; groups past the listed count (read on the device) exit at once
-    unsigned gr = grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
+    // sized for every pair up to kListGrid workgroups, which then step through the list
+    // (its count is read on the device); groups past the listed count exit at once
+    const unsigned gr = std::min<unsigned>(grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group)), kListGrid);
     auto go = [&](auto lanes)
     {
         constexpr int G = decltype(lanes)::value;

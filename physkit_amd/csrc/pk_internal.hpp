@@ -236,10 +236,14 @@ void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, 
                   uint32_t *list2, unsigned long long *counters, hipStream_t st);
 // GJK over every pair with `group` (2 or 4) lanes per pair; pairs whose bounding
 // spheres are clearly apart are written separated first and the rest listed in `list`
-// (capacity n_pairs), so GJK wavefronts run only pairs that need it; cull_rec: 20
+// (capacity n_pairs), so GJK wavefronts run only pairs that need it; cull_rec: 22
 // doubles per body of scratch for the per-body cull records. With s32 (the mixed-
-// precision form), the FP32 pass above runs between the cull and the FP64 GJK, which
+// precision form), the cull first tests FP32 spheres (certainly apart or decided by the
+// FP64 records), and the FP32 pass above runs between the cull and the FP64 GJK, which
 // then takes list2 (capacity n_pairs).
+// list kernels (GJK passes over the cull's list, whose count only the device knows)
+// launch at most this many workgroups, which step through longer lists
+constexpr unsigned kListGrid = 16384;
 struct gjk32_in
 {
     const shapes32_dev *s32; // null: no FP32 pass

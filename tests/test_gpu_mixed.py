@@ -65,6 +65,14 @@ def test_fp32_filters_bitexact_hulls(pk, oracle):
     assert t["hits"] > 0
 
 
+def test_fp32_filters_bitexact_granular(pk, oracle):
+    """The granular pile (about 100 pairs per body: the cull reads per-body records and,
+    in the mixed form, tests FP32 spheres first) with the FP32 filters: bit-exact."""
+    s = pk.scene(4, None, 343)
+    t = _run(pk, oracle, 4, 343, 60, flags=s["flags"] | pk.FLAG_FP32_GJK | pk.FLAG_FP32_BROAD)
+    assert t["pairs"] > 2 * 343 * 60
+
+
 def test_fp32_filters_bitexact_boxes(pk, oracle):
     t = _run(pk, oracle, 0, 256, 60, flags=pk.FLAG_FP32_GJK | pk.FLAG_FP32_BROAD)
     assert t["rows"] > 0
