@@ -83,7 +83,8 @@ def shape_radii(s):
 def algorithmic_bytes(stage, s, pairs, hits_frac, pos=None):
     """Compulsory HBM bytes of one launch of the narrowphase stage (FP64).
     GJK stage: a pair whose bounding spheres are apart (the cull's first test) needs its
-    key (8 B), both bodies' positions and radii (2 x 32) and its status byte. Any other
+    key (8 B) and its status byte; the positions and radii of the bodies in such pairs
+    (32 B each) are read once per body, however many pairs share it. Any other
     pair also reads each body's quaternion (32), shape id and vertex offsets (12) and the
     support vertices of both shapes once (24 B each; 12 candidates for a UV sphere); a
     hit writes its hit-list entry (4) and simplex (17 x 8: count, then per point p.xyz
@@ -102,8 +103,9 @@ def algorithmic_bytes(stage, s, pairs, hits_frac, pos=None):
         r = rr[a] + rr[b]
         apart = d > r + 1e-6 * r + 1e-6
         culled = float(np.count_nonzero(apart))
+        bodies = float(len(np.unique(np.concatenate([a[apart], b[apart]]))))
         rest = np.sum((per_pair_fixed + verts + 1)[~apart])
-        return float(culled * (8 + 2 * 32 + 1) + rest + hits_frac * len(pairs) * (4 + 17 * 8))
+        return float(culled * (8 + 1) + bodies * 32 + rest + hits_frac * len(pairs) * (4 + 17 * 8))
     if stage == "epa":
         return float(hits_frac * np.sum(4 + per_pair_fixed + verts + 17 * 8 + 80 + 1))
     raise ValueError(stage)
@@ -138,7 +140,7 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
-def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0):
+def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, integrate=True):
     """Compulsory HBM bytes of each stage's launches in one step, with the step's own
     counts (n bodies, N dynamic, M moved leaves, K candidate pairs, P pairs, H GJK hits,
     C manifold contacts, R solver rows; s_r = 4 for FP32 rows, else 8):
@@ -152,16 +154,21 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0):
                         (no keys); averaged over one build and tree_refits refits
       pair_update       48 M + 80 n + 36 K + 20 P: the moving leaves' boxes, the tree read
                         once, candidates written, sorted and matched, the previous list
-                        read and the new one with its sources written
+                        read and the new one with its sources written; n when no leaf
+                        moved (the moved flags read; the list is kept as it is, each pair
+                        its own source)
       gjk, epa          algorithmic_bytes()
-      manifold          22 P + 208 C + 192 H: keys, sources, status, counts and events;
+      manifold          22 P + 208 C + 192 H: sources, status, previous and new counts,
+                        slots and events, and the list of pairs with contacts or a hit;
                         contacts read and written; hits' contact data and poses
       solver            496 R: contacts read, both bodies' pose, velocity and inverse
                         inertia read for the rows, impulses and velocities written
       solve_iterations  R (38 s_r + 96): every row read once and its bodies' velocities
                         read and written once (rows that stay on chip across passes)
       integrate_pos     n + 448 N (k_integrate_pos: pose, velocities and local inertia
-                        read; pose and world inertia written)"""
+                        read; pose and world inertia written)
+    With integration off the first stage only clears the moved flags (n) and the last
+    launches nothing."""
     n = len(s["shape"])
     nd = n - int(s["n_static"])
     n_shapes = len(s["shape_offsets"]) - 1
@@ -172,7 +179,7 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0):
         "lbvh_build": ("LBVH build or refit (k_morton, k_rs_*, k_build_internal, k_leaf_gather, k_refit, k_span)",
                        ((253 + 229 * tree_refits) / (1 + tree_refits) + (32 if mixed else 0)) * n),
         "pair_update": ("pair-set update (k_pair_query, sort, k_match, k_prev_*, k_merge)",
-                        48 * M + 80 * n + 36 * K + 20 * P),
+                        n if M == 0 and K == 0 else 48 * M + 80 * n + 36 * K + 20 * P),
         "gjk": ("GJK stage (k_gjk_cull" + (", k_gjk32" if mixed else "") + ", k_gjk)",
                 algorithmic_bytes("gjk", s, pairs, hits_frac, pos) if len(pairs) else 0.0),
         "epa": ("k_epa (+ larger-store reruns)", algorithmic_bytes("epa", s, pairs, hits_frac) if len(pairs) else 0.0),
@@ -181,6 +188,9 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0):
         "solve_iterations": ("k_solve_islands" + ("<true> (FP32 rows)" if mixed else "<false>"), R * (38 * sr + 96)),
         "integrate_pos": ("k_integrate_pos", n + 448 * nd),
     }
+    if not integrate:
+        out["integrate_vel"] = ("k_fill_u32 (moved flags cleared; integration off)", n)
+        out["integrate_pos"] = ("none (integration off)", 0)
     return out
 
 
@@ -579,7 +589,7 @@ def main():
     print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
     models = stage_bytes(s, c, pairs, hits_frac, r["pos"], s["iterations"], mixed or prec == "fp32-solve",
-                         args.tree_refits)
+                         args.tree_refits, integrate=(int(s["flags"]) & pk.FLAG_NO_INTEGRATE) == 0)
     dom, table = roofline_of(stage, args.steps, models)
     d = table[dom]
     roof = {"bound": "hbm", "stage": dom, "kernel": d["kernel"], "achieved": d["achieved_GBs"],

@@ -250,6 +250,7 @@ struct pkg_ctx
     dbuf<unsigned long long> claim, used; // colouring: per-body claims, colours taken per body
     dbuf<uint32_t> clist;                 // colouring: the row manifolds
     uint32_t n_colours = 0;
+    uint32_t n_listed = 0; // manifolds with rows in clist after the last step
 
     // joints, dense in key order; an add or remove rebuilds the records into the other set
     std::vector<uint32_t> jkey_h;         // live keys
@@ -514,7 +515,7 @@ void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
     jck.reserve(J);
     jck_alt.reserve(J);
     jcs.reserve(c + 1);
-    jord = launch_colour_bucket(jcol.p, J, c, jck.p, jck_alt.p, jorder.p, jorder_alt.p, jcs.p, sc, st);
+    jord = launch_colour_bucket(jcol.p, nullptr, J, J, c, jck.p, jck_alt.p, jorder.p, jorder_alt.p, jcs.p, sc, st);
     jcs_h.assign(c + 1, 0u);
     hip_check(hipMemcpyAsync(jcs_h.data(), jcs.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
               "joint colour copy");
@@ -677,16 +678,17 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     // carry-over: a candidate found in the previous sorted list keeps its manifold;
     // previous pairs no candidate covers stay while their leaves overlap
     uint32_t K = 0;
+    bool kept_whole = false;
     if (!all_moved && Pprev && h_counters[C_MOVED] == 0)
     {
         // No leaf moved: there are no candidates and every previous pair keeps both fat
         // boxes, so the carry-over would keep the previous list whole and in order. It
-        // is taken over as it stands, each pair's manifold coming from its own index.
+        // is taken over as it stands, each pair's manifold coming from its own index
+        // (no source map: a null map stands for that).
         if (M != 0) throw pk_error(PKG_DEVICE_ERROR, "broadphase found candidates with no leaf moved");
         K = Pprev;
         keys[nxt].swap(keys[cur]);
-        src.reserve(Pprev);
-        launch_iota_u32(src.p, Pprev, st);
+        kept_whole = true;
     }
     else
     {
@@ -716,6 +718,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         launch_merge(akeys.p, asrc.p, K, ck, M, keys[nxt].p, src.p, st);
     }
     const uint32_t Pn = M + K;
+    const uint32_t *srcp = kept_whole ? nullptr : src.p;
     record(ev, 3, st);
 
     // ---- 3. narrowphase (collision_phases.h:244-320)
@@ -760,7 +763,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
         {
             gjk_list2.reserve(Pn);
             f32.s32 = &s32;
-            f32.src = src.p;
+            f32.src = srcp;
             f32.prev_count = mcount[cur].p;
             f32.list2 = gjk_list2.p;
         }
@@ -862,8 +865,9 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             mowner[nxt].reserve(c);
             mdcap[nxt] = c;
         }
-        launch_manifold_update(keys[nxt].p, src.p, Pn, b, g, MF(cur), MF(nxt), events.p,
-                               (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, counters.p, st);
+        // the GJK list (Pn entries) is free again: it lists the pairs whose manifolds change
+        launch_manifold_update(keys[nxt].p, srcp, Pn, dneed, b, g, MF(cur), MF(nxt), events.p,
+                               (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, gjk_list.p, counters.p, st);
     }
     else
     {
@@ -878,6 +882,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
 
     // ---- 4. contacts and joints (constraint.h:1052-1201)
     n_colours = 0;
+    n_listed = 0;
     const uint32_t n_joints = static_cast<uint32_t>(jkey_h.size());
     if (solve && (Pn || n_joints))
     {
@@ -932,6 +937,7 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             }
         }
         n_colours = c;
+        n_listed = nl;
         if (n_joints) joint_prepare(b, dt);
         bool solved = false;
         if (c && !n_joints && c <= kIslandMaxColours)
@@ -990,11 +996,11 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
             if (c)
             {
                 colour_start.reserve(c + 1);
-                order.reserve(Pn);
-                order_alt.reserve(Pn);
-                ckeys.reserve(Pn);
-                ckeys_alt.reserve(Pn);
-                ord = launch_colour_bucket(colour.p, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
+                order.reserve(nl);
+                order_alt.reserve(nl);
+                ckeys.reserve(nl);
+                ckeys_alt.reserve(nl);
+                ord = launch_colour_bucket(colour.p, clist.p, nl, Pn, c, ckeys.p, ckeys_alt.p, order.p, order_alt.p,
                                            colour_start.p, sc, st);
                 cs.assign(c + 1, 0u);
                 hip_check(hipMemcpyAsync(cs.data(), colour_start.p, (c + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st),
@@ -1930,21 +1936,23 @@ pkg_status pkg_get_colours(pkg_ctx *ctx, uint32_t *colours, uint64_t *keys, size
     PKG_GUARD(ctx, {
         uint32_t P = ctx->P[ctx->cur];
         size_t cnt = 0;
-        if (ctx->n_colours && P)
+        if (ctx->n_colours && P && ctx->n_listed)
         {
+            // the coloured manifolds are the listed ones, reported in pair order
+            std::vector<uint32_t> list = download(ctx->clist, ctx->n_listed, ctx->st);
+            std::sort(list.begin(), list.end());
             std::vector<uint32_t> col = download(ctx->colour, P, ctx->st);
             std::vector<uint64_t> k;
             if (keys) k = download(ctx->keys[ctx->cur], P, ctx->st);
-            for (uint32_t p = 0; p < P; ++p)
-                if (col[p] != kNone)
+            for (uint32_t p : list)
+            {
+                if (cnt < cap)
                 {
-                    if (cnt < cap)
-                    {
-                        if (colours) colours[cnt] = col[p];
-                        if (keys) keys[cnt] = k[p];
-                    }
-                    ++cnt;
+                    if (colours) colours[cnt] = col[p];
+                    if (keys) keys[cnt] = k[p];
                 }
+                ++cnt;
+            }
         }
         *n = cnt;
     })

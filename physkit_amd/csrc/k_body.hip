@@ -131,11 +131,6 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
     size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_iota_u32(uint32_t *p, size_t n)
-{
-    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = static_cast<uint32_t>(i);
-}
 
 unsigned grid(size_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 } // namespace
@@ -164,10 +159,6 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipS
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st)
 {
     if (n) k_fill_u32<<<grid(n), kBlock, 0, st>>>(p, v, n);
-}
-void launch_iota_u32(uint32_t *p, size_t n, hipStream_t st)
-{
-    if (n) k_iota_u32<<<grid(n), kBlock, 0, st>>>(p, n);
 }
 
 } // namespace pk

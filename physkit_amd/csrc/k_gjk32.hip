@@ -70,11 +70,11 @@ __global__ void __launch_bounds__(kBlock)
             const int32_t *prev_count, uint8_t *status, uint32_t *list2, unsigned long long *counters)
 {
     const uint32_t n_list = static_cast<uint32_t>(counters[C_GJK_LIST]);
-    const uint32_t threads = gridDim.x * blockDim.x;
+    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     // the same number of rounds on every lane (each round's append is workgroup-wide)
-    for (uint32_t base = 0; base < n_list; base += threads)
+    for (uint64_t base = 0; base < n_list; base += threads)
     {
-        const uint32_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
         bool pass = false;
         uint32_t p = 0;
         if (i < n_list)
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kBlock)
             p = list[i];
             // a pair in contact after the previous step is most likely still touching: it
             // goes to the FP64 GJK without an attempt
-            const uint32_t sp = src ? src[p] : kNone;
+            const uint32_t sp = source_of(src, p);
             const bool touched = sp != kNone && prev_count[sp] > 0;
             const uint64_t key = keys[p];
             if (!touched && certify_apart<UV>(b, s, static_cast<uint32_t>(key >> 32),
@@ -93,6 +93,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         const unsigned long long slot = block_append(&counters[C_GJK_LIST2], pass);
         if (pass) list2[slot] = p;
+        __syncthreads(); // the next round's append reuses the same LDS
     }
 }
 

@@ -239,28 +239,94 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-// one thread per pair: a pair whose bodies are apart (above) is separated (status 0)
-// without GJK; the others are listed for k_gjk
+// A lane takes Q consecutive pairs: all are written separated (status 0) and those
+// whose bodies are not apart (above) are listed for k_gjk, which writes their status
+// again. With per-body records (many pairs per body, most culled by the FP32 spheres)
+// Q is 4: one 32-byte key load, one 4-byte status store and eight sphere loads in
+// flight per lane. From the planar arrays (few pairs per body, each test reading
+// several arrays) Q is 1. A capped grid steps through the pairs, the same number of
+// rounds on every lane.
 template <bool REC>
 __global__ void __launch_bounds__(kBlock)
     k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec,
                const float4 *rec32, uint8_t *status, uint32_t *list, unsigned long long *counters)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = false;
-    if (p < n_pairs)
+    constexpr uint32_t Q = REC ? 4 : 1;
+    const uint64_t n_groups = (static_cast<uint64_t>(n_pairs) + Q - 1) / Q;
+    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t base = 0; base < n_groups; base += threads)
     {
-        const uint64_t key = keys[p];
-        const uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        if (REC)
-            keep = !((rec32 && spheres32_apart(rec32[ia], rec32[ib])) ||
-                     records_apart(rec + static_cast<size_t>(kCullRec) * ia, rec + static_cast<size_t>(kCullRec) * ib));
-        else
-            keep = !bodies_apart(cull_of(b, s, ia), cull_of(b, s, ib));
-        if (!keep) status[p] = 0;
+        const uint64_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t p0 = static_cast<uint32_t>(Q * q);
+        uint32_t n_here = 0, keepmask = 0;
+        uint64_t key[Q] = {};
+        if (q < n_groups)
+        {
+            n_here = min(Q, n_pairs - p0);
+            if constexpr (Q == 4)
+            {
+                if (n_here == 4)
+                {
+                    const ulonglong2 k01 = *reinterpret_cast<const ulonglong2 *>(keys + p0);
+                    const ulonglong2 k23 = *reinterpret_cast<const ulonglong2 *>(keys + p0 + 2);
+                    key[0] = k01.x;
+                    key[1] = k01.y;
+                    key[2] = k23.x;
+                    key[3] = k23.y;
+                    *reinterpret_cast<uint32_t *>(status + p0) = 0u;
+                }
+                else
+                    for (uint32_t k = 0; k < n_here; ++k)
+                    {
+                        key[k] = keys[p0 + k];
+                        status[p0 + k] = 0;
+                    }
+            }
+            else
+            {
+                key[0] = keys[p0];
+                status[p0] = 0;
+            }
+            for (uint32_t k = n_here; k < Q; ++k) key[k] = key[0]; // a valid pair, never listed
+            if (REC && rec32)
+            {
+                // the sphere loads are issued before any test
+                float4 sa[Q], sb[Q];
+#pragma unroll
+                for (uint32_t k = 0; k < Q; ++k)
+                {
+                    sa[k] = rec32[static_cast<uint32_t>(key[k] >> 32)];
+                    sb[k] = rec32[static_cast<uint32_t>(key[k] & 0xFFFFFFFFu)];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < Q; ++k)
+                    if (k < n_here && !spheres32_apart(sa[k], sb[k])) keepmask |= 1u << k;
+            }
+            else
+                keepmask = (1u << n_here) - 1u;
+#pragma unroll
+            for (uint32_t k = 0; k < Q; ++k)
+                if (keepmask & (1u << k))
+                {
+                    const uint32_t ia = static_cast<uint32_t>(key[k] >> 32);
+                    const uint32_t ib = static_cast<uint32_t>(key[k] & 0xFFFFFFFFu);
+                    bool apart;
+                    if (REC)
+                        apart = records_apart(rec + static_cast<size_t>(kCullRec) * ia,
+                                              rec + static_cast<size_t>(kCullRec) * ib);
+                    else
+                        apart = bodies_apart(cull_of(b, s, ia), cull_of(b, s, ib));
+                    if (apart) keepmask &= ~(1u << k);
+                }
+        }
+        const uint32_t nk = static_cast<uint32_t>(__popc(keepmask));
+        // a workgroup with nothing to list skips the append; the barrier also separates
+        // the rounds' appends
+        if (!__syncthreads_or(nk != 0)) continue;
+        unsigned long long at = block_reserve(&counters[C_GJK_LIST], nk);
+        for (uint32_t k = 0; k < Q; ++k)
+            if (keepmask & (1u << k)) list[at++] = p0 + k;
     }
-    const unsigned long long slot = block_append(&counters[C_GJK_LIST], keep);
-    if (keep) list[slot] = p;
 }
 
 // GJK over the listed pairs, G lanes per pair: the lanes of a group read consecutive
@@ -278,10 +344,10 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
     const int gl = static_cast<int>(t % G);
     // groups step through the list together: the same number of rounds on every lane,
     // so each round's workgroup-wide append sees all of them
-    const uint32_t per_round = gridDim.x * blockDim.x / G;
-    for (uint32_t base = 0; base < n_list; base += per_round)
+    const uint64_t per_round = gridDim.x * blockDim.x / G;
+    for (uint64_t base = 0; base < n_list; base += per_round)
     {
-        const uint32_t i = base + t / G;
+        const uint64_t i = base + t / G;
         bool hit = false;
         simplex x;
         x.n = 0;
@@ -308,6 +374,7 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
                 save_simplex(g.simplex, g.cap, static_cast<uint32_t>(slot), x);
             }
         }
+        __syncthreads(); // the next round's append reuses the same LDS
     }
 }
 
@@ -493,100 +560,168 @@ __device__ __forceinline__ void add_contact(mani &m, const contact &k) // :124-1
         add_reduce(m, k);
 }
 
+// Every pair, four consecutive ones per lane: all four are written empty (no contacts,
+// no slot, no event; vector stores) and those with a GJK hit or contacts before are
+// listed for k_manifold, which writes them again. A pair with neither stays empty and
+// no body state is read for it. A capped grid steps through the pairs, the same number
+// of rounds on every lane.
 __global__ void __launch_bounds__(kBlock)
-    k_manifold(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, bodies_dev b, gjk_out_dev g,
+    k_manifold_live(const uint32_t *src, uint32_t n_pairs, const uint8_t *status, const int32_t *prev_count,
+                    manifolds_dev next, uint8_t *events, uint32_t *live_list, unsigned long long *counters)
+{
+    const uint64_t n_quads = (static_cast<uint64_t>(n_pairs) + 3) / 4;
+    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t base = 0; base < n_quads; base += threads)
+    {
+        const uint64_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t p0 = static_cast<uint32_t>(4 * q);
+        uint32_t livemask = 0;
+        if (q < n_quads)
+        {
+            const uint32_t n_here = min(4u, n_pairs - p0);
+            if (n_here == 4)
+            {
+                const uint32_t st4 = *reinterpret_cast<const uint32_t *>(status + p0);
+                int pc[4];
+                if (src)
+                {
+                    const uint4 s4 = *reinterpret_cast<const uint4 *>(src + p0);
+                    const uint32_t sp[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pc[k] = sp[k] != kNone ? prev_count[sp[k]] : 0;
+                }
+                else
+                {
+                    // the previous list kept whole: the previous counts at the same indices
+                    const int4 c4 = *reinterpret_cast<const int4 *>(prev_count + p0);
+                    pc[0] = c4.x;
+                    pc[1] = c4.y;
+                    pc[2] = c4.z;
+                    pc[3] = c4.w;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (((st4 >> (8 * k)) & 0xFFu) == 1u || pc[k] > 0) livemask |= 1u << k;
+                *reinterpret_cast<int4 *>(next.count + p0) = make_int4(0, 0, 0, 0);
+                *reinterpret_cast<uint4 *>(next.slot + p0) = make_uint4(kNone, kNone, kNone, kNone);
+                *reinterpret_cast<uint32_t *>(events + p0) = 0u;
+            }
+            else
+                for (uint32_t k = 0; k < n_here; ++k)
+                {
+                    const uint32_t p = p0 + k, sp = source_of(src, p);
+                    if (status[p] == 1 || (sp != kNone && prev_count[sp] > 0)) livemask |= 1u << k;
+                    next.count[p] = 0;
+                    next.slot[p] = kNone;
+                    events[p] = 0;
+                }
+        }
+        const uint32_t nl = static_cast<uint32_t>(__popc(livemask));
+        // a workgroup with no live pair (most of them where pairs far outnumber contacts)
+        // skips the append; the barrier also separates the rounds' appends
+        if (!__syncthreads_or(nl != 0)) continue;
+        unsigned long long at = block_reserve(&counters[C_MLIVE], nl);
+        for (int k = 0; k < 4; ++k)
+            if (livemask & (1u << k)) live_list[at++] = p0 + static_cast<uint32_t>(k);
+    }
+}
+
+// The listed pairs' manifolds (collision_phases.h:197-290)
+__global__ void __launch_bounds__(kBlock)
+    k_manifold(const uint64_t *keys, const uint32_t *src, const uint32_t *live_list, bodies_dev b, gjk_out_dev g,
                manifolds_dev prev, manifolds_dev next, uint8_t *events, int epa_guard,
                unsigned long long *counters)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t count = 0;
-    mani nm;
-    nm.n = 0;
-    // a pair with no old contacts and no new one stays empty: no body state is read
-    const bool live = p < n_pairs && (g.status[p] == 1 || (src[p] != kNone && prev.count[src[p]] > 0));
-    if (p < n_pairs && !live)
+    const uint64_t n_live = counters[C_MLIVE];
+    // the same number of rounds on every lane (each round's append and sum are workgroup-wide)
+    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t base = 0; base < n_live; base += threads)
     {
-        next.count[p] = 0;
-        next.slot[p] = kNone;
-        events[p] = 0;
-    }
-    if (live)
-    {
-        const double distance2_eps = 0.005 * 0.005; // :212
-        const double breaking = 0.05;               // :213
-        const double drift2_eps = 0.06;             // :267
-        uint64_t key = keys[p];
-        uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
-        v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
-        qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
-        bool has_new = g.status[p] == 1;
-        contact nc{};
-        if (has_new)
+        const uint64_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool live = i < n_live;
+        const uint32_t p = live ? live_list[i] : 0u;
+        uint32_t count = 0;
+        mani nm;
+        nm.n = 0;
+        if (live)
         {
-            const double *inf = g.info + g.hit_of[p];
-            size_t s = g.cap;
-            nc.n = mk(inf[0], inf[s], inf[2 * s]);
-            v3 wa = mk(inf[3 * s], inf[4 * s], inf[5 * s]);
-            v3 wb = mk(inf[6 * s], inf[7 * s], inf[8 * s]);
-            nc.la = rotate(conj(qa), wa - pa); // particle.h:107-108
-            nc.lb = rotate(conj(qb), wb - pb);
-            nc.depth = inf[9 * s];
-            nc.ln = 0.0;
-            nc.lt1 = 0.0;
-            nc.lt2 = 0.0;
-            if (epa_guard)
-            {
-                bool ok = sqn(nc.n) > 0.0 && isfinite(wa.x) && isfinite(wa.y) && isfinite(wa.z) &&
-                          isfinite(wb.x) && isfinite(wb.y) && isfinite(wb.z) && isfinite(nc.depth);
-                has_new = ok;
-            }
-        }
-        uint32_t sp = src[p];
-        int oldn = sp != kNone ? prev.count[sp] : 0;
-        const uint32_t osl = oldn > 0 ? prev.slot[sp] : kNone;
-        for (int i = 0; i < oldn; ++i)
-        {
-            contact old = ld_contact(prev, osl, i);
+            const double distance2_eps = 0.005 * 0.005; // :212
+            const double breaking = 0.05;               // :213
+            const double drift2_eps = 0.06;             // :267
+            uint64_t key = keys[p];
+            uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+            v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
+            qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
+            bool has_new = g.status[p] == 1;
+            contact nc{};
             if (has_new)
             {
-                if (sqn(nc.la - old.la) < distance2_eps || sqn(nc.lb - old.lb) < distance2_eps)
+                const double *inf = g.info + g.hit_of[p];
+                size_t s = g.cap;
+                nc.n = mk(inf[0], inf[s], inf[2 * s]);
+                v3 wa = mk(inf[3 * s], inf[4 * s], inf[5 * s]);
+                v3 wb = mk(inf[6 * s], inf[7 * s], inf[8 * s]);
+                nc.la = rotate(conj(qa), wa - pa); // particle.h:107-108
+                nc.lb = rotate(conj(qb), wb - pb);
+                nc.depth = inf[9 * s];
+                nc.ln = 0.0;
+                nc.lt1 = 0.0;
+                nc.lt2 = 0.0;
+                if (epa_guard)
                 {
-                    nc.ln = old.ln;
-                    nc.lt1 = old.lt1;
-                    nc.lt2 = old.lt2;
-                    continue;
+                    bool ok = sqn(nc.n) > 0.0 && isfinite(wa.x) && isfinite(wa.y) && isfinite(wa.z) &&
+                              isfinite(wb.x) && isfinite(wb.y) && isfinite(wb.z) && isfinite(nc.depth);
+                    has_new = ok;
                 }
             }
-            v3 wa = rotate(qa, old.la) + pa; // particle.h:110-111
-            v3 wb = rotate(qb, old.lb) + pb;
-            v3 nrm = has_new ? nc.n : old.n;
-            v3 rel = wb - wa;
-            double depth = dot(rel, nrm);
-            v3 pja = wa + nrm * depth;
-            double drift2 = sqn(pja - wb);
-            if (depth > -breaking && drift2 < drift2_eps)
+            const uint32_t sp = source_of(src, p);
+            int oldn = sp != kNone ? prev.count[sp] : 0;
+            const uint32_t osl = oldn > 0 ? prev.slot[sp] : kNone;
+            for (int i = 0; i < oldn; ++i)
             {
-                old.depth = depth;
-                old.n = nrm;
-                add_contact(nm, old);
+                contact old = ld_contact(prev, osl, i);
+                if (has_new)
+                {
+                    if (sqn(nc.la - old.la) < distance2_eps || sqn(nc.lb - old.lb) < distance2_eps)
+                    {
+                        nc.ln = old.ln;
+                        nc.lt1 = old.lt1;
+                        nc.lt2 = old.lt2;
+                        continue;
+                    }
+                }
+                v3 wa = rotate(qa, old.la) + pa; // particle.h:110-111
+                v3 wb = rotate(qb, old.lb) + pb;
+                v3 nrm = has_new ? nc.n : old.n;
+                v3 rel = wb - wa;
+                double depth = dot(rel, nrm);
+                v3 pja = wa + nrm * depth;
+                double drift2 = sqn(pja - wb);
+                if (depth > -breaking && drift2 < drift2_eps)
+                {
+                    old.depth = depth;
+                    old.n = nrm;
+                    add_contact(nm, old);
+                }
             }
+            if (has_new) add_contact(nm, nc);
+            next.count[p] = nm.n;
+            bool was = oldn > 0, is = nm.n > 0;
+            events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
+            count = static_cast<uint32_t>(nm.n);
         }
-        if (has_new) add_contact(nm, nc);
-        next.count[p] = nm.n;
-        bool was = oldn > 0, is = nm.n > 0;
-        events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
-        count = static_cast<uint32_t>(nm.n);
+        // a data slot for each non-empty manifold (the host sized the store for every pair
+        // that had contacts or a GJK hit, so the slots fit)
+        const unsigned long long sl = block_append(&counters[C_MSLOTS], count > 0);
+        if (live)
+        {
+            next.slot[p] = count > 0 ? static_cast<uint32_t>(sl) : kNone;
+            if (count > 0) next.owner[sl] = p;
+            for (int i = 0; i < nm.n; ++i) st_contact(next, static_cast<uint32_t>(sl), i, nm.c[i]);
+        }
+        block_add(&counters[C_CONTACTS], count);
+        __syncthreads(); // the next round's append and sum reuse the same LDS
     }
-    // a data slot for each non-empty manifold (the host sized the store for every pair
-    // that had contacts or a GJK hit, so the slots fit)
-    const unsigned long long sl = block_append(&counters[C_MSLOTS], count > 0);
-    if (live)
-    {
-        next.slot[p] = count > 0 ? static_cast<uint32_t>(sl) : kNone;
-        if (count > 0) next.owner[sl] = p;
-        for (int i = 0; i < nm.n; ++i) st_contact(next, static_cast<uint32_t>(sl), i, nm.c[i]);
-    }
-    block_add(&counters[C_CONTACTS], count);
 }
 
 // ---- standalone batch ----
@@ -728,12 +863,12 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
         // the mixed form adds the FP32 sphere records after the FP64 ones (16-byte aligned)
         float4 *rec32 = f32.s32 ? reinterpret_cast<float4 *>(cull_rec + static_cast<size_t>(kCullRec) * b.n) : nullptr;
         k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec, rec32);
-        k_gjk_cull<true><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, rec32, g.status, list,
-                                                           counters);
+        k_gjk_cull<true><<<std::min<unsigned>(grid((n_pairs + 3) / 4), kListGrid), kBlock, 0, st>>>(
+            keys, n_pairs, b, s, cull_rec, rec32, g.status, list, counters);
     }
     else
-        k_gjk_cull<false><<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, b, s, cull_rec, nullptr, g.status, list,
-                                                            counters);
+        k_gjk_cull<false><<<std::min<unsigned>(grid(n_pairs), kListGrid), kBlock, 0, st>>>(
+            keys, n_pairs, b, s, cull_rec, nullptr, g.status, list, counters);
     int slot = C_GJK_LIST;
     if (f32.s32)
     {
@@ -809,14 +944,19 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                                                         static_cast<epa_store_big *>(pool), counters);
 }
 
-void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t live_bound,
                             const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, bool epa_guard,
+                            manifolds_dev next, uint8_t *events, bool epa_guard, uint32_t *live_list,
                             unsigned long long *counters, hipStream_t st)
 {
-    if (n_pairs)
-        k_manifold<<<grid(n_pairs), kBlock, 0, st>>>(keys, src, n_pairs, b, g, prev, next, events,
-                                                     epa_guard ? 1 : 0, counters);
+    if (!n_pairs) return;
+    k_manifold_live<<<std::min<unsigned>(grid((n_pairs + 3) / 4), kListGrid), kBlock, 0, st>>>(src, n_pairs, g.status,
+                                                                                      prev.count, next, events,
+                                                                                      live_list, counters);
+    const uint32_t bound = std::min(live_bound, n_pairs);
+    if (bound)
+        k_manifold<<<std::min<unsigned>(grid(bound), kListGrid), kBlock, 0, st>>>(
+            keys, src, live_list, b, g, prev, next, events, epa_guard ? 1 : 0, counters);
 }
 
 void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,

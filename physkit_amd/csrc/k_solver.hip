@@ -214,9 +214,9 @@ __device__ __forceinline__ bool contact_frame(const manifolds_dev &m, uint32_t s
     return !(pen <= 0.0);
 }
 
-// which manifolds make solver rows: colour[p] = kUncoloured (to be coloured) or kNone,
-// and the row count; the rows themselves are written by k_setup once the solve order
-// is known
+// which manifolds make solver rows: each is listed and its colour[p] set to kUncoloured
+// (no other pair's colour is read), and the row count; the rows themselves are written
+// by k_setup once the solve order is known
 __global__ void __launch_bounds__(kBlock)
     k_row_flags(const uint64_t *keys, bodies_dev b, manifolds_dev m, uint32_t *colour, uint32_t *list,
                 unsigned long long *counters)
@@ -441,27 +441,29 @@ __global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n,
     block_max(&counters[C_NCOL], ncol);
 }
 
-// colour buckets by a stable radix sort of (colour, manifold index); manifolds
-// without rows go to bucket n_colours, after every colour
-__global__ void k_colour_keys(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
+// colour buckets by a radix sort of the listed items' (colour, index) keys: each
+// colour's items in index order (list null: items 0..nl-1)
+__global__ void k_colour_keys(const uint32_t *colour, const uint32_t *list, uint32_t nl, uint64_t *keys,
                               uint32_t *vals)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    uint32_t c = colour[p];
-    keys[p] = c < n_colours ? c : n_colours;
-    vals[p] = p;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const uint32_t p = list ? list[i] : i;
+    keys[i] = (static_cast<uint64_t>(colour[p]) << 32) | p;
+    vals[i] = p;
 }
 
+// start[c]: the first sorted key of colour c or above (start[n_colours] = n)
 __global__ void k_colour_bounds(const uint64_t *sorted, uint32_t n, uint32_t n_colours, uint32_t *start)
 {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > n_colours) return;
+    const uint64_t first = static_cast<uint64_t>(c) << 32;
     uint32_t lo = 0, hi = n;
     while (lo < hi)
     {
         uint32_t mid = lo + ((hi - lo) >> 1);
-        if (sorted[mid] < c)
+        if (sorted[mid] < first)
             lo = mid + 1;
         else
             hi = mid;
@@ -1194,7 +1196,6 @@ void launch_row_flags(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &
                       uint32_t *list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    launch_fill_u32(colour, kNone, n_pairs, st);
     if (m.dcap) k_row_flags<<<grid(m.dcap), kBlock, 0, st>>>(keys, b, m, colour, list, counters);
 }
 void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord, uint32_t cap, const bodies_dev &b,
@@ -1226,20 +1227,23 @@ void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n
         k_assign<<<grid(n), kBlock, 0, st>>>(keys, list, n, b, colour, world_of, pair_base, claim, used, round,
                                              counters);
 }
-uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
-                               uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,
-                               sort_scratch &sc, hipStream_t st)
+uint32_t *launch_colour_bucket(const uint32_t *colour, const uint32_t *list, uint32_t nl, uint32_t n_pairs,
+                               uint32_t n_colours, uint64_t *keys, uint64_t *keys_alt, uint32_t *order,
+                               uint32_t *order_alt, uint32_t *colour_start, sort_scratch &sc, hipStream_t st)
 {
-    if (!n_pairs || !n_colours) return order;
-    k_colour_keys<<<grid(n_pairs), kBlock, 0, st>>>(colour, n_pairs, n_colours, keys, order);
-    int bits = 1;
-    while (bits < 32 && (1ull << bits) <= n_colours) ++bits; // keys 0..n_colours
-    int shifts[4];
+    if (!nl || !n_colours) return order;
+    k_colour_keys<<<grid(nl), kBlock, 0, st>>>(colour, list, nl, keys, order);
+    int pbits = 1; // pair indices below n_pairs
+    while (pbits < 32 && (1ull << pbits) < n_pairs) ++pbits;
+    int cbits = 1; // colours below n_colours
+    while (cbits < 32 && (1ull << cbits) < n_colours) ++cbits;
+    int shifts[8];
     int ns = 0;
-    for (int sft = 0; sft < bits; sft += 8) shifts[ns++] = sft;
-    bool alt = radix_sort_u64(keys, order, keys_alt, order_alt, n_pairs, shifts, ns, sc, st);
+    for (int sft = 0; sft < pbits; sft += 8) shifts[ns++] = sft;
+    for (int sft = 32; sft < 32 + cbits; sft += 8) shifts[ns++] = sft;
+    bool alt = radix_sort_u64(keys, order, keys_alt, order_alt, nl, shifts, ns, sc, st);
     const uint64_t *sk = alt ? keys_alt : keys;
-    k_colour_bounds<<<grid(n_colours + 1), kBlock, 0, st>>>(sk, n_pairs, n_colours, colour_start);
+    k_colour_bounds<<<grid(n_colours + 1), kBlock, 0, st>>>(sk, nl, n_colours, colour_start);
     return alt ? order_alt : order;
 }
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,

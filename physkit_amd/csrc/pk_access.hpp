@@ -73,8 +73,9 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long *co
 // counter take 0.79 ms; one add per 256-thread workgroup, 0.20 ms), so a kernel that
 // adds once per wavefront is bounded by 12 ns x its wavefronts. These add once per
 // workgroup. Every thread of the workgroup must call them (none may have returned),
-// the workgroup size must be a multiple of 64 and at most 1024, and each may be called
-// once per kernel: they use static LDS.
+// the workgroup size must be a multiple of 64 and at most 1024. Their LDS is static: a
+// kernel that calls one again (a loop over rounds) needs a workgroup barrier between
+// the calls, after every thread has returned from the earlier one.
 
 // exclusive offset of this thread's v slots in a range reserved at *counter
 __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *counter, uint32_t v)

@@ -13,6 +13,9 @@ namespace pk
 {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// the previous step's index of pair p (kNone for a new pair); src is null where the
+// previous list was kept whole, each pair at its own index
+__host__ __device__ __forceinline__ uint32_t source_of(const uint32_t *src, uint32_t p) { return src ? src[p] : p; }
 
 // per-body state, n bodies, body id = index
 struct bodies_dev
@@ -91,6 +94,7 @@ enum counter_slot
     C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
     C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
     C_MSLOTS = 19,    // manifold data slots taken by the manifold update
+    C_MLIVE = 20,     // pairs the manifold update lists (a GJK hit or contacts before)
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
     // addresses instead of serialising on one
@@ -108,7 +112,6 @@ void launch_integrate_vel(const bodies_dev &b, const shapes_dev &s, double gx, d
 void launch_integrate_pos(const bodies_dev &b, double dt, hipStream_t st);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
-void launch_iota_u32(uint32_t *p, size_t n, hipStream_t st); // p[i] = i
 
 // ---------------- k_sort.hip ----------------
 // LSD radix sort of (key, value) pairs on 8-bit digits at the given bit shifts
@@ -229,8 +232,8 @@ struct shapes32_dev
 void launch_verts32(const shapes_dev &s, float4 *out, hipStream_t st);
 // Pairs the cull listed (counters[C_GJK_LIST] entries of `list`, at most n_pairs) that
 // FP32 proves apart by more than GJK's margin are written separated (status 0); the
-// rest, and every pair with contacts after the previous step (prev_count[src[p]] > 0),
-// are listed in list2, count in counters[C_GJK_LIST2].
+// rest, and every pair with contacts after the previous step (prev_count of its source,
+// source_of(src, p), above 0), are listed in list2, count in counters[C_GJK_LIST2].
 void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, const bodies_dev &b,
                   const shapes32_dev &s, bool uv, const uint32_t *src, const int32_t *prev_count, uint8_t *status,
                   uint32_t *list2, unsigned long long *counters, hipStream_t st);
@@ -277,9 +280,11 @@ bool epa_first_tier_is_lds(uint32_t n_hits, const tuning &tu);
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
                     const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,
                     unsigned long long *counters, hipStream_t st);
-void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs,
+// live_list holds n_pairs entries; live_bound bounds the pairs that have a GJK hit or
+// had contacts (it sizes the second kernel's grid; the kernel reads the true count)
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t live_bound,
                             const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, bool epa_guard,
+                            manifolds_dev next, uint8_t *events, bool epa_guard, uint32_t *live_list,
                             unsigned long long *counters, hipStream_t st);
 
 // standalone narrowphase over shape pairs (pkg_gjk_epa_batch)
@@ -337,11 +342,13 @@ void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n
                           uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, unsigned long long *used, uint32_t round,
                           unsigned long long *counters, hipStream_t st);
-// stable bucket of the solver manifolds by colour: returns the order array (one of
-// the two given) and colour_start[0..n_colours] (manifolds without rows come last)
-uint32_t *launch_colour_bucket(const uint32_t *colour, uint32_t n_pairs, uint32_t n_colours, uint64_t *keys,
-                               uint64_t *keys_alt, uint32_t *order, uint32_t *order_alt, uint32_t *colour_start,
-                               sort_scratch &sc, hipStream_t st);
+// the listed solver manifolds (list[0, nl), or 0..nl-1 when list is null; every one
+// coloured below n_colours) by colour, each colour in index order: returns the order
+// array (one of the two given, nl entries each) and colour_start[0..n_colours];
+// n_pairs bounds the indices
+uint32_t *launch_colour_bucket(const uint32_t *colour, const uint32_t *list, uint32_t nl, uint32_t n_pairs,
+                               uint32_t n_colours, uint64_t *keys, uint64_t *keys_alt, uint32_t *order,
+                               uint32_t *order_alt, uint32_t *colour_start, sort_scratch &sc, hipStream_t st);
 void launch_solve_colour(const uint64_t *keys, const uint32_t *order, uint32_t begin, uint32_t end,
                          const bodies_dev &b, rows_dev r, uint32_t cap, bool warm, hipStream_t st);
 void launch_impulse_writeback(const uint32_t *order, uint32_t n_act, manifolds_dev m, rows_dev r, uint32_t cap,

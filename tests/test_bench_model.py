@@ -44,14 +44,17 @@ def test_support_vertices_and_radii():
 
 
 def test_gjk_stage_bytes_closed_form():
-    """Two box pairs: one whose bounding spheres are apart (73 bytes), one overlapping
-    (key 8 + 2 x (24 + 32 + 12) + 24 x (8 + 8) vertices + status 1 = 529 bytes), and a
-    hit fraction adding (4 + 17 x 8) per pair."""
+    """Two box pairs: one whose bounding spheres are apart (key and status 9 bytes, and
+    32 bytes for each of its two bodies), one overlapping (key 8 + 2 x (24 + 32 + 12) +
+    24 x (8 + 8) vertices + status 1 = 529 bytes), and a hit fraction adding
+    (4 + 17 x 8) per pair. Culled pairs sharing a body read that body once."""
     b = _box(0.5)
-    s = _scene([b], [0, 0, 0, 0], [[0, 0, 0], [0.9, 0, 0], [10, 0, 0], [20, 0, 0]])
+    s = _scene([b], [0, 0, 0, 0, 0], [[0, 0, 0], [0.9, 0, 0], [10, 0, 0], [20, 0, 0], [30, 0, 0]])
     pairs = np.array([(0 << 32) | 1, (2 << 32) | 3], np.uint64)
     got = bench.algorithmic_bytes("gjk", s, pairs, 0.5)
-    assert got == 73 + 529 + 0.5 * 2 * (4 + 17 * 8)
+    assert got == 9 + 2 * 32 + 529 + 0.5 * 2 * (4 + 17 * 8)
+    shared = np.array([(2 << 32) | 3, (2 << 32) | 4], np.uint64)
+    assert bench.algorithmic_bytes("gjk", s, shared, 0.0) == 2 * 9 + 3 * 32
     epa = bench.algorithmic_bytes("epa", s, pairs, 0.5)
     assert epa == 0.5 * 2 * (4 + 8 + 2 * (24 + 32 + 12) + 24 * 16 + 17 * 8 + 80 + 1)
 
