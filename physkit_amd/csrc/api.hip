@@ -1845,7 +1845,11 @@ pkg_status pkg_download_bodies(pkg_ctx *ctx, const pkg_body_state *out)
         get(ctx->vel, 3, out->vel);
         get(ctx->ang, 3, out->angvel);
         get(ctx->quat, 4, out->quat);
-        get(ctx->fat, 6, out->fat);
+        if (out->fat) // already one record per body
+        {
+            std::vector<double> h = download(ctx->fat, static_cast<size_t>(n) * 6, st);
+            std::memcpy(out->fat, h.data(), h.size() * sizeof(double));
+        }
         if (out->moved)
         {
             std::vector<uint8_t> m = download(ctx->moved, n, st);

@@ -60,7 +60,7 @@ __global__ void k_body_init(bodies_dev b, shapes_dev s) // particle.h:16-28, bvh
         fat.min = mk(nan, nan, nan);
         fat.max = fat.min;
     }
-    stbox(b.fat, b.n, i, fat);
+    st_fat(b.fat, i, fat);
     st3(b.acc, b.n, i, mk(0.0, 0.0, 0.0));
     st3(b.torque, b.n, i, mk(0.0, 0.0, 0.0));
     b.moved[i] = 0;
@@ -89,7 +89,7 @@ __global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy
         st3(b.ang, b.n, i, w);
         qt q = ldq(b.quat, b.n, i);
         box3 tb = instance_bounds(ldbox(s.local_box, s.n, b.shape[i]), q, ld3(b.pos, b.n, i));
-        box3 fat = ldbox(b.fat, b.n, i);
+        box3 fat = ld_fat(b.fat, i);
         if (!box_contains(fat, tb))
         {
             const double margin = 0.1;
@@ -100,7 +100,7 @@ __global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy
             if (disp.x < 0.0) nb.min.x = nb.min.x + disp.x; else nb.max.x = nb.max.x + disp.x;
             if (disp.y < 0.0) nb.min.y = nb.min.y + disp.y; else nb.max.y = nb.max.y + disp.y;
             if (disp.z < 0.0) nb.min.z = nb.min.z + disp.z; else nb.max.z = nb.max.z + disp.z;
-            stbox(b.fat, b.n, i, nb);
+            st_fat(b.fat, i, nb);
             moved = true;
         }
         st3(b.acc, b.n, i, mk(0.0, 0.0, 0.0));

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_scene_bounds(bodies_dev b, unsigned 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x)
         for (int k = 0; k < 3; ++k)
         {
-            double v = (b.fat[static_cast<size_t>(k) * b.n + i] + b.fat[static_cast<size_t>(k + 3) * b.n + i]) * 0.5;
+            double v = (b.fat[6 * static_cast<size_t>(i) + k] + b.fat[6 * static_cast<size_t>(i) + k + 3]) * 0.5;
             c[k] = fmin(c[k], v);
             d[k] = fmax(d[k], v);
         }
@@ -125,7 +125,7 @@ __global__ void k_morton(bodies_dev b, lbvh_dev t, const unsigned long long *sce
     for (int k = 0; k < 3; ++k)
     {
         double lo = unord(scene[k]), hi = unord(scene[3 + k]);
-        double v = (b.fat[static_cast<size_t>(k) * b.n + i] + b.fat[static_cast<size_t>(k + 3) * b.n + i]) * 0.5;
+        double v = (b.fat[6 * static_cast<size_t>(i) + k] + b.fat[6 * static_cast<size_t>(i) + k + 3]) * 0.5;
         double u = hi > lo ? (v - lo) / (hi - lo) : 0.0;
         u = fmin(fmax(u, 0.0), 1.0);
         uint64_t x = static_cast<uint64_t>(u * 65536.0);
@@ -194,12 +194,14 @@ __global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, 
     uint32_t a = ids[i];
     t.leaf_body[i] = a;
     t.leaf_moved[i] = (!b.is_static[a] && (all_moved || b.moved[a])) ? 1 : 0;
-    double f[6];
-    for (int k = 0; k < 6; ++k)
-    {
-        f[k] = b.fat[static_cast<size_t>(k) * b.n + a];
-        t.leaf_box[static_cast<size_t>(i) * 6 + k] = f[k];
-    }
+    // the body's 48-byte fat record, copied to the leaf's as three 16-byte moves
+    const double2 *src = reinterpret_cast<const double2 *>(b.fat + 6 * static_cast<size_t>(a));
+    double2 *dst = reinterpret_cast<double2 *>(t.leaf_box + 6 * static_cast<size_t>(i));
+    const double2 r0 = src[0], r1 = src[1], r2 = src[2];
+    dst[0] = r0;
+    dst[1] = r1;
+    dst[2] = r2;
+    const double f[6] = {r0.x, r0.y, r1.x, r1.y, r2.x, r2.y};
     if (t.leaf_f32)
     {
         t.leaf_f32[2 * static_cast<size_t>(i)] = make_float4(f_down(f[0]), f_down(f[1]), f_down(f[2]), 0.0f);

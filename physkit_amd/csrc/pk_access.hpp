@@ -43,6 +43,24 @@ __device__ __forceinline__ box3 ldbox(const double *p, uint32_t n, uint32_t i)
     b.max = mk(p[3 * n + i], p[4 * n + i], p[5 * n + i]);
     return b;
 }
+// a body's fat box: one 48-byte record per body (min xyz, max xyz), so a read at a
+// body id (the tree's leaf gather, in Morton order) touches one or two lines, not six
+__device__ __forceinline__ box3 ld_fat(const double *p, uint32_t i)
+{
+    const double2 *r = reinterpret_cast<const double2 *>(p + 6 * static_cast<size_t>(i));
+    const double2 a = r[0], c = r[1], d = r[2];
+    box3 b;
+    b.min = mk(a.x, a.y, c.x);
+    b.max = mk(c.y, d.x, d.y);
+    return b;
+}
+__device__ __forceinline__ void st_fat(double *p, uint32_t i, const box3 &b)
+{
+    double2 *r = reinterpret_cast<double2 *>(p + 6 * static_cast<size_t>(i));
+    r[0] = make_double2(b.min.x, b.min.y);
+    r[1] = make_double2(b.min.z, b.max.x);
+    r[2] = make_double2(b.max.y, b.max.z);
+}
 __device__ __forceinline__ void stbox(double *p, uint32_t n, uint32_t i, const box3 &b)
 {
     p[i] = b.min.x;

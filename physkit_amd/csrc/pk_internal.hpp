@@ -35,7 +35,7 @@ struct bodies_dev
     double *Iw_inv;     // 9
     double *restitution;
     double *friction;
-    double *fat; // 6 planes: min xyz, max xyz
+    double *fat; // 6 doubles per body: min xyz, max xyz
     uint32_t *shape;
     uint8_t *is_static;
     uint8_t *moved;
