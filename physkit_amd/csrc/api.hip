@@ -198,6 +198,7 @@ struct pkg_ctx
     dbuf<double> leaf_box;
     dbuf<float4> leaf_f32; // with PKG_FLAG_FP32_BROAD
     dbuf<uint8_t> leaf_moved;
+    dbuf<uint32_t> moved_list;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
     // batched worlds
@@ -449,6 +450,7 @@ lbvh_dev pkg_ctx::tree()
     t.leaf_f32 = (cfg.flags & PKG_FLAG_FP32_BROAD) ? leaf_f32.p : nullptr;
     t.leaf_body = leaf_body.p;
     t.leaf_moved = leaf_moved.p;
+    t.moved_list = moved_list.p;
     t.n_chunks = refit_chunks(n);
     t.span_levels = refit_levels(n);
     t.span_box = span_box.p;
@@ -466,7 +468,7 @@ lbvh_dev pkg_ctx::build_tree(const bodies_dev &b, bool all_moved)
     lbvh_dev t = tree();
     if (topo_ok && topo_age < tune.tree_refits)
     {
-        launch_lbvh_refit(t, b, all_moved, st);
+        launch_lbvh_refit(t, b, all_moved, counters.p, st);
         ++topo_age;
     }
     else
@@ -1279,6 +1281,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->escape.reserve(2 * static_cast<size_t>(n));
         ctx->leaf_body.reserve(n);
         ctx->leaf_moved.reserve(n);
+        ctx->moved_list.reserve(n);
         for (auto *b : {&ctx->range_first, &ctx->range_last, &ctx->lastf}) b->reserve(n);
         ctx->span_box.reserve(4 * static_cast<size_t>(n));
         ctx->span_tab.reserve(2 * static_cast<size_t>(refit_levels(n)) * refit_chunks(n));

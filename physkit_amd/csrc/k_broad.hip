@@ -186,14 +186,10 @@ __global__ void k_build_internal(const uint64_t *k, int n, lbvh_dev t)
     if (i == 0) t.parent[0] = kNone;
 }
 
-__global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, lbvh_dev t)
+__device__ __forceinline__ void gather_leaf(const bodies_dev &b, uint32_t a, uint32_t i, bool mv, const lbvh_dev &t)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = t.n;
-    if (i >= n) return;
-    uint32_t a = ids[i];
     t.leaf_body[i] = a;
-    t.leaf_moved[i] = (!b.is_static[a] && (all_moved || b.moved[a])) ? 1 : 0;
+    t.leaf_moved[i] = mv ? 1 : 0;
     // the body's 48-byte fat record, copied to the leaf's as three 16-byte moves
     const double2 *src = reinterpret_cast<const double2 *>(b.fat + 6 * static_cast<size_t>(a));
     double2 *dst = reinterpret_cast<double2 *>(t.leaf_box + 6 * static_cast<size_t>(i));
@@ -207,7 +203,25 @@ __global__ void k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, 
         t.leaf_f32[2 * static_cast<size_t>(i)] = make_float4(f_down(f[0]), f_down(f[1]), f_down(f[2]), 0.0f);
         t.leaf_f32[2 * static_cast<size_t>(i) + 1] = make_float4(f_up(f[3]), f_up(f[4]), f_up(f[5]), 0.0f);
     }
-    if (n == 1) t.parent[0] = kNone;
+    if (t.n == 1) t.parent[0] = kNone;
+}
+
+// Leaf i's body, moved flag and boxes; the moved leaves are listed for the pair query
+// (in leaf order within the workgroup, so a query wavefront walks neighbouring leaves)
+__global__ void __launch_bounds__(kBlock) k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, lbvh_dev t,
+                                                        unsigned long long *counters)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = t.n;
+    bool mv = false;
+    if (i < n)
+    {
+        const uint32_t a = ids[i];
+        mv = !b.is_static[a] && (all_moved || b.moved[a]);
+        gather_leaf(b, a, i, mv, t);
+    }
+    const unsigned long long at = block_append(&counters[C_MLEAF], mv);
+    if (mv && at < n) t.moved_list[at] = i; // the launchers clear the count first: at < n
 }
 
 // next node in left-first depth-first order after the subtree of u
@@ -529,8 +543,10 @@ __global__ void __launch_bounds__(kBlock)
     unsigned long long *cursor = counters + C_QCUR + region * kCursorStride;
     uint64_t buf[kEmitBuf];
     uint32_t nbuf = 0;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = t.n;
+    // thread k walks for the k-th listed (moved) leaf, sorted index i
+    const uint32_t n = t.n;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = k < static_cast<uint32_t>(counters[C_MLEAF]) ? t.moved_list[k] : n;
     bool active = false;
     uint32_t a = 0;
     box3 q{};
@@ -746,7 +762,8 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
         t.ids_alt = ti;
     }
     if (n >= 2) k_build_internal<<<grid(n - 1), kBlock, 0, st>>>(t.morton, static_cast<int>(n), t);
-    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    launch_fill_u64(counters + C_MLEAF, 0ull, 1, st); // the moved list starts empty
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t, counters);
     k_escape<<<grid(2 * n - 1), kBlock, 0, st>>>(t);
     if (n >= 2)
     {
@@ -761,12 +778,14 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
 
 // The last build's leaf order and hierarchy with this step's leaf boxes and moved flags:
 // the pair set and query answers do not depend on the tree's shape, only the walks' cost.
-void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, hipStream_t st)
+void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, unsigned long long *counters,
+                       hipStream_t st)
 {
     const uint32_t n = b.n;
     t.n = n;
     if (n == 0) return;
-    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    launch_fill_u64(counters + C_MLEAF, 0ull, 1, st); // the moved list starts empty
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t, counters);
     if (n >= 2)
     {
         k_refit<<<t.n_chunks, kRefitThreads, 0, st>>>(t);

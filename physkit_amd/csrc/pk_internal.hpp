@@ -95,6 +95,7 @@ enum counter_slot
     C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
     C_MSLOTS = 19,    // manifold data slots taken by the manifold update
     C_MLIVE = 20,     // pairs the manifold update lists (a GJK hit or contacts before)
+    C_MLEAF = 21,     // leaves listed for the pair query (dynamic and moved)
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
     // addresses instead of serialising on one
@@ -153,6 +154,8 @@ struct lbvh_dev
                          // tested before the exact bounds in the pair query
     uint32_t *leaf_body; // n: body id of leaf i
     uint8_t *leaf_moved; // n: leaf queries this step (dynamic and moved)
+    uint32_t *moved_list; // n: the querying leaves' sorted indices (count at counters[C_MLEAF]),
+                          // in order within each workgroup of the gather
     // refit chunks of kRefitChunk leaves: per leaf, the union of its chunk's leaf boxes
     // up to it and from it on (4n: prefix lo, prefix hi, suffix lo, suffix hi; w of lo =
     // every-leaf-moved flag), and a sparse table over chunk totals (2 * levels * chunks)
@@ -183,7 +186,8 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
                        unsigned long long *counters, hipStream_t st);
 // the last build's hierarchy refitted to the current leaf boxes (no Morton codes, sort
 // or topology); the same ids, node count and world layout as that build
-void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, hipStream_t st);
+void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, unsigned long long *counters,
+                       hipStream_t st);
 // candidate keys (pairs with at least one moved leaf), appended per region (below);
 // a malformed tree is reported in counters[C_TREE_FAIL]
 // workgroup w writes its candidates to region r = w % n_regions: out_keys[r * rcap, ...)
