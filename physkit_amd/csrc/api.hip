@@ -198,7 +198,7 @@ struct pkg_ctx
     dbuf<double> leaf_box;
     dbuf<float4> leaf_f32; // with PKG_FLAG_FP32_BROAD
     dbuf<uint8_t> leaf_moved;
-    dbuf<uint32_t> moved_list;
+    dbuf<uint32_t> moved_list, moved_wg;
     dbuf<unsigned long long> scene;
     sort_scratch sc;
     // batched worlds
@@ -451,6 +451,7 @@ lbvh_dev pkg_ctx::tree()
     t.leaf_body = leaf_body.p;
     t.leaf_moved = leaf_moved.p;
     t.moved_list = moved_list.p;
+    t.moved_wg = moved_wg.p;
     t.n_chunks = refit_chunks(n);
     t.span_levels = refit_levels(n);
     t.span_box = span_box.p;
@@ -1282,6 +1283,7 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->leaf_body.reserve(n);
         ctx->leaf_moved.reserve(n);
         ctx->moved_list.reserve(n);
+        ctx->moved_wg.reserve(2 * ((static_cast<size_t>(n) + 255) / 256));
         for (auto *b : {&ctx->range_first, &ctx->range_last, &ctx->lastf}) b->reserve(n);
         ctx->span_box.reserve(4 * static_cast<size_t>(n));
         ctx->span_tab.reserve(2 * static_cast<size_t>(refit_levels(n)) * refit_chunks(n));

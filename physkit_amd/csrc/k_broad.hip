@@ -206,10 +206,12 @@ __device__ __forceinline__ void gather_leaf(const bodies_dev &b, uint32_t a, uin
     if (t.n == 1) t.parent[0] = kNone;
 }
 
-// Leaf i's body, moved flag and boxes; the moved leaves are listed for the pair query
-// (in leaf order within the workgroup, so a query wavefront walks neighbouring leaves)
-__global__ void __launch_bounds__(kBlock) k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, lbvh_dev t,
-                                                        unsigned long long *counters)
+// Leaf i's body, moved flag and boxes, and the moved leaves per 256-leaf block. The
+// moved leaves are then listed for the pair query in ascending leaf order (offsets of
+// the block counts, then a scatter), so the query's concurrent wavefronts walk one
+// contiguous range of leaves and share the tree nodes above it in L2.
+static_assert(kBlock == 256, "the moved-leaf blocks are the gather's workgroups");
+__global__ void __launch_bounds__(kBlock) k_leaf_gather(bodies_dev b, const uint32_t *ids, int all_moved, lbvh_dev t)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = t.n;
@@ -220,8 +222,52 @@ __global__ void __launch_bounds__(kBlock) k_leaf_gather(bodies_dev b, const uint
         mv = !b.is_static[a] && (all_moved || b.moved[a]);
         gather_leaf(b, a, i, mv, t);
     }
-    const unsigned long long at = block_append(&counters[C_MLEAF], mv);
-    if (mv && at < n) t.moved_list[at] = i; // the launchers clear the count first: at < n
+    const int cnt = __syncthreads_count(mv);
+    if (threadIdx.x == 0) t.moved_wg[blockIdx.x] = static_cast<uint32_t>(cnt);
+}
+
+// one workgroup: exclusive offsets of the n_blk block counts (moved_wg[n_blk + b]) and
+// the total at counters[C_MLEAF]
+__global__ void __launch_bounds__(1024) k_moved_offsets(lbvh_dev t, uint32_t n_blk, unsigned long long *counters)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (n_blk + 1023) / 1024, b0 = min(threadIdx.x * per, n_blk), b1 = min(b0 + per, n_blk);
+    uint32_t s = 0;
+    for (uint32_t k = b0; k < b1; ++k) s += t.moved_wg[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) // inclusive scan of the thread sums
+    {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t k = b0; k < b1; ++k)
+    {
+        const uint32_t c = t.moved_wg[k];
+        t.moved_wg[n_blk + k] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) counters[C_MLEAF] = part[1023];
+}
+
+// moved leaf i at its block's offset plus its rank in the block
+__global__ void __launch_bounds__(kBlock) k_moved_scatter(lbvh_dev t, uint32_t n_blk)
+{
+    __shared__ uint32_t wcnt[kBlock / 64];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool mv = i < t.n && t.leaf_moved[i];
+    const unsigned long long mask = __ballot(mv);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) wcnt[wave] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    if (!mv) return;
+    uint32_t at = t.moved_wg[n_blk + blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) at += wcnt[w];
+    at += static_cast<uint32_t>(__popcll(mask & ((1ull << __lane_id()) - 1ull)));
+    t.moved_list[at] = i;
 }
 
 // next node in left-first depth-first order after the subtree of u
@@ -543,10 +589,16 @@ __global__ void __launch_bounds__(kBlock)
     unsigned long long *cursor = counters + C_QCUR + region * kCursorStride;
     uint64_t buf[kEmitBuf];
     uint32_t nbuf = 0;
-    // thread k walks for the k-th listed (moved) leaf, sorted index i
+    // Thread k walks for the k-th moved leaf (sorted index i). Workgroups go to the 8
+    // XCDs in turn, so XCD x's workgroups take the x-th eighth of the list: each XCD's
+    // L2 serves one contiguous range of leaves. Workgroups past the list idle.
     const uint32_t n = t.n;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = k < static_cast<uint32_t>(counters[C_MLEAF]) ? t.moved_list[k] : n;
+    const uint32_t n_moved = static_cast<uint32_t>(counters[C_MLEAF]);
+    const uint32_t g_active = (n_moved + blockDim.x - 1) / blockDim.x;
+    const uint32_t per = (g_active + 7) / 8, xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+    const uint32_t wg = local < per ? xcd * per + local : g_active;
+    const uint32_t k = wg * blockDim.x + threadIdx.x;
+    const uint32_t i = k < n_moved ? t.moved_list[k] : n;
     bool active = false;
     uint32_t a = 0;
     box3 q{};
@@ -762,8 +814,9 @@ void launch_lbvh_build(lbvh_dev &t, const bodies_dev &b, bool all_moved, sort_sc
         t.ids_alt = ti;
     }
     if (n >= 2) k_build_internal<<<grid(n - 1), kBlock, 0, st>>>(t.morton, static_cast<int>(n), t);
-    launch_fill_u64(counters + C_MLEAF, 0ull, 1, st); // the moved list starts empty
-    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t, counters);
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    k_moved_offsets<<<1, 1024, 0, st>>>(t, grid(n), counters);
+    k_moved_scatter<<<grid(n), kBlock, 0, st>>>(t, grid(n));
     k_escape<<<grid(2 * n - 1), kBlock, 0, st>>>(t);
     if (n >= 2)
     {
@@ -784,8 +837,9 @@ void launch_lbvh_refit(lbvh_dev &t, const bodies_dev &b, bool all_moved, unsigne
     const uint32_t n = b.n;
     t.n = n;
     if (n == 0) return;
-    launch_fill_u64(counters + C_MLEAF, 0ull, 1, st); // the moved list starts empty
-    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t, counters);
+    k_leaf_gather<<<grid(n), kBlock, 0, st>>>(b, t.ids, all_moved ? 1 : 0, t);
+    k_moved_offsets<<<1, 1024, 0, st>>>(t, grid(n), counters);
+    k_moved_scatter<<<grid(n), kBlock, 0, st>>>(t, grid(n));
     if (n >= 2)
     {
         k_refit<<<t.n_chunks, kRefitThreads, 0, st>>>(t);
@@ -817,7 +871,8 @@ void launch_pair_query(const lbvh_dev &t, const bodies_dev &b, uint64_t *out_key
                        uint32_t n_regions, unsigned long long *counters, hipStream_t st)
 {
     if (t.n < 2) return;
-    k_pair_query<<<grid(t.n), kBlock, 0, st>>>(t, b, out_keys, cap, n_regions, counters);
+    // a multiple of 8 workgroups, so every XCD's eighth of the list has its workgroups
+    k_pair_query<<<(grid(t.n) + 7) / 8 * 8, kBlock, 0, st>>>(t, b, out_keys, cap, n_regions, counters);
 }
 
 void launch_pair_match(const uint64_t *cand, uint32_t m, const uint64_t *prev, uint32_t n_prev, uint32_t *src,

@@ -154,8 +154,8 @@ struct lbvh_dev
                          // tested before the exact bounds in the pair query
     uint32_t *leaf_body; // n: body id of leaf i
     uint8_t *leaf_moved; // n: leaf queries this step (dynamic and moved)
-    uint32_t *moved_list; // n: the querying leaves' sorted indices (count at counters[C_MLEAF]),
-                          // in order within each workgroup of the gather
+    uint32_t *moved_list; // n: the querying leaves' sorted indices, ascending (count at counters[C_MLEAF])
+    uint32_t *moved_wg;   // 2 * ceil(n / 256): moved leaves per 256-leaf block, then their exclusive offsets
     // refit chunks of kRefitChunk leaves: per leaf, the union of its chunk's leaf boxes
     // up to it and from it on (4n: prefix lo, prefix hi, suffix lo, suffix hi; w of lo =
     // every-leaf-moved flag), and a sparse table over chunk totals (2 * levels * chunks)
