@@ -589,15 +589,10 @@ __global__ void __launch_bounds__(kBlock)
     unsigned long long *cursor = counters + C_QCUR + region * kCursorStride;
     uint64_t buf[kEmitBuf];
     uint32_t nbuf = 0;
-    // Thread k walks for the k-th moved leaf (sorted index i). Workgroups go to the 8
-    // XCDs in turn, so XCD x's workgroups take the x-th eighth of the list: each XCD's
-    // L2 serves one contiguous range of leaves. Workgroups past the list idle.
+    // thread k walks for the k-th moved leaf (sorted index i), list blocks in XCD order
     const uint32_t n = t.n;
     const uint32_t n_moved = static_cast<uint32_t>(counters[C_MLEAF]);
-    const uint32_t g_active = (n_moved + blockDim.x - 1) / blockDim.x;
-    const uint32_t per = (g_active + 7) / 8, xcd = blockIdx.x % 8, local = blockIdx.x / 8;
-    const uint32_t wg = local < per ? xcd * per + local : g_active;
-    const uint32_t k = wg * blockDim.x + threadIdx.x;
+    const uint32_t k = xcd_block(n_moved, blockDim.x) * blockDim.x + threadIdx.x;
     const uint32_t i = k < n_moved ? t.moved_list[k] : n;
     bool active = false;
     uint32_t a = 0;

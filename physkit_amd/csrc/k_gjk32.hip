@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kBlock)
     // the same number of rounds on every lane (each round's append is workgroup-wide)
     for (uint64_t base = 0; base < n_list; base += threads)
     {
-        const uint64_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t i = base + static_cast<uint64_t>(xcd_block(n_list, blockDim.x)) * blockDim.x + threadIdx.x;
         bool pass = false;
         uint32_t p = 0;
         if (i < n_list)
@@ -121,7 +121,7 @@ void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, 
                   uint32_t *list2, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    const unsigned gr = std::min<unsigned>(static_cast<unsigned>((n_pairs + kBlock - 1) / kBlock), kListGrid);
+    const unsigned gr = list_grid(n_pairs);
     if (uv)
         k_gjk32<true><<<gr, kBlock, 0, st>>>(keys, list, b, s, src, prev_count, status, list2, counters);
     else

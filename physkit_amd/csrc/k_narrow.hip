@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t base = 0; base < n_groups; base += threads)
     {
-        const uint64_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t q = base + static_cast<uint64_t>(xcd_block(n_groups, blockDim.x)) * blockDim.x + threadIdx.x;
         const uint32_t p0 = static_cast<uint32_t>(Q * q);
         uint32_t n_here = 0, keepmask = 0;
         uint64_t key[Q] = {};
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(kBlock, 3) // 3 waves per SIMD fit without spi
           unsigned long long *counters)
 {
     const uint32_t n_list = static_cast<uint32_t>(counters[list_slot]);
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = xcd_block(static_cast<uint64_t>(n_list) * G, blockDim.x) * blockDim.x + threadIdx.x;
     const int gl = static_cast<int>(t % G);
     // groups step through the list together: the same number of rounds on every lane,
     // so each round's workgroup-wide append sees all of them
@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t base = 0; base < n_quads; base += threads)
     {
-        const uint64_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t q = base + static_cast<uint64_t>(xcd_block(n_quads, blockDim.x)) * blockDim.x + threadIdx.x;
         const uint32_t p0 = static_cast<uint32_t>(4 * q);
         uint32_t livemask = 0;
         if (q < n_quads)
@@ -637,7 +637,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t base = 0; base < n_live; base += threads)
     {
-        const uint64_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+        const uint64_t i = base + static_cast<uint64_t>(xcd_block(n_live, blockDim.x)) * blockDim.x + threadIdx.x;
         const bool live = i < n_live;
         const uint32_t p = live ? live_list[i] : 0u;
         uint32_t count = 0;
@@ -863,11 +863,11 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
         // the mixed form adds the FP32 sphere records after the FP64 ones (16-byte aligned)
         float4 *rec32 = f32.s32 ? reinterpret_cast<float4 *>(cull_rec + static_cast<size_t>(kCullRec) * b.n) : nullptr;
         k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec, rec32);
-        k_gjk_cull<true><<<std::min<unsigned>(grid((n_pairs + 3) / 4), kListGrid), kBlock, 0, st>>>(
+        k_gjk_cull<true><<<list_grid((n_pairs + 3) / 4), kBlock, 0, st>>>(
             keys, n_pairs, b, s, cull_rec, rec32, g.status, list, counters);
     }
     else
-        k_gjk_cull<false><<<std::min<unsigned>(grid(n_pairs), kListGrid), kBlock, 0, st>>>(
+        k_gjk_cull<false><<<list_grid(n_pairs), kBlock, 0, st>>>(
             keys, n_pairs, b, s, cull_rec, nullptr, g.status, list, counters);
     int slot = C_GJK_LIST;
     if (f32.s32)
@@ -878,7 +878,7 @@ void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, con
     }
     // sized for every pair up to kListGrid workgroups, which then step through the list
     // (its count is read on the device); groups past the listed count exit at once
-    const unsigned gr = std::min<unsigned>(grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group)), kListGrid);
+    const unsigned gr = list_grid(static_cast<size_t>(n_pairs) * static_cast<size_t>(group));
     auto go = [&](auto lanes)
     {
         constexpr int G = decltype(lanes)::value;
@@ -950,12 +950,12 @@ void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t 
                             unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    k_manifold_live<<<std::min<unsigned>(grid((n_pairs + 3) / 4), kListGrid), kBlock, 0, st>>>(src, n_pairs, g.status,
+    k_manifold_live<<<list_grid((n_pairs + 3) / 4), kBlock, 0, st>>>(src, n_pairs, g.status,
                                                                                       prev.count, next, events,
                                                                                       live_list, counters);
     const uint32_t bound = std::min(live_bound, n_pairs);
     if (bound)
-        k_manifold<<<std::min<unsigned>(grid(bound), kListGrid), kBlock, 0, st>>>(
+        k_manifold<<<list_grid(bound), kBlock, 0, st>>>(
             keys, src, live_list, b, g, prev, next, events, epa_guard ? 1 : 0, counters);
 }
 

@@ -122,6 +122,20 @@ __device__ __forceinline__ unsigned long long block_reserve(unsigned long long *
     return base + (incl - v);
 }
 
+// XCD-aware order of a list's blocks: MI355X hands workgroups to its 8 XCDs in turn
+// (workgroup b to XCD b mod 8), so workgroup b takes list block (b mod 8) * per + b / 8,
+// per = ceil(active / 8), where active = the workgroups a round of the list fills (at
+// most the grid). Each XCD's L2 then serves one contiguous eighth of the list. A
+// workgroup past the list gets block `active` (no items). The grid must be a multiple
+// of 8; when the list fills every workgroup the order is a permutation of the grid.
+__device__ __forceinline__ uint32_t xcd_block(uint64_t n_items, uint32_t items_per_wg)
+{
+    const uint64_t need = (n_items + items_per_wg - 1) / items_per_wg;
+    const uint32_t active = need < gridDim.x ? static_cast<uint32_t>(need) : gridDim.x;
+    const uint32_t per = (active + 7) / 8, xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+    return local < per ? xcd * per + local : active;
+}
+
 // lanes with `pred` get consecutive slots; returns this lane's (undefined when !pred)
 __device__ __forceinline__ unsigned long long block_append(unsigned long long *counter, bool pred)
 {

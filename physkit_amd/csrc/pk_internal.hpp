@@ -6,6 +6,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 
@@ -251,6 +252,13 @@ void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, 
 // list kernels (GJK passes over the cull's list, whose count only the device knows)
 // launch at most this many workgroups, which step through longer lists
 constexpr unsigned kListGrid = 16384;
+// workgroups of 256 for n items, rounded up to a multiple of 8 (one per XCD in each
+// turn of the dispatcher) and capped at kListGrid; the kernels step through the rest
+inline unsigned list_grid(size_t n)
+{
+    const size_t g = (n + 255) / 256;
+    return static_cast<unsigned>(std::min<size_t>((g + 7) / 8 * 8, kListGrid));
+}
 struct gjk32_in
 {
     const shapes32_dev *s32; // null: no FP32 pass
