@@ -140,14 +140,19 @@ def step_bytes(s, c, unique_vertices, scalar=8):
             + 10 * scalar * c["gjk_hits"] + c["contacts"] * (20 * scalar + 48))
 
 
-def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, integrate=True):
+def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, integrate=True, steps=1):
     """Compulsory HBM bytes of each stage's launches in one step, with the step's own
     counts (n bodies, N dynamic, M moved leaves, K candidate pairs, P pairs, H GJK hits,
-    C manifold contacts, R solver rows; s_r = 4 for FP32 rows, else 8):
-      integrate_vel     2n + 460 N + 48 M + 48 per shape (k_integrate_vel: mass, inverse
+    C manifold contacts, R solver rows; s_r = 4 for FP32 rows, else 8), averaged over
+    `steps` steps taken in one call:
+      integrate_vel     2n + 412 N + 48 M + 48 per shape (k_integrate_vel: mass, inverse
                         mass, force and torque accumulators, velocities, world inertia and
-                        its inverse, pose, leaf box read; velocities and accumulators
-                        written, the leaf box when it moves)
+                        its inverse, pose, leaf box read; velocities written, the
+                        accumulators only when set (none here), the leaf box when it
+                        moves) on a call's first step; 2n + 612 N + 48 M + 48 per shape
+                        on the others, whose first kernel also runs the previous step's
+                        integrate_pos (k_integrate_pos_vel: local inertia read; pose and
+                        world inertia written, and kept on chip instead of read back)
       lbvh_build        253 n (+32 n with FP32 leaf boxes) on a full build: leaf boxes
                         read, Morton keys and ids written and read, leaf records written,
                         2 internal nodes of 32 B written and read per leaf; 229 n on a refit
@@ -166,7 +171,7 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, i
       solve_iterations  R (38 s_r + 96): every row read once and its bodies' velocities
                         read and written once (rows that stay on chip across passes)
       integrate_pos     n + 448 N (k_integrate_pos: pose, velocities and local inertia
-                        read; pose and world inertia written)
+                        read; pose and world inertia written) on a call's last step only
     With integration off the first stage only clears the moved flags (n) and the last
     launches nothing."""
     n = len(s["shape"])
@@ -175,7 +180,8 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, i
     sr = 4 if mixed else 8
     M, K, P, H, C, R = (c[k] for k in ("moved", "candidates", "pairs", "gjk_hits", "contacts", "rows"))
     out = {
-        "integrate_vel": ("k_integrate_vel", 2 * n + 460 * nd + 48 * M + 48 * n_shapes),
+        "integrate_vel": ("k_integrate_vel, then k_integrate_pos_vel",
+                          ((2 * n + 412 * nd) + (steps - 1) * (2 * n + 612 * nd)) / steps + 48 * M + 48 * n_shapes),
         "lbvh_build": ("LBVH build or refit (k_morton, k_rs_*, k_build_internal, k_leaf_gather, k_refit, k_span)",
                        ((253 + 229 * tree_refits) / (1 + tree_refits) + (32 if mixed else 0)) * n),
         "pair_update": ("pair-set update (k_pair_query, sort, k_match, k_prev_*, k_merge)",
@@ -186,7 +192,7 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, i
         "manifold": ("k_manifold", 22 * P + 208 * C + 192 * H),
         "solver": ("contact solver stage", 496 * R),
         "solve_iterations": ("k_solve_islands" + ("<true> (FP32 rows)" if mixed else "<false>"), R * (38 * sr + 96)),
-        "integrate_pos": ("k_integrate_pos", n + 448 * nd),
+        "integrate_pos": ("k_integrate_pos", (n + 448 * nd) / steps),
     }
     if not integrate:
         out["integrate_vel"] = ("k_fill_u32 (moved flags cleared; integration off)", n)
@@ -283,8 +289,8 @@ def tuned_shapes(pk, s, flags, warmup):
 
 def timed_run(pk, s, flags, warmup, steps, autotune=True, fixed=None, tree_refits=0):
     """Loads s into a context with flags, runs warmup untimed steps, then `steps` timed
-    steps (each pkg_step_timed call returns after its device work has finished, so the
-    timed region is bracketed by device synchronisation), with the launch shapes
+    steps in one pkg_step_timed call (it returns after its device work has finished, so
+    the timed region is bracketed by device synchronisation), with the launch shapes
     pkg_autotune picked on a copy of the world (tuned_shapes), the `fixed` ones
     (gjk_lanes, epa_block, epa_lds_max) or the defaults, and tree_refits LBVH refits
     between full builds. Returns wall seconds, summed
@@ -304,17 +310,16 @@ def timed_run(pk, s, flags, warmup, steps, autotune=True, fixed=None, tree_refit
     tuning = dict(tuning, tree_refits=tree_refits)
     for _ in range(warmup):
         g.step(s["dt"])
-    stage = {k: 0.0 for k in pk.GpuWorld.STAGES}
-    pairs_total = hits_total = 0
+    c0 = g.counters()
+    # the timed steps in one call (each step's integrate_pos runs fused with the next
+    # step's integrate_vel; the call returns with every step integrated and its device
+    # work finished)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        ms = g.step_timed(s["dt"], 1)
-        for k, v in ms.items():
-            stage[k] += v
-        c = g.counters()
-        pairs_total += c["pairs"]
-        hits_total += c["gjk_hits"]
+    stage = g.step_timed(s["dt"], steps)
     wall = time.perf_counter() - t0
+    c = g.counters()
+    pairs_total = c["pairs_total"] - c0["pairs_total"]
+    hits_total = c["gjk_hits_total"] - c0["gjk_hits_total"]
     out = {"wall": wall, "stage": stage, "pairs_total": pairs_total, "hits_total": hits_total,
            "counters": g.counters(), "pairs": g.pairs(), "pos": g.bodies()["pos"], "tuning": tuning}
     g.close()
@@ -589,7 +594,8 @@ def main():
     print(f"[bench] timed: {wall * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     hits_frac = c["gjk_hits"] / max(c["pairs"], 1)
     models = stage_bytes(s, c, pairs, hits_frac, r["pos"], s["iterations"], mixed or prec == "fp32-solve",
-                         args.tree_refits, integrate=(int(s["flags"]) & pk.FLAG_NO_INTEGRATE) == 0)
+                         args.tree_refits, integrate=(int(s["flags"]) & pk.FLAG_NO_INTEGRATE) == 0,
+                         steps=args.steps)
     dom, table = roofline_of(stage, args.steps, models)
     d = table[dom]
     roof = {"bound": "hbm", "stage": dom, "kernel": d["kernel"], "achieved": d["achieved_GBs"],

@@ -106,7 +106,8 @@ typedef struct pkg_body_state
     uint8_t *moved; /* n: leaf reinserted in the last step */
 } pkg_body_state;
 
-/* Counters of the last step. */
+/* Counters of the last step, and running totals since the bodies were uploaded (a
+ * caller stepping many steps in one pkg_step call reads the pairs it went through). */
 typedef struct pkg_counters
 {
     uint64_t moved;    /* leaves reinserted (dynamic_bvh::update_leaf returned true) */
@@ -120,6 +121,9 @@ typedef struct pkg_counters
     uint64_t gjk_listed;   /* pairs the bounding-volume cull left to the GJK stage   */
     uint64_t gjk_fp64;     /* of those, pairs the FP64 GJK ran (all of them unless
                               PKG_FLAG_FP32_GJK certified the others apart)          */
+    uint64_t steps_total;    /* steps since pkg_upload_bodies                          */
+    uint64_t pairs_total;    /* the pairs of those steps, summed                       */
+    uint64_t gjk_hits_total; /* their GJK hits, summed                                 */
 } pkg_counters;
 
 /* Replaces constructing a physkit::world (world.h:382-385). */

@@ -76,7 +76,7 @@ class pkg_body_state(C.Structure):
 class pkg_counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("moved", "pairs", "gjk_hits", "contacts", "rows",
                                           "colours", "epa_overflow", "candidates", "gjk_listed",
-                                          "gjk_fp64")]
+                                          "gjk_fp64", "steps_total", "pairs_total", "gjk_hits_total")]
 
 
 class pkg_tuning(C.Structure):

@@ -415,7 +415,11 @@ struct pkg_ctx
     }
     void reset_counter(int slot) { launch_fill_u64(counters.p + slot, 0ull, 1, st); }
 
-    void step_once(double dt, hipEvent_t *ev);
+    // defer_pos: another step follows in the same call, so this step's integrate_pos is
+    // left to the next step's first kernel (pos_pending, pending_dt)
+    void step_once(double dt, hipEvent_t *ev, bool defer_pos = false);
+    bool pos_pending = false;
+    double pending_dt = 0.0;
 };
 
 namespace
@@ -599,7 +603,7 @@ query_scratch pkg_ctx::qscratch(uint32_t m)
     return qs;
 }
 
-void pkg_ctx::step_once(double dt, hipEvent_t *ev)
+void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
 {
     const bool integrate = (cfg.flags & PKG_FLAG_NO_INTEGRATE) == 0;
     const bool solve = (cfg.flags & PKG_FLAG_NO_SOLVER) == 0;
@@ -610,7 +614,13 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     record(ev, 0, st);
 
     // ---- 1. gravity, integrate_vel, leaf update (world.cpp:22-34)
-    if (integrate)
+    if (integrate && pos_pending)
+    {
+        launch_integrate_pos_vel(b, s, pending_dt, cfg.gravity[0], cfg.gravity[1], cfg.gravity[2], dt, counters.p,
+                                 st);
+        pos_pending = false;
+    }
+    else if (integrate)
         launch_integrate_vel(b, s, cfg.gravity[0], cfg.gravity[1], cfg.gravity[2], dt, counters.p, st);
     else
         launch_fill_u32(reinterpret_cast<uint32_t *>(moved.p), 0u, (n + 3) / 4, st);
@@ -1048,7 +1058,13 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     record(ev, 7, st);
 
     // ---- 5. integrate_pos (world.cpp:50-55)
-    if (integrate) launch_integrate_pos(b, dt, st);
+    if (integrate && defer_pos)
+    {
+        pos_pending = true; // the next step_once of this call runs it first
+        pending_dt = dt;
+    }
+    else if (integrate)
+        launch_integrate_pos(b, dt, st);
     record(ev, 8, st);
 
     P[nxt] = Pn;
@@ -1067,6 +1083,9 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev)
     mused[cur] = Pn ? static_cast<uint32_t>(h_counters[C_MSLOTS]) : 0u;
     last.gjk_listed = Pn ? h_counters[C_GJK_LIST] : 0;
     last.gjk_fp64 = Pn ? ((cfg.flags & PKG_FLAG_FP32_GJK) ? h_counters[C_GJK_LIST2] : h_counters[C_GJK_LIST]) : 0;
+    last.steps_total += 1;
+    last.pairs_total += last.pairs;
+    last.gjk_hits_total += last.gjk_hits;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -1299,6 +1318,8 @@ pkg_status pkg_upload_bodies(pkg_ctx *ctx, uint32_t n, const pkg_body_desc *d)
         ctx->n_worlds = 1;
         ctx->have_bodies = true;
         ctx->poisoned = false;
+        ctx->pos_pending = false;
+        ctx->last = pkg_counters{};
         ctx->tree_ok = false;
         ctx->topo_ok = false;
         hip_check(hipStreamSynchronize(st), "body upload");
@@ -1698,7 +1719,7 @@ pkg_status pkg_step(pkg_ctx *ctx, double dt, int nsteps)
     const pkg_status r = [&]() -> pkg_status
     {
         PKG_GUARD(ctx, {
-            for (int i = 0; i < nsteps; ++i) ctx->step_once(dt, nullptr);
+            for (int i = 0; i < nsteps; ++i) ctx->step_once(dt, nullptr, i + 1 < nsteps);
             hip_check(hipGetLastError(), "step");
         })
     }();
@@ -1719,7 +1740,7 @@ pkg_status pkg_step_timed(pkg_ctx *ctx, double dt, int nsteps, double *stage_ms,
         double acc[kStages] = {};
         for (int i = 0; i < nsteps; ++i)
         {
-            ctx->step_once(dt, ev);
+            ctx->step_once(dt, ev, i + 1 < nsteps);
             hip_check(hipEventSynchronize(ev[8]), "event sync");
             for (int k = 0; k < kStages; ++k)
             {

@@ -66,29 +66,33 @@ __global__ void k_body_init(bodies_dev b, shapes_dev s) // particle.h:16-28, bvh
     b.moved[i] = 0;
 }
 
-// src/world.cpp:22-34: gravity force, integrate_vel (integrate.h:36-40,
-// particle.h:72-76), world bounds, dynamic_bvh::update_leaf (bvh.cpp:475-514),
-// clear_forces (particle.h:101-105)
-__global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy, double gz,
-                                double dt, unsigned long long *counters)
+// a force or torque accumulator cleared (clear_forces, particle.h:101-105); a store
+// only where a bit is set, so untouched accumulators cost no write
+__device__ __forceinline__ void clear3(double *p, uint32_t n, uint32_t i, v3 v)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (__double_as_longlong(v.x) | __double_as_longlong(v.y) | __double_as_longlong(v.z))
+        st3(p, n, i, mk(0.0, 0.0, 0.0));
+}
+
+// src/world.cpp:22-34 for dynamic body i: gravity force, integrate_vel (integrate.h:36-40,
+// particle.h:72-76), world bounds, dynamic_bvh::update_leaf (bvh.cpp:475-514),
+// clear_forces; from its velocities, pose and world inertia (read or just computed).
+// Returns whether the leaf moved.
+__device__ __forceinline__ bool integrate_vel_body(const bodies_dev &b, const shapes_dev &s, uint32_t i, v3 g,
+                                                   double dt, v3 vel, v3 w, qt q, v3 pos, const m3 &Iw, const m3 &Iwi)
+{
     bool moved = false;
-    if (i < b.n && !b.is_static[i])
     {
         double m = b.mass[i], im = b.inv_mass[i];
-        v3 force = mk(gx, gy, gz) * m;
-        v3 acc = ld3(b.acc, b.n, i) + force * im;
-        v3 vel = ld3(b.vel, b.n, i) + acc * dt;
-        v3 w = ld3(b.ang, b.n, i);
-        m3 Iw = ld9(b.Iw, b.n, i);
-        m3 Iwi = ld9(b.Iw_inv, b.n, i);
-        v3 aa = mul(Iwi, ld3(b.torque, b.n, i) - cross(w, mul(Iw, w)));
+        v3 force = g * m;
+        const v3 acc_in = ld3(b.acc, b.n, i), torque_in = ld3(b.torque, b.n, i);
+        v3 acc = acc_in + force * im;
+        vel = vel + acc * dt;
+        v3 aa = mul(Iwi, torque_in - cross(w, mul(Iw, w)));
         w = w + aa * dt;
         st3(b.vel, b.n, i, vel);
         st3(b.ang, b.n, i, w);
-        qt q = ldq(b.quat, b.n, i);
-        box3 tb = instance_bounds(ldbox(s.local_box, s.n, b.shape[i]), q, ld3(b.pos, b.n, i));
+        box3 tb = instance_bounds(ldbox(s.local_box, s.n, b.shape[i]), q, pos);
         box3 fat = ld_fat(b.fat, i);
         if (!box_contains(fat, tb))
         {
@@ -103,9 +107,20 @@ __global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy
             st_fat(b.fat, i, nb);
             moved = true;
         }
-        st3(b.acc, b.n, i, mk(0.0, 0.0, 0.0));
-        st3(b.torque, b.n, i, mk(0.0, 0.0, 0.0));
+        clear3(b.acc, b.n, i, acc_in);
+        clear3(b.torque, b.n, i, torque_in);
     }
+    return moved;
+}
+
+__global__ void k_integrate_vel(bodies_dev b, shapes_dev s, double gx, double gy, double gz,
+                                double dt, unsigned long long *counters)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (i < b.n && !b.is_static[i])
+        moved = integrate_vel_body(b, s, i, mk(gx, gy, gz), dt, ld3(b.vel, b.n, i), ld3(b.ang, b.n, i),
+                                   ldq(b.quat, b.n, i), ld3(b.pos, b.n, i), ld9(b.Iw, b.n, i), ld9(b.Iw_inv, b.n, i));
     if (i < b.n) b.moved[i] = moved ? 1 : 0;
     block_add(&counters[C_MOVED], moved ? 1u : 0u);
 }
@@ -119,6 +134,35 @@ __global__ void k_integrate_pos(bodies_dev b, double dt)
     qt q = qmul(exp_map(ld3(b.ang, b.n, i), dt), ldq(b.quat, b.n, i));
     stq(b.quat, b.n, i, q);
     derived(b, i, q);
+}
+
+// One step's integrate_pos (dt_prev) and the next step's integrate_vel (dt) in one pass
+// over the bodies, for consecutive steps with nothing between them: the same arithmetic
+// as k_integrate_pos then k_integrate_vel, with the new pose, velocities and world
+// inertia kept in registers instead of written and read back.
+__global__ void k_integrate_pos_vel(bodies_dev b, shapes_dev s, double dt_prev, double gx, double gy, double gz,
+                                    double dt, unsigned long long *counters)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (i < b.n && !b.is_static[i])
+    {
+        const v3 vel = ld3(b.vel, b.n, i), w = ld3(b.ang, b.n, i);
+        const v3 pos = ld3(b.pos, b.n, i) + vel * dt_prev;
+        st3(b.pos, b.n, i, pos);
+        const qt q = qmul(exp_map(w, dt_prev), ldq(b.quat, b.n, i));
+        stq(b.quat, b.n, i, q);
+        // derived(b, i, q), kept
+        const m3 R = rotation_matrix(q);
+        const m3 Rt = transpose(R);
+        const m3 Iw = mul(mul(R, ld9(b.I_local, b.n, i)), Rt);
+        const m3 Iwi = mul(mul(R, ld9(b.Iinv_local, b.n, i)), Rt);
+        st9(b.Iw, b.n, i, Iw);
+        st9(b.Iw_inv, b.n, i, Iwi);
+        moved = integrate_vel_body(b, s, i, mk(gx, gy, gz), dt, vel, w, q, pos, Iw, Iwi);
+    }
+    if (i < b.n) b.moved[i] = moved ? 1 : 0;
+    block_add(&counters[C_MOVED], moved ? 1u : 0u);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, size_t n)
@@ -147,6 +191,11 @@ void launch_integrate_vel(const bodies_dev &b, const shapes_dev &s, double gx, d
                           double dt, unsigned long long *counters, hipStream_t st)
 {
     if (b.n) k_integrate_vel<<<grid(b.n), kBlock, 0, st>>>(b, s, gx, gy, gz, dt, counters);
+}
+void launch_integrate_pos_vel(const bodies_dev &b, const shapes_dev &s, double dt_prev, double gx, double gy,
+                              double gz, double dt, unsigned long long *counters, hipStream_t st)
+{
+    if (b.n) k_integrate_pos_vel<<<grid(b.n), kBlock, 0, st>>>(b, s, dt_prev, gx, gy, gz, dt, counters);
 }
 void launch_integrate_pos(const bodies_dev &b, double dt, hipStream_t st)
 {

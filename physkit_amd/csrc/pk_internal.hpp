@@ -112,6 +112,10 @@ void launch_shape_boxes(const shapes_dev &s, hipStream_t st);
 void launch_integrate_vel(const bodies_dev &b, const shapes_dev &s, double gx, double gy, double gz,
                           double dt, unsigned long long *counters, hipStream_t st);
 void launch_integrate_pos(const bodies_dev &b, double dt, hipStream_t st);
+// integrate_pos with dt_prev, then integrate_vel with dt, in one pass (bit-identical to
+// the two launches)
+void launch_integrate_pos_vel(const bodies_dev &b, const shapes_dev &s, double dt_prev, double gx, double gy,
+                              double gz, double dt, unsigned long long *counters, hipStream_t st);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 
