@@ -161,7 +161,9 @@ pkg_status pkg_set_body_state(pkg_ctx *ctx, uint32_t count, const uint32_t *ids,
                               const double *quat);
 
 /* world::step_impl(dt) (src/world.cpp:20-56), nsteps times. Synchronous: returns
- * when the device results of the last step are final.
+ * when the device results of the last step are final. Within one call each step's
+ * position integration runs in the next step's first kernel (one pass over the
+ * bodies instead of two); the results are bit-identical to nsteps calls of one step.
  * A step that fails (any status but PKG_OK: PKG_OOM, PKG_OVERFLOW_*, PKG_DEVICE_ERROR)
  * leaves the bodies part-way through it (velocities may carry its gravity, leaf bounds
  * may be updated and their moved flags used up). The ctx then refuses pkg_step and
