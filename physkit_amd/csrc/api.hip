@@ -223,7 +223,7 @@ struct pkg_ctx
     dbuf<uint8_t> events;
 
     // narrowphase
-    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list, gjk_list2, hit_of;
+    dbuf<uint32_t> hit_list, ovf_list, ovf2_list, gjk_list, gjk_list2, hit_of, mlive;
     // launch shapes: GJK lanes per pair defaults to 2 when the bodies' shapes average at
     // most 14 support candidates (boxes, certified UV spheres, small hulls), else 4, until
     // pkg_autotune or pkg_set_tuning picks them
@@ -782,22 +782,31 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
         }
         // the GJK stage; it is idempotent once its three counters are cleared (they start
         // the step at zero)
+        // the cull also writes every pair's next manifold empty and lists the culled pairs
+        // that had contacts: at most the previous step's non-empty manifolds
+        const uint32_t live_cap = std::max<uint32_t>(mused[cur], 1);
+        mlive.reserve(live_cap);
+        cull_manifold_out mo{srcp, mcount[cur].p, MF(nxt), events.p, mlive.p, live_cap};
         auto gjk_stage = [&](int group, bool again)
         {
             if (again)
-                for (int c : {C_GJK_LIST, C_GJK_LIST2, C_HITS}) reset_counter(c);
-            launch_gjk(keys[nxt].p, Pn, b, s, has_uv, group, g, gjk_list.p, cull_rec.p, f32, counters.p, st);
+                for (int c : {C_GJK_LIST, C_GJK_LIST2, C_HITS, C_MLIVE}) reset_counter(c);
+            launch_gjk(keys[nxt].p, Pn, b, s, has_uv, group, g, gjk_list.p, cull_rec.p, f32, mo, counters.p, st);
         };
         gjk_stage(tune.gjk_group, false);
         record(ev, 4, st);
-        uint32_t hits = static_cast<uint32_t>(read_counter(C_HITS));
+        read_counters(C_HITS, C_MLIVE);
+        uint32_t hits = static_cast<uint32_t>(h_counters[C_HITS]);
+        if (h_counters[C_MLIVE] > live_cap)
+            throw pk_error(PKG_DEVICE_ERROR, "more culled pairs with contacts than manifolds with contacts");
         if (hits > g.cap)
         {
             // more hits than the hit lists hold: grow them and run the stage again
             size_hits(hits + hits / 4);
             gjk_stage(tune.gjk_group, true);
             record(ev, 4, st);
-            hits = static_cast<uint32_t>(read_counter(C_HITS));
+            read_counters(C_HITS, C_MLIVE);
+            hits = static_cast<uint32_t>(h_counters[C_HITS]);
             if (hits > g.cap) throw pk_error(PKG_DEVICE_ERROR, "GJK hit count changed between identical runs");
         }
         last_hits = hits;
@@ -878,9 +887,15 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
             mowner[nxt].reserve(c);
             mdcap[nxt] = c;
         }
-        // the GJK list (Pn entries) is free again: it lists the pairs whose manifolds change
-        launch_manifold_update(keys[nxt].p, srcp, Pn, dneed, b, g, MF(cur), MF(nxt), events.p,
-                               (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, gjk_list.p, counters.p, st);
+        // with per-body records the cull settled every culled pair's manifold: the update
+        // runs over the culled pairs that had contacts, then every pair GJK ran on;
+        // otherwise it lists its own pairs into the GJK list's buffer (free again)
+        const bool culled = cull_uses_records(Pn, n);
+        const uint32_t bound = culled ? static_cast<uint32_t>(h_counters[C_MLIVE] + h_counters[C_GJK_LIST])
+                                      : std::min(dneed, Pn);
+        launch_manifold_update(keys[nxt].p, srcp, Pn, bound, b, g, MF(cur), MF(nxt), events.p,
+                               (cfg.flags & PKG_FLAG_EPA_GUARD) != 0, culled, culled ? mlive.p : gjk_list.p,
+                               gjk_list.p, counters.p, st);
     }
     else
     {

@@ -239,17 +239,22 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-// A lane takes Q consecutive pairs: all are written separated (status 0) and those
+// A lane takes Q consecutive pairs. All are written separated (status 0); the pairs
 // whose bodies are not apart (above) are listed for k_gjk, which writes their status
 // again. With per-body records (many pairs per body, most culled by the FP32 spheres)
-// Q is 4: one 32-byte key load, one 4-byte status store and eight sphere loads in
-// flight per lane. From the planar arrays (few pairs per body, each test reading
-// several arrays) Q is 1. A capped grid steps through the pairs, the same number of
-// rounds on every lane.
+// Q is 4: one 32-byte key load, vector stores, and the sphere loads in flight together;
+// the cull then also writes every pair's next manifold empty (count 0, no slot, no
+// event) and lists the culled pairs that had contacts for the manifold update
+// (cull_manifold_out), which rewrites those and the GJK list's manifolds. From the
+// planar arrays (few pairs per body, each test reading several arrays) Q is 1 and the
+// manifold update lists its own pairs (k_manifold_live): there a source and count
+// gather in the cull would lengthen each lane's chain of dependent loads. A capped grid
+// steps through the pairs, the same number of rounds on every lane.
 template <bool REC>
 __global__ void __launch_bounds__(kBlock)
     k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec,
-               const float4 *rec32, uint8_t *status, uint32_t *list, unsigned long long *counters)
+               const float4 *rec32, uint8_t *status, uint32_t *list, cull_manifold_out mo,
+               unsigned long long *counters)
 {
     constexpr uint32_t Q = REC ? 4 : 1;
     const uint64_t n_groups = (static_cast<uint64_t>(n_pairs) + Q - 1) / Q;
@@ -258,11 +263,12 @@ __global__ void __launch_bounds__(kBlock)
     {
         const uint64_t q = base + static_cast<uint64_t>(xcd_block(n_groups, blockDim.x)) * blockDim.x + threadIdx.x;
         const uint32_t p0 = static_cast<uint32_t>(Q * q);
-        uint32_t n_here = 0, keepmask = 0;
+        uint32_t n_here = 0, keepmask = 0, prevmask = 0;
         uint64_t key[Q] = {};
         if (q < n_groups)
         {
             n_here = min(Q, n_pairs - p0);
+            int pc[Q] = {};
             if constexpr (Q == 4)
             {
                 if (n_here == 4)
@@ -273,20 +279,45 @@ __global__ void __launch_bounds__(kBlock)
                     key[1] = k01.y;
                     key[2] = k23.x;
                     key[3] = k23.y;
-                    *reinterpret_cast<uint32_t *>(status + p0) = 0u;
-                }
-                else
-                    for (uint32_t k = 0; k < n_here; ++k)
+                    if (mo.src)
                     {
-                        key[k] = keys[p0 + k];
-                        status[p0 + k] = 0;
+                        const uint4 s4 = *reinterpret_cast<const uint4 *>(mo.src + p0);
+                        const uint32_t sp[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) pc[k] = sp[k] != kNone ? mo.prev_count[sp[k]] : 0;
                     }
+                    else
+                    {
+                        // the previous list kept whole: the previous counts at the same indices
+                        const int4 c4 = *reinterpret_cast<const int4 *>(mo.prev_count + p0);
+                        pc[0] = c4.x;
+                        pc[1] = c4.y;
+                        pc[2] = c4.z;
+                        pc[3] = c4.w;
+                    }
+                    *reinterpret_cast<uint32_t *>(status + p0) = 0u;
+                    *reinterpret_cast<int4 *>(mo.next.count + p0) = make_int4(0, 0, 0, 0);
+                    *reinterpret_cast<uint4 *>(mo.next.slot + p0) = make_uint4(kNone, kNone, kNone, kNone);
+                    *reinterpret_cast<uint32_t *>(mo.events + p0) = 0u;
+                }
             }
-            else
-            {
-                key[0] = keys[p0];
-                status[p0] = 0;
-            }
+            if (Q != 4 || n_here != 4)
+                for (uint32_t k = 0; k < n_here; ++k)
+                {
+                    const uint32_t p = p0 + k;
+                    key[k] = keys[p];
+                    status[p] = 0;
+                    if constexpr (REC)
+                    {
+                        const uint32_t sp = source_of(mo.src, p);
+                        pc[k] = sp != kNone ? mo.prev_count[sp] : 0;
+                        mo.next.count[p] = 0;
+                        mo.next.slot[p] = kNone;
+                        mo.events[p] = 0;
+                    }
+                }
+            for (uint32_t k = 0; k < Q; ++k)
+                if (k < n_here && pc[k] > 0) prevmask |= 1u << k;
             for (uint32_t k = n_here; k < Q; ++k) key[k] = key[0]; // a valid pair, never listed
             if (REC && rec32)
             {
@@ -319,13 +350,26 @@ __global__ void __launch_bounds__(kBlock)
                     if (apart) keepmask &= ~(1u << k);
                 }
         }
-        const uint32_t nk = static_cast<uint32_t>(__popc(keepmask));
-        // a workgroup with nothing to list skips the append; the barrier also separates
-        // the rounds' appends
-        if (!__syncthreads_or(nk != 0)) continue;
-        unsigned long long at = block_reserve(&counters[C_GJK_LIST], nk);
-        for (uint32_t k = 0; k < Q; ++k)
-            if (keepmask & (1u << k)) list[at++] = p0 + k;
+        const uint32_t livemask = prevmask & ~keepmask;
+        const uint32_t nk = static_cast<uint32_t>(__popc(keepmask)), nl = static_cast<uint32_t>(__popc(livemask));
+        // workgroup-wide appends only where something is listed; each barrier also
+        // separates the appends' shared LDS from the one before
+        if (__syncthreads_or(nk != 0))
+        {
+            unsigned long long at = block_reserve(&counters[C_GJK_LIST], nk);
+            for (uint32_t k = 0; k < Q; ++k)
+                if (keepmask & (1u << k)) list[at++] = p0 + k;
+        }
+        if (REC && __syncthreads_or(nl != 0))
+        {
+            unsigned long long at = block_reserve(&counters[C_MLIVE], nl);
+            for (uint32_t k = 0; k < Q; ++k)
+                if (livemask & (1u << k))
+                {
+                    if (at < mo.live_cap) mo.live_list[at] = p0 + k; // at most the previous non-empty manifolds
+                    ++at;
+                }
+        }
     }
 }
 
@@ -560,9 +604,9 @@ __device__ __forceinline__ void add_contact(mani &m, const contact &k) // :124-1
         add_reduce(m, k);
 }
 
-// Every pair, four consecutive ones per lane: all four are written empty (no contacts,
-// no slot, no event; vector stores) and those with a GJK hit or contacts before are
-// listed for k_manifold, which writes them again. A pair with neither stays empty and
+// Where the cull read the planar arrays: every pair, four consecutive ones per lane,
+// written empty (no contacts, no slot, no event; vector stores), and those with a GJK
+// hit or contacts before listed for k_manifold, which writes them again. A pair with neither stays empty and
 // no body state is read for it. A capped grid steps through the pairs, the same number
 // of rounds on every lane.
 __global__ void __launch_bounds__(kBlock)
@@ -626,20 +670,22 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// The listed pairs' manifolds (collision_phases.h:197-290)
+// The listed pairs' manifolds (collision_phases.h:197-290): live_list (the culled pairs
+// that had contacts, or every pair with a hit or contacts before), then gjk_list (the
+// pairs GJK ran on; null when live_list holds them)
 __global__ void __launch_bounds__(kBlock)
-    k_manifold(const uint64_t *keys, const uint32_t *src, const uint32_t *live_list, bodies_dev b, gjk_out_dev g,
-               manifolds_dev prev, manifolds_dev next, uint8_t *events, int epa_guard,
+    k_manifold(const uint64_t *keys, const uint32_t *src, const uint32_t *live_list, const uint32_t *gjk_list,
+               bodies_dev b, gjk_out_dev g, manifolds_dev prev, manifolds_dev next, uint8_t *events, int epa_guard,
                unsigned long long *counters)
 {
-    const uint64_t n_live = counters[C_MLIVE];
+    const uint64_t n1 = counters[C_MLIVE], n_live = n1 + (gjk_list ? counters[C_GJK_LIST] : 0ull);
     // the same number of rounds on every lane (each round's append and sum are workgroup-wide)
     const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t base = 0; base < n_live; base += threads)
     {
         const uint64_t i = base + static_cast<uint64_t>(xcd_block(n_live, blockDim.x)) * blockDim.x + threadIdx.x;
         const bool live = i < n_live;
-        const uint32_t p = live ? live_list[i] : 0u;
+        const uint32_t p = !live ? 0u : (i < n1 ? live_list[i] : gjk_list[i - n1]);
         uint32_t count = 0;
         mani nm;
         nm.n = 0;
@@ -852,23 +898,21 @@ void launch_gjk_epa_batch_dev(const shape_pair_dev &sp, uint32_t n, uint32_t n_v
 }
 
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,
-                hipStream_t st)
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, const cull_manifold_out &mo,
+                unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    // per-body records pay off with several pairs per body (hull clusters: 4; the
-    // batched worlds, 0.6, read the planar arrays faster)
-    if (static_cast<size_t>(n_pairs) >= 2 * static_cast<size_t>(b.n))
+    if (cull_uses_records(n_pairs, b.n))
     {
         // the mixed form adds the FP32 sphere records after the FP64 ones (16-byte aligned)
         float4 *rec32 = f32.s32 ? reinterpret_cast<float4 *>(cull_rec + static_cast<size_t>(kCullRec) * b.n) : nullptr;
         k_cull_prep<<<grid(b.n), kBlock, 0, st>>>(b, s, cull_rec, rec32);
         k_gjk_cull<true><<<list_grid((n_pairs + 3) / 4), kBlock, 0, st>>>(
-            keys, n_pairs, b, s, cull_rec, rec32, g.status, list, counters);
+            keys, n_pairs, b, s, cull_rec, rec32, g.status, list, mo, counters);
     }
     else
         k_gjk_cull<false><<<list_grid(n_pairs), kBlock, 0, st>>>(
-            keys, n_pairs, b, s, cull_rec, nullptr, g.status, list, counters);
+            keys, n_pairs, b, s, cull_rec, nullptr, g.status, list, mo, counters);
     int slot = C_GJK_LIST;
     if (f32.s32)
     {
@@ -944,19 +988,25 @@ void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_o
                                                         static_cast<epa_store_big *>(pool), counters);
 }
 
-void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t live_bound,
-                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, bool epa_guard, uint32_t *live_list,
-                            unsigned long long *counters, hipStream_t st)
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t bound,
+                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev, manifolds_dev next,
+                            uint8_t *events, bool epa_guard, bool culled, uint32_t *live_list,
+                            const uint32_t *gjk_list, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;
-    k_manifold_live<<<list_grid((n_pairs + 3) / 4), kBlock, 0, st>>>(src, n_pairs, g.status,
-                                                                                      prev.count, next, events,
-                                                                                      live_list, counters);
-    const uint32_t bound = std::min(live_bound, n_pairs);
+    if (!culled)
+        k_manifold_live<<<list_grid((n_pairs + 3) / 4), kBlock, 0, st>>>(src, n_pairs, g.status, prev.count, next,
+                                                                         events, live_list, counters);
     if (bound)
-        k_manifold<<<list_grid(bound), kBlock, 0, st>>>(
-            keys, src, live_list, b, g, prev, next, events, epa_guard ? 1 : 0, counters);
+        k_manifold<<<list_grid(bound), kBlock, 0, st>>>(keys, src, live_list, culled ? gjk_list : nullptr, b, g,
+                                                        prev, next, events, epa_guard ? 1 : 0, counters);
+}
+
+bool cull_uses_records(uint32_t n_pairs, uint32_t n_bodies)
+{
+    // per-body records pay off with several pairs per body (hull clusters: 4; the
+    // batched worlds, 0.6, read the planar arrays faster)
+    return static_cast<size_t>(n_pairs) >= 2 * static_cast<size_t>(n_bodies);
 }
 
 void launch_gjk_epa_batch(const shape_pair_dev &sp, uint32_t n, double *out, int32_t *hit,

@@ -95,7 +95,7 @@ enum counter_slot
     C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
     C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
     C_MSLOTS = 19,    // manifold data slots taken by the manifold update
-    C_MLIVE = 20,     // pairs the manifold update lists (a GJK hit or contacts before)
+    C_MLIVE = 20,     // culled pairs that had contacts, listed by the cull for the manifold update
     C_MLEAF = 21,     // leaves listed for the pair query (dynamic and moved)
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions
@@ -270,9 +270,24 @@ struct gjk32_in
     const int32_t *prev_count;
     uint32_t *list2;
 };
+// What the cull also settles of the manifold update: a pair whose bodies are apart has
+// no new contact, so its next manifold is empty unless it had contacts (prev_count of
+// its source, source_of(src, p), above 0). The cull writes every pair's next count,
+// slot and event empty, and lists the culled pairs that had contacts in live_list
+// (count at counters[C_MLIVE], at most live_cap: the previous step's non-empty
+// manifolds); the manifold update then runs over live_list and the GJK list.
+struct cull_manifold_out
+{
+    const uint32_t *src;
+    const int32_t *prev_count;
+    manifolds_dev next;
+    uint8_t *events;
+    uint32_t *live_list;
+    uint32_t live_cap;
+};
 void launch_gjk(const uint64_t *keys, uint32_t n_pairs, const bodies_dev &b, const shapes_dev &s, bool uv, int group,
-                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, unsigned long long *counters,
-                hipStream_t st);
+                gjk_out_dev &g, uint32_t *list, double *cull_rec, const gjk32_in &f32, const cull_manifold_out &mo,
+                unsigned long long *counters, hipStream_t st);
 // Launch shapes the autotuner picks (pkg_autotune); none changes a result.
 struct tuning
 {
@@ -296,12 +311,19 @@ bool epa_first_tier_is_lds(uint32_t n_hits, const tuning &tu);
 void launch_epa_big(const uint64_t *keys, const uint32_t *ovf_list, uint32_t n_ovf,
                     const bodies_dev &b, const shapes_dev &s, gjk_out_dev &g, void *pool,
                     unsigned long long *counters, hipStream_t st);
-// live_list holds n_pairs entries; live_bound bounds the pairs that have a GJK hit or
-// had contacts (it sizes the second kernel's grid; the kernel reads the true count)
-void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t live_bound,
-                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev,
-                            manifolds_dev next, uint8_t *events, bool epa_guard, uint32_t *live_list,
-                            unsigned long long *counters, hipStream_t st);
+// whether the cull of n_pairs pairs reads per-body records, and with them settles the
+// culled pairs' manifolds (cull_manifold_out)
+bool cull_uses_records(uint32_t n_pairs, uint32_t n_bodies);
+// culled (the cull settled the manifolds): the manifolds of the pairs the cull listed
+// (live_list, counters[C_MLIVE] entries) and of the pairs GJK ran on (gjk_list,
+// counters[C_GJK_LIST] entries); every other pair's manifold the cull wrote empty.
+// Otherwise a first pass writes every pair empty and lists those with a GJK hit or
+// contacts before in live_list (n_pairs entries), and the update runs over that list.
+// bound (at least the listed count) sizes the grid; the kernels read the counts.
+void launch_manifold_update(const uint64_t *keys, const uint32_t *src, uint32_t n_pairs, uint32_t bound,
+                            const bodies_dev &b, const gjk_out_dev &g, manifolds_dev prev, manifolds_dev next,
+                            uint8_t *events, bool epa_guard, bool culled, uint32_t *live_list,
+                            const uint32_t *gjk_list, unsigned long long *counters, hipStream_t st);
 
 // standalone narrowphase over shape pairs (pkg_gjk_epa_batch)
 struct shape_pair_dev
