@@ -252,6 +252,7 @@ struct pkg_ctx
     dbuf<uint32_t> clist;                 // colouring: the row manifolds
     uint32_t n_colours = 0;
     uint32_t n_listed = 0; // manifolds with rows in clist after the last step
+    uint32_t colour_rounds = 4; // rounds after which the last step's colouring was seen done
 
     // joints, dense in key order; an add or remove rebuilds the records into the other set
     std::vector<uint32_t> jkey_h;         // live keys
@@ -945,19 +946,29 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
                 used.reserve(std::max<uint32_t>(n, 1));
                 launch_fill_u64(used.p, 0ull, n, st);
                 launch_fill_u64(claim.p, 0ull, n, st);
-                // the host looks at the remaining count every kColourCheck rounds (with the
-                // colour count in the same copy); rounds after the last manifold is
-                // coloured change nothing
+                // the host looks at the remaining count (with the colour count in the same
+                // copy) first after as many rounds as the last step took, then every 2:
+                // consecutive steps colour alike, so one wait usually does. Rounds after the
+                // last manifold is coloured change nothing (and cost two short launches)
+                const uint32_t first = std::max<uint32_t>(colour_rounds, 1);
+                uint32_t check = first;
                 for (uint32_t round = 0;; ++round)
                 {
                     launch_colour_claim(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, round, st);
                     reset_counter(C_REMAIN);
                     launch_colour_assign(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, used.p, round,
                                          counters.p, st);
-                    if (round % kColourCheck == kColourCheck - 1)
+                    if (round + 1 == check)
                     {
                         read_counters(C_REMAIN, C_NCOL);
-                        if (h_counters[C_REMAIN] == 0) break;
+                        if (h_counters[C_REMAIN] == 0)
+                        {
+                            // done at the first look: try one round fewer next time, so a
+                            // step that needed many rounds does not hold later steps back
+                            colour_rounds = check == first ? std::max<uint32_t>(check - 1, 1) : check;
+                            break;
+                        }
+                        check += 2;
                     }
                     if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
