@@ -95,7 +95,7 @@ enum counter_slot
     C_GJK_LIST = 17, // pairs left for GJK after the bounding-sphere cull
     C_GJK_LIST2 = 18, // of those, pairs the FP32 certified-separation pass left to the FP64 GJK
     C_MSLOTS = 19,    // manifold data slots taken by the manifold update
-    C_MLIVE = 20,     // culled pairs that had contacts, listed by the cull for the manifold update
+    C_MLIVE = 20,     // pairs listed for the manifold update (see launch_manifold_update)
     C_MLEAF = 21,     // leaves listed for the pair query (dynamic and moved)
     // pair query cursors: one per output region, each in its own 256-byte line, so
     // wavefronts reserving output space spread their atomics over kPairRegions

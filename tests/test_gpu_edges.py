@@ -1,6 +1,7 @@
 """Edge inputs of the world step, bit for bit against the oracle: a world with no
-bodies, batched partitions with empty and one-body worlds, and bodies so fast that
-pairs appear and vanish every step and leaves travel far between full LBVH builds."""
+bodies, batched partitions with empty and one-body worlds, bodies so fast that pairs
+appear and vanish every step and leaves travel far between full LBVH builds, and
+drifting hull clusters whose contacts break while the pair list is kept whole."""
 import numpy as np
 import pytest
 
@@ -98,4 +99,34 @@ def test_fast_bodies_bitexact(pk, oracle):
         hits += c["gjk_hits"]
     assert moved > 60 * 100 and hits > 0
     assert np.isfinite(g.bodies()["pos"]).all()
+    g.close()
+
+
+def test_drifting_hulls_record_cull_bitexact(pk, oracle):
+    """Hull clusters (several pairs per body, so the cull reads per-body records and
+    settles the culled pairs' manifolds) drifting apart without gravity: contacts form,
+    then break as pairs get culled, and some steps move no leaf (the pair list is kept
+    whole, with no source map)."""
+    s = dict(pk.scene(pk.SCENE_HULLS, 5, 1024))
+    rng = np.random.default_rng(11)
+    s["vel"] = rng.normal(scale=1.0, size=s["vel"].shape)
+    s["angvel"] = rng.normal(scale=1.0, size=s["angvel"].shape)
+    flags = pk.FLAG_EPA_GUARD
+    o = oracle.OracleWorld(gravity=(0, 0, 0), iterations=s["iterations"], solver_mode=1, flags=flags)
+    o.load_scene(s)
+    g = pk.GpuWorld(gravity=(0, 0, 0), iterations=s["iterations"], flags=flags)
+    g.load_scene(s)
+    n = len(s["shape"])
+    print(f"[edges] drifting hulls: {n}", flush=True)
+    still = record = hits = ends = 0
+    for k in range(40):
+        g.step(s["dt"])
+        o.step(s["dt"])
+        _compare_world(k, g, o)
+        c = o.counters()
+        still += c["moved"] == 0 and c["pairs"] > 0
+        record += c["pairs"] >= 2 * n
+        hits += c["gjk_hits"]
+        ends += int(np.sum(g.events()[1] == 2))
+    assert still >= 1 and record >= 30 and hits > 0 and ends > 0
     g.close()
