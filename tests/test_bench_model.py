@@ -61,3 +61,51 @@ def test_gjk_stage_bytes_closed_form():
 
 def test_ray_bytes_closed_form():
     assert bench.ray_algorithmic_bytes(10, 4) == 680 + 96 + 228
+
+
+def _counters(**kw):
+    c = {k: 0 for k in ("moved", "candidates", "pairs", "gjk_hits", "contacts", "rows", "gjk_listed")}
+    c.update(kw)
+    return c
+
+
+def test_stage_bytes_integrate_fusion_and_kept_list():
+    """Four dynamic boxes and one static, 10 steps in one call: the first step runs
+    k_integrate_vel (2n + 412 N), the other nine k_integrate_pos_vel (2n + 612 N), and
+    only the last k_integrate_pos (n + 448 N); a step with no moved leaf and no
+    candidate prices the pair update at n (the list is kept as it is)."""
+    b = _box(0.5)
+    s = _scene([b], [0] * 5, [[3.0 * i, 0, 0] for i in range(5)])
+    s["n_static"] = 1
+    n, nd = 5, 4
+    c = _counters(pairs=3)
+    pairs = np.zeros(0, np.uint64)
+    out = bench.stage_bytes(s, c, pairs, 0.0, s["pos"], 10, False, steps=10)
+    assert out["integrate_vel"][1] == ((2 * n + 412 * nd) + 9 * (2 * n + 612 * nd)) / 10 + 48 * 1
+    assert out["integrate_pos"][1] == (n + 448 * nd) / 10
+    assert out["pair_update"][1] == n
+    one = bench.stage_bytes(s, c, pairs, 0.0, s["pos"], 10, False, steps=1)
+    assert one["integrate_vel"][1] == 2 * n + 412 * nd + 48 and one["integrate_pos"][1] == n + 448 * nd
+    moved = bench.stage_bytes(s, _counters(pairs=3, moved=2, candidates=1), pairs, 0.0, s["pos"], 10, False)
+    assert moved["pair_update"][1] == 48 * 2 + 80 * n + 36 * 1 + 20 * 3
+    off = bench.stage_bytes(s, c, pairs, 0.0, s["pos"], 10, False, integrate=False)
+    assert off["integrate_vel"][1] == n and off["integrate_pos"][1] == 0
+
+
+def test_stage_bytes_manifold_forms():
+    """Below 2 pairs per body the manifold update runs its own pass over every pair
+    (22 P); from 2 pairs per body the cull settles the culled pairs, so the update
+    prices only the L pairs GJK ran on (26 L) and the GJK stage carries 17 P."""
+    b = _box(0.5)
+    s = _scene([b], [0] * 5, [[10.0 * i, 0, 0] for i in range(5)])
+    s["n_static"] = 0
+    few = np.array([(0 << 32) | 1], np.uint64)
+    c = _counters(pairs=1, gjk_listed=1, contacts=2, gjk_hits=1)
+    out = bench.stage_bytes(s, c, few, 1.0, s["pos"], 10, False)
+    assert out["manifold"][1] == 22 * 1 + 208 * 2 + 192 * 1
+    assert out["gjk"][1] == bench.algorithmic_bytes("gjk", s, few, 1.0, s["pos"])
+    every = np.array([(a << 32) | c_ for a in range(5) for c_ in range(a + 1, 5)], np.uint64)  # 10 = 2 per body
+    c10 = _counters(pairs=10, gjk_listed=3, contacts=2, gjk_hits=1)
+    out10 = bench.stage_bytes(s, c10, every, 0.1, s["pos"], 10, False)
+    assert out10["manifold"][1] == 26 * 3 + 208 * 2 + 192 * 1
+    assert out10["gjk"][1] == bench.algorithmic_bytes("gjk", s, every, 0.1, s["pos"]) + 17 * 10
