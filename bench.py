@@ -162,17 +162,19 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, i
                         read and the new one with its sources written; n when no leaf
                         moved (the moved flags read; the list is kept as it is, each pair
                         its own source)
-      gjk               algorithmic_bytes(); with per-body cull records (P >= 2n) + 17 P:
-                        the cull also reads each pair's source and previous contact
-                        count and writes its next count, slot and event (empty; the
-                        manifold update rewrites the listed pairs)
+      gjk               algorithmic_bytes(); with per-body cull records (P >= 2n) + 6 P:
+                        the cull also reads each pair's source (4) and previous contact
+                        count (1) and writes its next count (1) and event (1), empty
+                        (the manifold update rewrites the listed pairs), and a culled
+                        pair writes no status byte (-1; the listed pairs' share of P is
+                        a fraction of a percent and is not subtracted)
       epa               algorithmic_bytes()
-      manifold          with per-body cull records, 26 L + 208 C + 192 H over the L pairs
+      manifold          with per-body cull records, 28 L + 208 C + 192 H over the L pairs
                         GJK ran on (and the few culled pairs that had contacts): key,
-                        source, status, counts, slot and event; contacts read and
-                        written; hits' contact data and poses. Otherwise 22 P + 208 C +
-                        192 H: every pair's source, status, previous and new counts, slot
-                        and event, and the list of pairs with contacts or a hit
+                        source, status, previous count and slot, new count, slot and
+                        event; contacts read and written; hits' contact data and poses.
+                        Otherwise 8 P + 28 L + 208 C + 192 H: every pair's source, status,
+                        previous and new count and event, then the listed pairs as above
       solver            496 R: contacts read, both bodies' pose, velocity and inverse
                         inertia read for the rows, impulses and velocities written
       solve_iterations  R (38 s_r + 96): every row read once and its bodies' velocities
@@ -195,10 +197,10 @@ def stage_bytes(s, c, pairs, hits_frac, pos, iterations, mixed, tree_refits=0, i
         "pair_update": ("pair-set update (k_pair_query, sort, k_match, k_prev_*, k_merge)",
                         n if M == 0 and K == 0 else 48 * M + 80 * n + 36 * K + 20 * P),
         "gjk": ("GJK stage (k_gjk_cull" + (", k_gjk32" if mixed else "") + ", k_gjk)",
-                (algorithmic_bytes("gjk", s, pairs, hits_frac, pos) if len(pairs) else 0.0) + (17 * P if fused else 0)),
+                (algorithmic_bytes("gjk", s, pairs, hits_frac, pos) if len(pairs) else 0.0) + (6 * P if fused else 0)),
         "epa": ("k_epa (+ larger-store reruns)", algorithmic_bytes("epa", s, pairs, hits_frac) if len(pairs) else 0.0),
         "manifold": ("k_manifold" if fused else "k_manifold_live, k_manifold",
-                     (26 * c["gjk_listed"] if fused else 22 * P) + 208 * C + 192 * H),
+                     (0 if fused else 8 * P) + 28 * c["gjk_listed"] + 208 * C + 192 * H),
         "solver": ("contact solver stage", 496 * R),
         "solve_iterations": ("k_solve_islands" + ("<true> (FP32 rows)" if mixed else "<false>"), R * (38 * sr + 96)),
         "integrate_pos": ("k_integrate_pos", (n + 448 * nd) / steps),

@@ -210,7 +210,7 @@ struct pkg_ctx
     // pair list and manifold cache, double-buffered (cur = last step's result)
     dbuf<uint64_t> keys[2];
     dbuf<uint32_t> src;
-    dbuf<int32_t> mcount[2];
+    dbuf<uint8_t> mcount[2];
     dbuf<uint32_t> mslot[2];
     dbuf<double> mdata[2];
     dbuf<uint32_t> mowner[2]; // the pair of each data slot
@@ -1934,7 +1934,7 @@ pkg_status pkg_get_manifolds(pkg_ctx *ctx, int32_t *counts, double *data, size_t
         size_t m = std::min<size_t>(cap, P);
         if (m && (counts || data))
         {
-            std::vector<int32_t> cnt = download(ctx->mcount[c], P, ctx->st);
+            std::vector<uint8_t> cnt = download(ctx->mcount[c], P, ctx->st);
             std::vector<uint32_t> sl;
             std::vector<double> raw;
             const size_t dcap = ctx->mdcap[c];

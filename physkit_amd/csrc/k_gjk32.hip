@@ -67,7 +67,7 @@ template <bool UV> __device__ int certify_apart(const bodies_dev &b, const shape
 template <bool UV>
 __global__ void __launch_bounds__(kBlock)
     k_gjk32(const uint64_t *keys, const uint32_t *list, bodies_dev b, shapes32_dev s, const uint32_t *src,
-            const int32_t *prev_count, uint8_t *status, uint32_t *list2, unsigned long long *counters)
+            const uint8_t *prev_count, uint8_t *status, uint32_t *list2, unsigned long long *counters)
 {
     const uint32_t n_list = static_cast<uint32_t>(counters[C_GJK_LIST]);
     const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -117,7 +117,7 @@ void launch_verts32(const shapes_dev &s, float4 *out, hipStream_t st)
 }
 
 void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, const bodies_dev &b,
-                  const shapes32_dev &s, bool uv, const uint32_t *src, const int32_t *prev_count, uint8_t *status,
+                  const shapes32_dev &s, bool uv, const uint32_t *src, const uint8_t *prev_count, uint8_t *status,
                   uint32_t *list2, unsigned long long *counters, hipStream_t st)
 {
     if (!n_pairs) return;

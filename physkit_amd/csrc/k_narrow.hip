@@ -239,17 +239,18 @@ __device__ __forceinline__ void load_simplex(const double *d, uint32_t cap, uint
     if (x.n > 3) x.s3 = ld_spt(d, cap, slot, 3);
 }
 
-// A lane takes Q consecutive pairs. All are written separated (status 0); the pairs
-// whose bodies are not apart (above) are listed for k_gjk, which writes their status
-// again. With per-body records (many pairs per body, most culled by the FP32 spheres)
-// Q is 4: one 32-byte key load, vector stores, and the sphere loads in flight together;
-// the cull then also writes every pair's next manifold empty (count 0, no slot, no
-// event) and lists the culled pairs that had contacts for the manifold update
-// (cull_manifold_out), which rewrites those and the GJK list's manifolds. From the
-// planar arrays (few pairs per body, each test reading several arrays) Q is 1 and the
-// manifold update lists its own pairs (k_manifold_live): there a source and count
-// gather in the cull would lengthen each lane's chain of dependent loads. A capped grid
-// steps through the pairs, the same number of rounds on every lane.
+// A lane takes Q consecutive pairs; the pairs whose bodies are not apart (above) are
+// listed for k_gjk, which writes their status. With per-body records (many pairs per
+// body, most culled by the FP32 spheres) Q is 4: one 32-byte key load, vector stores,
+// and the sphere loads in flight together; the cull then also writes every pair's
+// next manifold empty (count 0, no event; an empty manifold's slot is never read) and
+// lists the culled pairs that had contacts for the manifold update (cull_manifold_out),
+// which rewrites those and the GJK list's manifolds. Nothing reads a culled pair's
+// status there, so none is written. From the planar arrays (few pairs per body, each
+// test reading several arrays) Q is 1, culled pairs are written separated (status 0)
+// and the manifold update lists its own pairs (k_manifold_live): there a source and
+// count gather in the cull would lengthen each lane's chain of dependent loads. A
+// capped grid steps through the pairs, the same number of rounds on every lane.
 template <bool REC>
 __global__ void __launch_bounds__(kBlock)
     k_gjk_cull(const uint64_t *keys, uint32_t n_pairs, bodies_dev b, shapes_dev s, const double *rec,
@@ -289,15 +290,11 @@ __global__ void __launch_bounds__(kBlock)
                     else
                     {
                         // the previous list kept whole: the previous counts at the same indices
-                        const int4 c4 = *reinterpret_cast<const int4 *>(mo.prev_count + p0);
-                        pc[0] = c4.x;
-                        pc[1] = c4.y;
-                        pc[2] = c4.z;
-                        pc[3] = c4.w;
+                        const uint32_t c4 = *reinterpret_cast<const uint32_t *>(mo.prev_count + p0);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) pc[k] = static_cast<int>((c4 >> (8 * k)) & 0xFFu);
                     }
-                    *reinterpret_cast<uint32_t *>(status + p0) = 0u;
-                    *reinterpret_cast<int4 *>(mo.next.count + p0) = make_int4(0, 0, 0, 0);
-                    *reinterpret_cast<uint4 *>(mo.next.slot + p0) = make_uint4(kNone, kNone, kNone, kNone);
+                    *reinterpret_cast<uint32_t *>(mo.next.count + p0) = 0u;
                     *reinterpret_cast<uint32_t *>(mo.events + p0) = 0u;
                 }
             }
@@ -306,15 +303,15 @@ __global__ void __launch_bounds__(kBlock)
                 {
                     const uint32_t p = p0 + k;
                     key[k] = keys[p];
-                    status[p] = 0;
                     if constexpr (REC)
                     {
                         const uint32_t sp = source_of(mo.src, p);
                         pc[k] = sp != kNone ? mo.prev_count[sp] : 0;
                         mo.next.count[p] = 0;
-                        mo.next.slot[p] = kNone;
                         mo.events[p] = 0;
                     }
+                    else
+                        status[p] = 0;
                 }
             for (uint32_t k = 0; k < Q; ++k)
                 if (k < n_here && pc[k] > 0) prevmask |= 1u << k;
@@ -605,12 +602,12 @@ __device__ __forceinline__ void add_contact(mani &m, const contact &k) // :124-1
 }
 
 // Where the cull read the planar arrays: every pair, four consecutive ones per lane,
-// written empty (no contacts, no slot, no event; vector stores), and those with a GJK
-// hit or contacts before listed for k_manifold, which writes them again. A pair with neither stays empty and
+// written empty (no contacts, no event; vector stores), and those with a GJK hit or
+// contacts before listed for k_manifold, which writes them again. A pair with neither stays empty and
 // no body state is read for it. A capped grid steps through the pairs, the same number
 // of rounds on every lane.
 __global__ void __launch_bounds__(kBlock)
-    k_manifold_live(const uint32_t *src, uint32_t n_pairs, const uint8_t *status, const int32_t *prev_count,
+    k_manifold_live(const uint32_t *src, uint32_t n_pairs, const uint8_t *status, const uint8_t *prev_count,
                     manifolds_dev next, uint8_t *events, uint32_t *live_list, unsigned long long *counters)
 {
     const uint64_t n_quads = (static_cast<uint64_t>(n_pairs) + 3) / 4;
@@ -637,17 +634,14 @@ __global__ void __launch_bounds__(kBlock)
                 else
                 {
                     // the previous list kept whole: the previous counts at the same indices
-                    const int4 c4 = *reinterpret_cast<const int4 *>(prev_count + p0);
-                    pc[0] = c4.x;
-                    pc[1] = c4.y;
-                    pc[2] = c4.z;
-                    pc[3] = c4.w;
+                    const uint32_t c4 = *reinterpret_cast<const uint32_t *>(prev_count + p0);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pc[k] = static_cast<int>((c4 >> (8 * k)) & 0xFFu);
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     if (((st4 >> (8 * k)) & 0xFFu) == 1u || pc[k] > 0) livemask |= 1u << k;
-                *reinterpret_cast<int4 *>(next.count + p0) = make_int4(0, 0, 0, 0);
-                *reinterpret_cast<uint4 *>(next.slot + p0) = make_uint4(kNone, kNone, kNone, kNone);
+                *reinterpret_cast<uint32_t *>(next.count + p0) = 0u;
                 *reinterpret_cast<uint32_t *>(events + p0) = 0u;
             }
             else
@@ -656,7 +650,6 @@ __global__ void __launch_bounds__(kBlock)
                     const uint32_t p = p0 + k, sp = source_of(src, p);
                     if (status[p] == 1 || (sp != kNone && prev_count[sp] > 0)) livemask |= 1u << k;
                     next.count[p] = 0;
-                    next.slot[p] = kNone;
                     events[p] = 0;
                 }
         }
@@ -698,7 +691,8 @@ __global__ void __launch_bounds__(kBlock)
             uint32_t ia = static_cast<uint32_t>(key >> 32), ib = static_cast<uint32_t>(key & 0xFFFFFFFFu);
             v3 pa = ld3(b.pos, b.n, ia), pb = ld3(b.pos, b.n, ib);
             qt qa = ldq(b.quat, b.n, ia), qb = ldq(b.quat, b.n, ib);
-            bool has_new = g.status[p] == 1;
+            // the cull's own entries (first n1 with a GJK list) were culled: no new contact
+            bool has_new = !(gjk_list && i < n1) && g.status[p] == 1;
             contact nc{};
             if (has_new)
             {
@@ -751,7 +745,7 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
             if (has_new) add_contact(nm, nc);
-            next.count[p] = nm.n;
+            next.count[p] = static_cast<uint8_t>(nm.n);
             bool was = oldn > 0, is = nm.n > 0;
             events[p] = (!was && is) ? 1 : ((was && !is) ? 2 : 0);
             count = static_cast<uint32_t>(nm.n);

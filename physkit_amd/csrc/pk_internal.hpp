@@ -56,14 +56,15 @@ struct shapes_dev
     const uint32_t *uv;  // n: UV-sphere tag ((stacks << 16) | sectors) or 0
 };
 
-// manifold cache: a contact count and a data slot per pair (kNone when empty); up to
-// 4 contacts x 13 fields per slot, planar with stride dcap: field f of contact c of the
+// manifold cache: a contact count (0-4, one byte) and a data slot per pair; up to 4
+// contacts x 13 fields per slot, planar with stride dcap: field f of contact c of the
 // manifold in slot s at data[(c * 13 + f) * dcap + s]. Only pairs with contacts hold a
-// slot, so the data scales with the contacts, not with the broadphase pairs.
+// slot, so the data scales with the contacts, not with the broadphase pairs. A pair's
+// slot is read only where its count is above 0 (an empty manifold's slot is not written).
 struct manifolds_dev
 {
     uint32_t cap;  // pairs
-    int32_t *count;
+    uint8_t *count;
     uint32_t *slot;
     uint32_t dcap; // slots
     double *data;
@@ -244,7 +245,7 @@ void launch_verts32(const shapes_dev &s, float4 *out, hipStream_t st);
 // rest, and every pair with contacts after the previous step (prev_count of its source,
 // source_of(src, p), above 0), are listed in list2, count in counters[C_GJK_LIST2].
 void launch_gjk32(const uint64_t *keys, const uint32_t *list, uint32_t n_pairs, const bodies_dev &b,
-                  const shapes32_dev &s, bool uv, const uint32_t *src, const int32_t *prev_count, uint8_t *status,
+                  const shapes32_dev &s, bool uv, const uint32_t *src, const uint8_t *prev_count, uint8_t *status,
                   uint32_t *list2, unsigned long long *counters, hipStream_t st);
 // GJK over every pair with `group` (2 or 4) lanes per pair; pairs whose bounding
 // spheres are clearly apart are written separated first and the rest listed in `list`
@@ -267,7 +268,7 @@ struct gjk32_in
 {
     const shapes32_dev *s32; // null: no FP32 pass
     const uint32_t *src;
-    const int32_t *prev_count;
+    const uint8_t *prev_count;
     uint32_t *list2;
 };
 // What the cull also settles of the manifold update: a pair whose bodies are apart has
@@ -279,7 +280,7 @@ struct gjk32_in
 struct cull_manifold_out
 {
     const uint32_t *src;
-    const int32_t *prev_count;
+    const uint8_t *prev_count;
     manifolds_dev next;
     uint8_t *events;
     uint32_t *live_list;

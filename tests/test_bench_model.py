@@ -94,18 +94,19 @@ def test_stage_bytes_integrate_fusion_and_kept_list():
 
 def test_stage_bytes_manifold_forms():
     """Below 2 pairs per body the manifold update runs its own pass over every pair
-    (22 P); from 2 pairs per body the cull settles the culled pairs, so the update
-    prices only the L pairs GJK ran on (26 L) and the GJK stage carries 17 P."""
+    (8 P) before the L listed pairs (28 L); from 2 pairs per body the cull settles the
+    culled pairs, so the update prices only the listed pairs and the GJK stage carries
+    6 P."""
     b = _box(0.5)
     s = _scene([b], [0] * 5, [[10.0 * i, 0, 0] for i in range(5)])
     s["n_static"] = 0
     few = np.array([(0 << 32) | 1], np.uint64)
     c = _counters(pairs=1, gjk_listed=1, contacts=2, gjk_hits=1)
     out = bench.stage_bytes(s, c, few, 1.0, s["pos"], 10, False)
-    assert out["manifold"][1] == 22 * 1 + 208 * 2 + 192 * 1
+    assert out["manifold"][1] == 8 * 1 + 28 * 1 + 208 * 2 + 192 * 1
     assert out["gjk"][1] == bench.algorithmic_bytes("gjk", s, few, 1.0, s["pos"])
     every = np.array([(a << 32) | c_ for a in range(5) for c_ in range(a + 1, 5)], np.uint64)  # 10 = 2 per body
     c10 = _counters(pairs=10, gjk_listed=3, contacts=2, gjk_hits=1)
     out10 = bench.stage_bytes(s, c10, every, 0.1, s["pos"], 10, False)
-    assert out10["manifold"][1] == 26 * 3 + 208 * 2 + 192 * 1
-    assert out10["gjk"][1] == bench.algorithmic_bytes("gjk", s, every, 0.1, s["pos"]) + 17 * 10
+    assert out10["manifold"][1] == 28 * 3 + 208 * 2 + 192 * 1
+    assert out10["gjk"][1] == bench.algorithmic_bytes("gjk", s, every, 0.1, s["pos"]) + 6 * 10
