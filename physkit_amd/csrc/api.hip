@@ -510,10 +510,10 @@ void pkg_ctx::joint_prepare(const bodies_dev &b, double dt)
     reset_counter(C_NCOL);
     for (uint32_t round = 0;; ++round)
     {
-        launch_colour_claim(jd.keys, nullptr, J, b, jcol.p, nullptr, nullptr, claim.p, round, st);
+        launch_colour_claim(jd.keys, nullptr, J, nullptr, b, jcol.p, nullptr, nullptr, claim.p, round, st);
         reset_counter(C_REMAIN);
-        launch_colour_assign(jd.keys, nullptr, J, b, jcol.p, nullptr, nullptr, claim.p, used.p, round, counters.p,
-                             st);
+        launch_colour_assign(jd.keys, nullptr, J, nullptr, b, jcol.p, nullptr, nullptr, claim.p, used.p, round,
+                             counters.p, st);
         if (round % kColourCheck == kColourCheck - 1 && read_counter(C_REMAIN) == 0) break;
         if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "joint colouring did not terminate");
     }
@@ -928,13 +928,14 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
             colour.reserve(ncap);
             claim.reserve(std::max<uint32_t>(n, 1));
             clist.reserve(std::max<uint32_t>(mdcap[nxt], 1));
-            // the manifolds with rows are listed as they are flagged, so one copy returns
-            // the row and list counts (C_NCOL and C_NLIST are zero from the step's start)
+            // The manifolds with rows are listed as they are flagged (C_NLIST, from zero at
+            // the step's start), and the colouring rounds read that count on the device
+            // over a grid sized by the list's capacity: the row count, the list count and
+            // the colouring's state come back together at its first check.
             launch_row_flags(keys[nxt].p, Pn, b, MF(nxt), colour.p, clist.p, counters.p, st);
-            read_counters(C_ROWS, C_NLIST);
-            if (h_counters[C_ROWS] > 0)
+            const uint32_t lcap = static_cast<uint32_t>(std::min<size_t>(clist.cap, 0xFFFFFFFFu));
             {
-                nl = static_cast<uint32_t>(h_counters[C_NLIST]);
+                const unsigned long long *nl_dev = counters.p + C_NLIST;
                 const uint32_t *wo = nullptr, *pb = nullptr;
                 if (n_worlds > 1)
                 {
@@ -954,13 +955,14 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
                 uint32_t check = first;
                 for (uint32_t round = 0;; ++round)
                 {
-                    launch_colour_claim(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, round, st);
+                    launch_colour_claim(keys[nxt].p, clist.p, lcap, nl_dev, b, colour.p, wo, pb, claim.p, round, st);
                     reset_counter(C_REMAIN);
-                    launch_colour_assign(keys[nxt].p, clist.p, nl, b, colour.p, wo, pb, claim.p, used.p, round,
-                                         counters.p, st);
+                    launch_colour_assign(keys[nxt].p, clist.p, lcap, nl_dev, b, colour.p, wo, pb, claim.p, used.p,
+                                         round, counters.p, st);
                     if (round + 1 == check)
                     {
-                        read_counters(C_REMAIN, C_NCOL);
+                        read_counters(C_ROWS, C_NLIST); // rows, remaining, colours and the list count
+                        if (h_counters[C_ROWS] == 0) break; // nothing to colour (the rounds did nothing)
                         if (h_counters[C_REMAIN] == 0)
                         {
                             // done at the first look: try one round fewer next time, so a
@@ -972,7 +974,11 @@ void pkg_ctx::step_once(double dt, hipEvent_t *ev, bool defer_pos)
                     }
                     if (round > 1000000) throw pk_error(PKG_DEVICE_ERROR, "colouring did not terminate");
                 }
-                c = static_cast<uint32_t>(h_counters[C_NCOL]);
+                if (h_counters[C_ROWS] > 0)
+                {
+                    nl = static_cast<uint32_t>(h_counters[C_NLIST]);
+                    c = static_cast<uint32_t>(h_counters[C_NCOL]);
+                }
             }
         }
         n_colours = c;

@@ -386,13 +386,19 @@ __global__ void k_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32
     if (p == 0 || world_of[static_cast<uint32_t>(keys[p - 1] >> 32)] != w) pair_base[w] = p;
 }
 
-// the items coloured: list[0, n) (or 0..n-1 when list is null)
-__global__ void k_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, bodies_dev b, const uint32_t *colour,
-                        const uint32_t *world_of, const uint32_t *pair_base, unsigned long long *claim,
-                        uint32_t round)
+// the items coloured: list[0, n) (or 0..n-1 when list is null); n_dev, when given,
+// holds the list's count on the device (n then bounds it)
+__device__ __forceinline__ uint32_t colour_count(uint32_t n, const unsigned long long *n_dev)
+{
+    return n_dev ? static_cast<uint32_t>(min(static_cast<unsigned long long>(n), *n_dev)) : n;
+}
+
+__global__ void k_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const unsigned long long *n_dev,
+                        bodies_dev b, const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                        unsigned long long *claim, uint32_t round)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= colour_count(n, n_dev)) return;
     uint32_t p = list ? list[i] : i;
     if (colour[p] != kUncoloured) return;
     uint64_t key = keys[p];
@@ -407,10 +413,12 @@ __global__ void k_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, 
 // that no coloured manifold at those bodies holds (`used`: colours 0-63 taken at each
 // body), or 64 + the round number when all 64 are taken. Manifolds coloured in one
 // round share no dynamic body, so their choices are independent.
-__global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, bodies_dev b, uint32_t *colour,
-                         const uint32_t *world_of, const uint32_t *pair_base, const unsigned long long *claim,
-                         unsigned long long *used, uint32_t round, unsigned long long *counters)
+__global__ void k_assign(const uint64_t *keys, const uint32_t *list, uint32_t n_bound, const unsigned long long *n_dev,
+                         bodies_dev b, uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+                         const unsigned long long *claim, unsigned long long *used, uint32_t round,
+                         unsigned long long *counters)
 {
+    const uint32_t n = colour_count(n_bound, n_dev);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool remain = false;
     uint32_t ncol = 0;
@@ -1212,20 +1220,21 @@ void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *wo
 {
     if (n_pairs) k_pair_base<<<grid(n_pairs), kBlock, 0, st>>>(keys, n_pairs, world_of, pair_base);
 }
-void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
-                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                         unsigned long long *claim, uint32_t round, hipStream_t st)
+void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const unsigned long long *n_dev,
+                         const bodies_dev &b, const uint32_t *colour, const uint32_t *world_of,
+                         const uint32_t *pair_base, unsigned long long *claim, uint32_t round, hipStream_t st)
 {
-    if (n) k_claim<<<grid(n), kBlock, 0, st>>>(keys, list, n, b, colour, world_of, pair_base, claim, round);
+    if (n)
+        k_claim<<<grid(n), kBlock, 0, st>>>(keys, list, n, n_dev, b, colour, world_of, pair_base, claim, round);
 }
-void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
-                          uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const unsigned long long *n_dev,
+                          const bodies_dev &b, uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, unsigned long long *used, uint32_t round,
                           unsigned long long *counters, hipStream_t st)
 {
     if (n)
-        k_assign<<<grid(n), kBlock, 0, st>>>(keys, list, n, b, colour, world_of, pair_base, claim, used, round,
-                                             counters);
+        k_assign<<<grid(n), kBlock, 0, st>>>(keys, list, n, n_dev, b, colour, world_of, pair_base, claim, used,
+                                             round, counters);
 }
 uint32_t *launch_colour_bucket(const uint32_t *colour, const uint32_t *list, uint32_t nl, uint32_t n_pairs,
                                uint32_t n_colours, uint64_t *keys, uint64_t *keys_alt, uint32_t *order,

@@ -370,15 +370,16 @@ void launch_contact_setup(const uint64_t *keys, uint32_t n, const uint32_t *ord,
 // world_of and pair_base are null for a single world.
 void launch_pair_base(const uint64_t *keys, uint32_t n_pairs, const uint32_t *world_of, uint32_t *pair_base,
                       hipStream_t st);
-// Colouring rounds over a list of items (null: all n). The claim array is cleared
-// once before the first round; `used` (colours 0-63 taken at each body) too; each round
-// the claim holders take the smallest colour free at their dynamic bodies, and C_NCOL
-// ends as the colour count.
-void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
-                         const uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
-                         unsigned long long *claim, uint32_t round, hipStream_t st);
-void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const bodies_dev &b,
-                          uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
+// Colouring rounds over a list of items (null: all n). n_dev, when not null, holds the
+// list's count on the device and n only bounds it (the grid). The claim array is
+// cleared once before the first round; `used` (colours 0-63 taken at each body) too;
+// each round the claim holders take the smallest colour free at their dynamic bodies,
+// and C_NCOL ends as the colour count.
+void launch_colour_claim(const uint64_t *keys, const uint32_t *list, uint32_t n, const unsigned long long *n_dev,
+                         const bodies_dev &b, const uint32_t *colour, const uint32_t *world_of,
+                         const uint32_t *pair_base, unsigned long long *claim, uint32_t round, hipStream_t st);
+void launch_colour_assign(const uint64_t *keys, const uint32_t *list, uint32_t n, const unsigned long long *n_dev,
+                          const bodies_dev &b, uint32_t *colour, const uint32_t *world_of, const uint32_t *pair_base,
                           const unsigned long long *claim, unsigned long long *used, uint32_t round,
                           unsigned long long *counters, hipStream_t st);
 // the listed solver manifolds (list[0, nl), or 0..nl-1 when list is null; every one
