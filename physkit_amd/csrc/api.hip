@@ -425,7 +425,7 @@ struct pkg_ctx
 
 namespace
 {
-constexpr uint32_t kColourCheck = 4; // colouring rounds between host checks of the remaining count
+constexpr uint32_t kColourCheck = 4; // joint colouring: rounds between host checks of the remaining count
 
 void record(hipEvent_t *ev, int k, hipStream_t st)
 {
